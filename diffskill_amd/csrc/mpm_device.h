@@ -1,0 +1,726 @@
+// Device math of the MLS-MPM engine for gfx950: quaternions, the register
+// 3x3 SVD, tool SDFs and their adjoints, grid collision, primitive kinematics.
+//
+// Everything here is fp32 (the reference runs ti.f32, plb/config/default_config.py:15).
+// Adjoints follow Taichi 0.7.26 reverse-mode rules (the dependency that
+// generated the reference's .grad kernels): only the taken branch is
+// differentiated; max(a, b) gives its adjoint to a iff b < a, else to b;
+// min(a, b) gives it to a iff a < b, else to b; abs'(x) = sgn(x), sgn(0) = 0.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dsk {
+
+#define DEV __device__ __forceinline__
+
+// ------------------------------------------------------------------ tables
+struct SceneC {
+    int n_grid, nb, p_off, p_cnt;
+    int blk_off, prim_off, n_prims, gf_mode;   // gf_mode: 0 slip walls, 1 Coulomb ground, 2 sticky ground
+    int m_off, chunk_off, n_chunks, pad;
+    long long g_off;                            // first node (block-major layout)
+    float dx, inv_dx, dt, p_mass;
+    float mu, lam, ys, stress_k;
+    float c4, lower, upper, gf;
+    float g30[4];
+};
+
+struct PrimC {
+    int type, action_dim, unit_shape, n_units;
+    float friction, softness, h, r;
+    float size[4];
+    float ph[4];
+    float prot[4];
+    float lo[4];
+    float hi[4];
+    float min_gap, max_gap, pad0, pad1;
+    float scale[8];
+};
+
+enum { P_SPHERE = 0, P_CAPSULE = 1, P_ROLLINGPIN = 2, P_ROLLINGPIN_EXT = 3, P_BOX = 4, P_GRIPPER = 5,
+       P_PRISM = 6, P_KNIFE = 7, P_CHOPSTICKS = 8, P_GRIPPER2 = 9 };
+enum { SH_SPHERE = 0, SH_CAPSULE = 1, SH_BOX = 2, SH_PRISM = 3, SH_KNIFE = 4, SH_CHOP = 5 };
+
+constexpr float LEN_EPS_SHAPE = 1e-14f;  // primitives.py:14-15
+constexpr float LEN_EPS_BASE = 1e-8f;    // primitive/utils.py:5-6
+
+// ------------------------------------------------------------------ vec3
+struct v3 { float x, y, z; };
+DEV v3 mk(float a, float b, float c) { v3 r; r.x = a; r.y = b; r.z = c; return r; }
+DEV v3 operator+(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+DEV v3 operator-(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+DEV v3 operator-(v3 a) { return mk(-a.x, -a.y, -a.z); }
+DEV v3 operator*(float s, v3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+DEV v3 operator*(v3 a, float s) { return mk(s * a.x, s * a.y, s * a.z); }
+DEV v3& operator+=(v3& a, v3 b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
+DEV v3& operator-=(v3& a, v3 b) { a.x -= b.x; a.y -= b.y; a.z -= b.z; return a; }
+DEV float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+DEV v3 cross(v3 a, v3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+DEV float comp(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+DEV void setc(v3& a, int i, float v) { if (i == 0) a.x = v; else if (i == 1) a.y = v; else a.z = v; }
+DEV float tmax(float a, float b) { return b < a ? a : b; }
+DEV float tmin(float a, float b) { return a < b ? a : b; }
+DEV float sgnf(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+
+struct q4 { float w, x, y, z; };
+DEV q4 mkq(float w, float x, float y, float z) { q4 q; q.w = w; q.x = x; q.y = y; q.z = z; return q; }
+DEV v3 qv(q4 q) { return mk(q.x, q.y, q.z); }
+
+// qrot (plb/engine/primitive/utils.py:9-15)
+DEV v3 qrot(q4 q, v3 v) {
+    v3 u = qv(q);
+    v3 uv = cross(u, v);
+    v3 uuv = cross(u, uv);
+    return v + 2.f * (q.w * uv + uuv);
+}
+// adjoint of qrot: adds to gq (if non-null) and returns gv
+DEV v3 qrot_rev(q4 q, v3 v, v3 go, q4* gq) {
+    v3 u = qv(q);
+    v3 uv = cross(u, v);
+    v3 guuv = 2.f * go;
+    v3 guv = (2.f * q.w) * go;
+    v3 gu = cross(uv, guuv);
+    guv += cross(guuv, u);
+    gu += cross(v, guv);
+    if (gq) { gq->w += 2.f * dot(go, uv); gq->x += gu.x; gq->y += gu.y; gq->z += gu.z; }
+    return cross(guv, u) + go;
+}
+// conjugate, normalized without epsilon (Vector.normalized(), utils.py:52)
+DEV q4 conj_n(q4 q) {
+    float n = sqrtf(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+    return mkq(q.w / n, -q.x / n, -q.y / n, -q.z / n);
+}
+DEV void conj_n_rev(q4 q, q4 g, q4* gq) {
+    q4 c = mkq(q.w, -q.x, -q.y, -q.z);
+    float n = sqrtf(c.w * c.w + c.x * c.x + c.y * c.y + c.z * c.z);
+    float d = c.w * g.w + c.x * g.x + c.y * g.y + c.z * g.z;
+    float n3 = n * n * n;
+    gq->w += g.w / n - c.w * d / n3;
+    gq->x -= g.x / n - c.x * d / n3;
+    gq->y -= g.y / n - c.y * d / n3;
+    gq->z -= g.z / n - c.z * d / n3;
+}
+// qmul (utils.py:18-25): Hamilton product, renormalized
+DEV q4 qmul_raw(q4 q, q4 r) {
+    return mkq(r.w * q.w - r.x * q.x - r.y * q.y - r.z * q.z,
+               r.w * q.x + r.x * q.w - r.y * q.z + r.z * q.y,
+               r.w * q.y + r.x * q.z + r.y * q.w - r.z * q.x,
+               r.w * q.z - r.x * q.y + r.y * q.x + r.z * q.w);
+}
+DEV q4 qmul(q4 q, q4 r) {
+    q4 o = qmul_raw(q, r);
+    float n = sqrtf(o.w * o.w + o.x * o.x + o.y * o.y + o.z * o.z);
+    return mkq(o.w / n, o.x / n, o.y / n, o.z / n);
+}
+DEV void qmul_rev(q4 q, q4 r, q4 go, q4* gq, q4* gr) {
+    q4 o = qmul_raw(q, r);
+    float n = sqrtf(o.w * o.w + o.x * o.x + o.y * o.y + o.z * o.z);
+    float d = o.w * go.w + o.x * go.x + o.y * go.y + o.z * go.z;
+    float n3 = n * n * n;
+    q4 g = mkq(go.w / n - o.w * d / n3, go.x / n - o.x * d / n3, go.y / n - o.y * d / n3, go.z / n - o.z * d / n3);
+    gq->w += g.w * r.w + g.x * r.x + g.y * r.y + g.z * r.z;
+    gq->x += -g.w * r.x + g.x * r.w - g.y * r.z + g.z * r.y;
+    gq->y += -g.w * r.y + g.x * r.z + g.y * r.w - g.z * r.x;
+    gq->z += -g.w * r.z - g.x * r.y + g.y * r.x + g.z * r.w;
+    gr->w += g.w * q.w + g.x * q.x + g.y * q.y + g.z * q.z;
+    gr->x += -g.w * q.x + g.x * q.w + g.y * q.z - g.z * q.y;
+    gr->y += -g.w * q.y - g.x * q.z + g.y * q.w + g.z * q.x;
+    gr->z += -g.w * q.z + g.x * q.y - g.y * q.x + g.z * q.w;
+}
+// w2quat (utils.py:28-40)
+DEV q4 w2quat(v3 a) {
+    float w = sqrtf(dot(a, a) + 1e-16f);
+    q4 o = mkq(1.f, 0.f, 0.f, 0.f);
+    if (w > 1e-9f) {
+        float s = sinf(w / 2.f);
+        o = mkq(cosf(w / 2.f), (a.x / w) * s, (a.y / w) * s, (a.z / w) * s);
+    }
+    return o;
+}
+DEV v3 w2quat_rev(v3 a, q4 go) {
+    float w = sqrtf(dot(a, a) + 1e-16f);
+    if (!(w > 1e-9f)) return mk(0.f, 0.f, 0.f);
+    float s = sinf(w / 2.f), c = cosf(w / 2.f);
+    float gw = -s / 2.f * go.w;
+    v3 gv = qv(go);
+    gw += dot(gv, a) * (c / (2.f * w) - s / (w * w));
+    return (s / w) * gv + (gw / w) * a;
+}
+
+// ------------------------------------------------------------------ 3x3 matrices
+struct m3 { float a[9]; };
+DEV m3 mzero() { m3 r; for (int i = 0; i < 9; i++) r.a[i] = 0.f; return r; }
+DEV m3 mmul(const m3& A, const m3& B) {
+    m3 C;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) C.a[i * 3 + j] = A.a[i * 3] * B.a[j] + A.a[i * 3 + 1] * B.a[3 + j] + A.a[i * 3 + 2] * B.a[6 + j];
+    return C;
+}
+DEV m3 mmul_tn(const m3& A, const m3& B) {  // A^T B
+    m3 C;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) C.a[i * 3 + j] = A.a[i] * B.a[j] + A.a[3 + i] * B.a[3 + j] + A.a[6 + i] * B.a[6 + j];
+    return C;
+}
+DEV m3 mmul_nt(const m3& A, const m3& B) {  // A B^T
+    m3 C;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) C.a[i * 3 + j] = A.a[i * 3] * B.a[j * 3] + A.a[i * 3 + 1] * B.a[j * 3 + 1] + A.a[i * 3 + 2] * B.a[j * 3 + 2];
+    return C;
+}
+DEV float mdet(const m3& F) {
+    const float* f = F.a;
+    return f[0] * (f[4] * f[8] - f[5] * f[7]) - f[1] * (f[3] * f[8] - f[5] * f[6]) + f[2] * (f[3] * f[7] - f[4] * f[6]);
+}
+DEV m3 mcof(const m3& F) {
+    const float* f = F.a;
+    m3 c;
+    c.a[0] = f[4] * f[8] - f[5] * f[7]; c.a[1] = f[5] * f[6] - f[3] * f[8]; c.a[2] = f[3] * f[7] - f[4] * f[6];
+    c.a[3] = f[2] * f[7] - f[1] * f[8]; c.a[4] = f[0] * f[8] - f[2] * f[6]; c.a[5] = f[1] * f[6] - f[0] * f[7];
+    c.a[6] = f[1] * f[5] - f[2] * f[4]; c.a[7] = f[2] * f[3] - f[0] * f[5]; c.a[8] = f[0] * f[4] - f[1] * f[3];
+    return c;
+}
+
+// ------------------------------------------------------------------ register SVD
+// Own implementation following the structure of McAdams et al. 2011,
+// "Computing the Singular Value Decomposition of 3x3 matrices with minimal
+// branching" (UW-Madison CS TR1690): cyclic Jacobi eigen-decomposition of
+// A^T A (here with exact Givens angles) gives V; B = A V; columns sorted by
+// norm, descending; Givens QR of B gives U and R ~ diag(sigma).  U and V are
+// rotations, sigma_1 >= sigma_2 >= |sigma_3|, and sigma_3 carries the sign of
+// det(A) -- the convention of the reference's ti.svd (mpm_simulator.py:129).
+DEV void jacobi_rot(float& s_pp, float& s_qq, float& s_pq, float& s_rp, float& s_rq,
+                    float& vp0, float& vp1, float& vp2, float& vq0, float& vq1, float& vq2) {
+    if (fabsf(s_pq) < 1e-30f) return;
+    float th = (s_qq - s_pp) / (2.f * s_pq);
+    float t = (th >= 0.f ? 1.f : -1.f) / (fabsf(th) + sqrtf(fmaf(th, th, 1.f)));
+    float c = 1.f / sqrtf(fmaf(t, t, 1.f));
+    float s = t * c;
+    s_pp -= t * s_pq;
+    s_qq += t * s_pq;
+    s_pq = 0.f;
+    float rp = s_rp, rq = s_rq;
+    s_rp = c * rp - s * rq;
+    s_rq = s * rp + c * rq;
+    float a0 = vp0, a1 = vp1, a2 = vp2;
+    vp0 = c * a0 - s * vq0; vp1 = c * a1 - s * vq1; vp2 = c * a2 - s * vq2;
+    vq0 = s * a0 + c * vq0; vq1 = s * a1 + c * vq1; vq2 = s * a2 + c * vq2;
+}
+DEV void givens_qr(float& a, float& b, float& c, float& s) {
+    float r = sqrtf(a * a + b * b);
+    if (r > 1e-30f) { c = a / r; s = b / r; } else { c = 1.f; s = 0.f; }
+}
+DEV void svd3(const m3& A, m3& U, float sig[3], m3& V) {
+    // S = A^T A
+    float s00 = A.a[0] * A.a[0] + A.a[3] * A.a[3] + A.a[6] * A.a[6];
+    float s11 = A.a[1] * A.a[1] + A.a[4] * A.a[4] + A.a[7] * A.a[7];
+    float s22 = A.a[2] * A.a[2] + A.a[5] * A.a[5] + A.a[8] * A.a[8];
+    float s01 = A.a[0] * A.a[1] + A.a[3] * A.a[4] + A.a[6] * A.a[7];
+    float s02 = A.a[0] * A.a[2] + A.a[3] * A.a[5] + A.a[6] * A.a[8];
+    float s12 = A.a[1] * A.a[2] + A.a[4] * A.a[5] + A.a[7] * A.a[8];
+    // V columns c0, c1, c2 stored as v[row][col] -> keep per column
+    float v00 = 1.f, v10 = 0.f, v20 = 0.f, v01 = 0.f, v11 = 1.f, v21 = 0.f, v02 = 0.f, v12 = 0.f, v22 = 1.f;
+#pragma unroll
+    for (int sweep = 0; sweep < 5; sweep++) {
+        // (p,q) = (0,1), r = 2
+        jacobi_rot(s00, s11, s01, s02, s12, v00, v10, v20, v01, v11, v21);
+        // (p,q) = (0,2), r = 1
+        jacobi_rot(s00, s22, s02, s01, s12, v00, v10, v20, v02, v12, v22);
+        // (p,q) = (1,2), r = 0
+        jacobi_rot(s11, s22, s12, s01, s02, v01, v11, v21, v02, v12, v22);
+    }
+    // B = A V
+    float b[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        b[i * 3 + 0] = A.a[i * 3] * v00 + A.a[i * 3 + 1] * v10 + A.a[i * 3 + 2] * v20;
+        b[i * 3 + 1] = A.a[i * 3] * v01 + A.a[i * 3 + 1] * v11 + A.a[i * 3 + 2] * v21;
+        b[i * 3 + 2] = A.a[i * 3] * v02 + A.a[i * 3 + 1] * v12 + A.a[i * 3 + 2] * v22;
+    }
+    float vv[9] = {v00, v01, v02, v10, v11, v12, v20, v21, v22};
+    float rho[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) rho[j] = b[j] * b[j] + b[3 + j] * b[3 + j] + b[6 + j] * b[6 + j];
+    // sort columns descending; each swap negates one column to keep det V = +1
+#define DSK_SWAP(i, j)                                                                        \
+    if (rho[i] < rho[j]) {                                                                    \
+        float t_ = rho[i]; rho[i] = rho[j]; rho[j] = t_;                                      \
+        _Pragma("unroll") for (int r_ = 0; r_ < 3; r_++) {                                    \
+            float tb = b[r_ * 3 + i]; b[r_ * 3 + i] = b[r_ * 3 + j]; b[r_ * 3 + j] = -tb;     \
+            float tv = vv[r_ * 3 + i]; vv[r_ * 3 + i] = vv[r_ * 3 + j]; vv[r_ * 3 + j] = -tv; \
+        }                                                                                     \
+    }
+    DSK_SWAP(0, 1)
+    DSK_SWAP(0, 2)
+    DSK_SWAP(1, 2)
+#undef DSK_SWAP
+    // Givens QR: zero b10, b20, b21 ; U accumulates the rotations
+    float u[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+    float c, s;
+    // rows 0,1 -> zero b[1][0]
+    givens_qr(b[0], b[3], c, s);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        float r0 = b[j], r1 = b[3 + j];
+        b[j] = c * r0 + s * r1; b[3 + j] = -s * r0 + c * r1;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {  // U = U * G^T, G rows (c s; -s c)
+        float u0 = u[i * 3], u1 = u[i * 3 + 1];
+        u[i * 3] = c * u0 + s * u1; u[i * 3 + 1] = -s * u0 + c * u1;
+    }
+    // rows 0,2 -> zero b[2][0]
+    givens_qr(b[0], b[6], c, s);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        float r0 = b[j], r2 = b[6 + j];
+        b[j] = c * r0 + s * r2; b[6 + j] = -s * r0 + c * r2;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float u0 = u[i * 3], u2 = u[i * 3 + 2];
+        u[i * 3] = c * u0 + s * u2; u[i * 3 + 2] = -s * u0 + c * u2;
+    }
+    // rows 1,2 -> zero b[2][1]
+    givens_qr(b[4], b[7], c, s);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        float r1 = b[3 + j], r2 = b[6 + j];
+        b[3 + j] = c * r1 + s * r2; b[6 + j] = -s * r1 + c * r2;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float u1 = u[i * 3 + 1], u2 = u[i * 3 + 2];
+        u[i * 3 + 1] = c * u1 + s * u2; u[i * 3 + 2] = -s * u1 + c * u2;
+    }
+    sig[0] = b[0]; sig[1] = b[4]; sig[2] = b[8];
+#pragma unroll
+    for (int i = 0; i < 9; i++) { U.a[i] = u[i]; V.a[i] = vv[i]; }
+}
+
+// ------------------------------------------------------------------ shapes (local frame)
+// Box._sdf (primitives.py:369-374); g receives d sdf / d p
+DEV float box_sdf(v3 s, v3 p, v3* g) {
+    float q0 = fabsf(p.x) - s.x, q1 = fabsf(p.y) - s.y, q2 = fabsf(p.z) - s.z;
+    float a0 = tmax(q0, 0.f), a1 = tmax(q1, 0.f), a2 = tmax(q2, 0.f);
+    float L = sqrtf(a0 * a0 + a1 * a1 + a2 * a2 + LEN_EPS_SHAPE);
+    float m12 = tmax(q1, q2), m = tmax(q0, m12);
+    if (g) {
+        float g0 = (0.f < q0) ? a0 / L : 0.f, g1 = (0.f < q1) ? a1 / L : 0.f, g2 = (0.f < q2) ? a2 / L : 0.f;
+        if (m < 0.f) {
+            if (m12 < q0) g0 += 1.f;
+            else if (q2 < q1) g1 += 1.f;
+            else g2 += 1.f;
+        }
+        *g = mk(sgnf(p.x) * g0, sgnf(p.y) * g1, sgnf(p.z) * g2);
+    }
+    return L + tmin(m, 0.f);
+}
+// Capsule (primitives.py:54-66): local y is the segment axis, h = full length
+DEV v3 capsule_p2(float h, v3 p, float* dy) {
+    float y = p.y + h / 2.f;
+    float mx = tmax(y, 0.f);
+    float c = tmin(mx, h);
+    *dy = ((0.f < y) && (mx < h)) ? 0.f : 1.f;
+    return mk(p.x, y - c, p.z);
+}
+DEV float capsule_sdf(float h, float r, v3 p, v3* g) {
+    float dy;
+    v3 v = capsule_p2(h, p, &dy);
+    float L = sqrtf(dot(v, v) + LEN_EPS_SHAPE);
+    if (g) *g = mk(v.x / L, v.y / L * dy, v.z / L);
+    return L - r;
+}
+DEV v3 capsule_normal(float h, v3 p) {
+    float dy;
+    v3 v = capsule_p2(h, p, &dy);
+    float L = sqrtf(dot(v, v) + LEN_EPS_SHAPE);
+    return mk(v.x / L, v.y / L, v.z / L);
+}
+DEV v3 capsule_normal_rev(float h, v3 p, v3 gn) {
+    float dy;
+    v3 v = capsule_p2(h, p, &dy);
+    float L = sqrtf(dot(v, v) + LEN_EPS_SHAPE);
+    float d = dot(v, gn);
+    float L3 = L * L * L;
+    v3 gv = mk(gn.x / L - v.x * d / L3, gn.y / L - v.y * d / L3, gn.z / L - v.z * d / L3);
+    return mk(gv.x, gv.y * dy, gv.z);
+}
+// Prism._sdf (primitives.py:711-719)
+DEV float prism_sdf(const PrimC& P, v3 p, v3* g) {
+    q4 iq = conj_n(mkq(P.prot[0], P.prot[1], P.prot[2], P.prot[3]));
+    v3 gg = qrot(iq, p);
+    float q0 = fabsf(gg.x), q2 = fabsf(gg.z);
+    float t1 = q2 - P.ph[1];
+    float ta = q0 * 0.866025f + gg.y * 0.5f;
+    float tb = -gg.y;
+    float t2 = tmax(ta, tb);
+    float t3 = t2 - P.ph[0] * 0.5f;
+    if (g) {
+        v3 ggg = mk(0.f, 0.f, 0.f);
+        if (t3 < t1) ggg.z = sgnf(gg.z);
+        else if (tb < ta) { ggg.x = sgnf(gg.x) * 0.866025f; ggg.y = 0.5f; }
+        else ggg.y = -1.f;
+        *g = qrot_rev(iq, p, ggg, nullptr);
+    }
+    return tmax(t1, t3);
+}
+DEV float chop_sdf(const PrimC& P, float gap, v3 p, v3* g, float* ggap) {
+    v3 pp = mk(p.x, p.y + P.h / 2.f, p.z);
+    v3 pa = mk(pp.x - gap / 2.f, pp.y, pp.z), pb = mk(pp.x + gap / 2.f, pp.y, pp.z);
+    v3 ga, gb;
+    float a = capsule_sdf(P.h, P.r, pa, g ? &ga : nullptr), b = capsule_sdf(P.h, P.r, pb, g ? &gb : nullptr);
+    if (g) {
+        if (a < b) { *g = ga; if (ggap) *ggap = -ga.x / 2.f; }
+        else { *g = gb; if (ggap) *ggap = gb.x / 2.f; }
+    }
+    return tmin(a, b);
+}
+DEV float local_sdf(const PrimC& P, int sh, float gap, v3 p, v3* g, float* ggap) {
+    if (ggap) *ggap = 0.f;
+    switch (sh) {
+    case SH_CAPSULE: return capsule_sdf(P.h, P.r, p, g);
+    case SH_BOX: return box_sdf(mk(P.size[0], P.size[1], P.size[2]), p, g);
+    case SH_PRISM: return prism_sdf(P, p, g);
+    case SH_KNIFE: {  // Knife._sdf = max(prism, box) (primitives.py:769-773)
+        v3 g1, g2;
+        float a = prism_sdf(P, p, g ? &g1 : nullptr);
+        float b = box_sdf(mk(P.size[0], P.size[1], P.size[2]), p, g ? &g2 : nullptr);
+        if (g) *g = (b < a) ? g1 : g2;
+        return tmax(a, b);
+    }
+    case SH_CHOP: return chop_sdf(P, gap, p, g, ggap);
+    default: {  // Sphere.sdf (primitives.py:28-30), world offset p = x - position
+        float L = sqrtf(dot(p, p) + LEN_EPS_SHAPE);
+        if (g) *g = mk(p.x / L, p.y / L, p.z / L);
+        return L - P.r;
+    }
+    }
+}
+// normals: analytic for sphere / capsule / chopsticks, central FD with the
+// float32 step 1e-4 for box / prism / knife (primitives.py:382-393,720-731,796-807)
+DEV v3 local_normal(const PrimC& P, int sh, float gap, v3 p) {
+    if (sh == SH_CAPSULE) return capsule_normal(P.h, p);
+    if (sh == SH_SPHERE) { float L = sqrtf(dot(p, p) + LEN_EPS_SHAPE); return mk(p.x / L, p.y / L, p.z / L); }
+    if (sh == SH_CHOP) {
+        v3 pp = mk(p.x, p.y + P.h / 2.f, p.z);
+        v3 pa = mk(pp.x - gap / 2.f, pp.y, pp.z), pb = mk(pp.x + gap / 2.f, pp.y, pp.z);
+        float a = capsule_sdf(P.h, P.r, pa, nullptr), b = capsule_sdf(P.h, P.r, pb, nullptr);
+        return (a <= b) ? capsule_normal(P.h, pa) : capsule_normal(P.h, pb);
+    }
+    const float d = 1e-4f;
+    v3 m;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        v3 inc = p, dec = p;
+        setc(inc, i, comp(p, i) + d);
+        setc(dec, i, comp(p, i) - d);
+        setc(m, i, (0.5f / d) * (local_sdf(P, sh, gap, inc, nullptr, nullptr) - local_sdf(P, sh, gap, dec, nullptr, nullptr)));
+    }
+    float L = sqrtf(dot(m, m) + LEN_EPS_SHAPE);
+    return mk(m.x / L, m.y / L, m.z / L);
+}
+DEV v3 local_normal_rev(const PrimC& P, int sh, float gap, v3 p, v3 gn, float* ggap) {
+    if (sh == SH_CAPSULE) return capsule_normal_rev(P.h, p, gn);
+    if (sh == SH_SPHERE) {
+        float L = sqrtf(dot(p, p) + LEN_EPS_SHAPE), dd = dot(p, gn), L3 = L * L * L;
+        return mk(gn.x / L - p.x * dd / L3, gn.y / L - p.y * dd / L3, gn.z / L - p.z * dd / L3);
+    }
+    if (sh == SH_CHOP) {
+        v3 pp = mk(p.x, p.y + P.h / 2.f, p.z);
+        v3 pa = mk(pp.x - gap / 2.f, pp.y, pp.z), pb = mk(pp.x + gap / 2.f, pp.y, pp.z);
+        float a = capsule_sdf(P.h, P.r, pa, nullptr), b = capsule_sdf(P.h, P.r, pb, nullptr);
+        v3 g3;
+        if (a <= b) { g3 = capsule_normal_rev(P.h, pa, gn); *ggap += -g3.x / 2.f; }
+        else { g3 = capsule_normal_rev(P.h, pb, gn); *ggap += g3.x / 2.f; }
+        return g3;
+    }
+    const float d = 1e-4f;
+    v3 m, gi[3], gd[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        v3 inc = p, dec = p;
+        setc(inc, i, comp(p, i) + d);
+        setc(dec, i, comp(p, i) - d);
+        float a = local_sdf(P, sh, gap, inc, &gi[i], nullptr);
+        float b = local_sdf(P, sh, gap, dec, &gd[i], nullptr);
+        setc(m, i, (0.5f / d) * (a - b));
+    }
+    float L = sqrtf(dot(m, m) + LEN_EPS_SHAPE), dd = dot(m, gn), L3 = L * L * L;
+    v3 gp = mk(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float gm = comp(gn, i) / L - comp(m, i) * dd / L3;
+        gp += (gm * (0.5f / d)) * (gi[i] - gd[i]);
+    }
+    return gp;
+}
+
+// ------------------------------------------------------------------ primitive frames
+struct PS { v3 pos; q4 rot; float gap; };
+struct PSg { v3 pos; q4 rot; float gap; };
+DEV PS load_ps(const float* p) {
+    PS s;
+    s.pos = mk(p[0], p[1], p[2]);
+    s.rot = mkq(p[3], p[4], p[5], p[6]);
+    s.gap = p[7];
+    return s;
+}
+DEV PSg zero_psg() { PSg g; g.pos = mk(0.f, 0.f, 0.f); g.rot = mkq(0.f, 0.f, 0.f, 0.f); g.gap = 0.f; return g; }
+
+// a collision unit: the primitive itself (jaw 0) or a Gripper / Gripper2 jaw
+// at get_pos(f, flag) = pos + qrot(rot, (gap / 2 * flag, 0, 0)) (primitives.py:471-473)
+struct Unit { int jaw, sh, norot; float len_eps; };
+
+DEV v3 unit_center(const PS& s, int jaw) {
+    if (!jaw) return s.pos;
+    return s.pos + qrot(s.rot, mk(s.gap / 2.f * (float)jaw, 0.f, 0.f));
+}
+DEV void unit_center_rev(const PS& s, int jaw, v3 gc, PSg* g) {
+    g->pos += gc;
+    if (jaw) {
+        v3 go = qrot_rev(s.rot, mk(s.gap / 2.f * (float)jaw, 0.f, 0.f), gc, &g->rot);
+        g->gap += go.x * (float)jaw / 2.f;
+    }
+}
+// inv_trans (utils.py:50-54)
+DEV v3 to_local(const PS& s, const Unit& u, v3 x) {
+    v3 d = x - unit_center(s, u.jaw);
+    if (u.norot) return d;
+    return qrot(conj_n(s.rot), d);
+}
+DEV void to_local_rev(const PS& s, const Unit& u, v3 x, v3 gp, PSg* g) {
+    v3 d = x - unit_center(s, u.jaw);
+    v3 gd;
+    if (u.norot) gd = gp;
+    else {
+        q4 giq = mkq(0.f, 0.f, 0.f, 0.f);
+        gd = qrot_rev(conj_n(s.rot), d, gp, &giq);
+        conj_n_rev(s.rot, giq, &g->rot);
+    }
+    unit_center_rev(s, u.jaw, -gd, g);
+}
+
+DEV float unit_sdf(const PrimC& P, const Unit& u, const PS& s, v3 x) {
+    return local_sdf(P, u.sh, s.gap, to_local(s, u, x), nullptr, nullptr);
+}
+DEV void unit_sdf_rev(const PrimC& P, const Unit& u, const PS& s, v3 x, float gd, PSg* g) {
+    v3 p = to_local(s, u, x), gp;
+    float ggap;
+    local_sdf(P, u.sh, s.gap, p, &gp, &ggap);
+    if (u.sh == SH_CHOP) g->gap += ggap * gd;
+    to_local_rev(s, u, x, gd * gp, g);
+}
+// world normal qrot(rot, _normal(local)) (primive_base.py:81-85)
+DEV v3 unit_normal(const PrimC& P, const Unit& u, const PS& s, v3 x) {
+    v3 n = local_normal(P, u.sh, s.gap, to_local(s, u, x));
+    return u.norot ? n : qrot(s.rot, n);
+}
+DEV void unit_normal_rev(const PrimC& P, const Unit& u, const PS& s, v3 x, v3 gD, PSg* g) {
+    v3 p = to_local(s, u, x);
+    v3 gn;
+    if (u.norot) gn = gD;
+    else gn = qrot_rev(s.rot, local_normal(P, u.sh, s.gap, p), gD, &g->rot);
+    float ggap = 0.f;
+    v3 gp = local_normal_rev(P, u.sh, s.gap, p, gn, &ggap);
+    if (u.sh == SH_CHOP) g->gap += ggap;
+    to_local_rev(s, u, x, gp, g);
+}
+// collider_v (primive_base.py:87-94, primitives.py:497-502)
+DEV v3 unit_collider_v(const Unit& u, const PS& s0, const PS& s1, v3 x, float dt) {
+    v3 rel = qrot(conj_n(s0.rot), x - unit_center(s0, u.jaw));
+    v3 nw = qrot(s1.rot, rel) + unit_center(s1, u.jaw);
+    return mk((nw.x - x.x) / dt, (nw.y - x.y) / dt, (nw.z - x.z) / dt);
+}
+DEV void unit_collider_v_rev(const Unit& u, const PS& s0, const PS& s1, v3 x, float dt, v3 gvc, PSg* g0, PSg* g1) {
+    v3 d = x - unit_center(s0, u.jaw);
+    q4 iq = conj_n(s0.rot);
+    v3 rel = qrot(iq, d);
+    v3 gn = mk(gvc.x / dt, gvc.y / dt, gvc.z / dt);
+    unit_center_rev(s1, u.jaw, gn, g1);
+    v3 grel = qrot_rev(s1.rot, rel, gn, &g1->rot);
+    q4 giq = mkq(0.f, 0.f, 0.f, 0.f);
+    v3 gd = qrot_rev(iq, d, grel, &giq);
+    conj_n_rev(s0.rot, giq, &g0->rot);
+    unit_center_rev(s0, u.jaw, -gd, g0);
+}
+
+DEV int prim_units(const PrimC& P, Unit* u) {
+    if (P.type == P_GRIPPER || P.type == P_GRIPPER2) {
+        u[0].jaw = -1; u[0].sh = P.unit_shape; u[0].norot = 0; u[0].len_eps = LEN_EPS_SHAPE;
+        u[1] = u[0]; u[1].jaw = 1;
+        return 2;
+    }
+    u[0].jaw = 0; u[0].sh = P.unit_shape; u[0].norot = (P.unit_shape == SH_SPHERE); u[0].len_eps = LEN_EPS_BASE;
+    return 1;
+}
+
+// Primitive.collide (primive_base.py:96-120) / Gripper.collide2 (primitives.py:508-536)
+struct CollTr {
+    int branch, flag;
+    float dist, ex, infl, nc, mn, vtn, mx;
+    v3 D, vc, u, vt, vt2;
+};
+DEV v3 unit_collide(const PrimC& P, const Unit& un, const PS& s0, const PS& s1, v3 x, float dt, v3 v, CollTr* tr) {
+    CollTr t;
+    t.dist = unit_sdf(P, un, s0, x);
+    t.ex = expf(-t.dist * P.softness);
+    t.infl = tmin(t.ex, 1.f);
+    t.branch = (P.softness > 0.f && t.infl > 0.1f) || t.dist <= 0.f;
+    v3 out = v;
+    if (t.branch) {
+        t.D = unit_normal(P, un, s0, x);
+        t.vc = unit_collider_v(un, s0, s1, x, dt);
+        t.u = v - t.vc;
+        t.nc = dot(t.u, t.D);
+        t.mn = tmin(t.nc, 0.f);
+        t.vt = t.u - t.mn * t.D;
+        t.vtn = sqrtf(dot(t.vt, t.vt) + un.len_eps);
+        t.mx = tmax(0.f, t.vtn + t.nc * P.friction);
+        t.flag = (t.nc < 0.f) && (sqrtf(dot(t.vt, t.vt)) > 1e-30f);
+        float fl = t.flag ? 1.f : 0.f;
+        v3 vtf = mk(t.vt.x / t.vtn * t.mx, t.vt.y / t.vtn * t.mx, t.vt.z / t.vtn * t.mx);
+        t.vt2 = vtf * fl + t.vt * (1.f - fl);
+        out = t.vc + t.u * (1.f - t.infl) + t.vt2 * t.infl;
+    }
+    if (tr) *tr = t;
+    return out;
+}
+DEV v3 unit_collide_rev(const PrimC& P, const Unit& un, const PS& s0, const PS& s1, v3 x, float dt, v3 v, v3 gout,
+                        PSg* g0, PSg* g1) {
+    CollTr t;
+    unit_collide(P, un, s0, s1, x, dt, v, &t);
+    if (!t.branch) return gout;
+    v3 gvc = gout;
+    v3 gu = gout * (1.f - t.infl);
+    float ginfl = dot(gout, t.vt2 - t.u);
+    v3 gvt2 = gout * t.infl;
+    v3 gvt = mk(0.f, 0.f, 0.f);
+    float gvtn = 0.f, gnc = 0.f;
+    if (t.flag) {
+        v3 gq = gvt2 * t.mx;
+        float gmx = dot(gvt2, mk(t.vt.x / t.vtn, t.vt.y / t.vtn, t.vt.z / t.vtn));
+        gvt += mk(gq.x / t.vtn, gq.y / t.vtn, gq.z / t.vtn);
+        gvtn -= dot(gq, t.vt) / (t.vtn * t.vtn);
+        if (!((t.vtn + t.nc * P.friction) < 0.f)) { gvtn += gmx; gnc += gmx * P.friction; }
+    } else {
+        gvt += gvt2;
+    }
+    gvt += (gvtn / t.vtn) * t.vt;
+    gu += gvt;
+    float gmn = -dot(gvt, t.D);
+    v3 gD = (-t.mn) * gvt;
+    if (t.nc < 0.f) gnc += gmn;
+    gu += gnc * t.D;
+    gD += gnc * t.u;
+    v3 gv = gu;
+    gvc -= gu;
+    unit_normal_rev(P, un, s0, x, gD, g0);
+    unit_collider_v_rev(un, s0, s1, x, dt, gvc, g0, g1);
+    if (t.ex < 1.f) {
+        float gdist = ginfl * t.ex * (-P.softness);
+        if (gdist != 0.f) unit_sdf_rev(P, un, s0, x, gdist, g0);
+    }
+    return gv;
+}
+
+// ------------------------------------------------------------------ primitive kinematics
+DEV float clampv(float a, float lo, float hi) { return tmax(tmin(a, hi), lo); }
+DEV float clampv_grad(float a, float lo, float hi) { return (a < hi && lo < tmin(a, hi)) ? 1.f : 0.f; }
+
+// forward_kinematics: primive_base.py:152-156; RollingPin primitives.py:103-117;
+// RollingPinExt :121-136; Chopsticks :230-235; Gripper :456-460; Gripper2 :588-592
+DEV void prim_fk(const PrimC& P, const float* s, const float* pv, float* o) {
+    PS st = load_ps(s);
+    v3 pos;
+    q4 rot;
+    float gap = st.gap;
+    if (P.type == P_ROLLINGPIN || P.type == P_ROLLINGPIN_EXT) {
+        float dw = pv[0], dth = pv[1], dy = pv[2];
+        v3 ydir = qrot(st.rot, mk(0.f, -1.f, 0.f));
+        float k = (P.type == P_ROLLINGPIN) ? dw * 0.03f : dw * 0.03f + pv[3];
+        v3 xdir = mk(ydir.z * k, dy, -ydir.x * k);
+        rot = qmul(w2quat(mk(0.f, -dth, 0.f)), qmul(st.rot, w2quat(mk(0.f, dw, 0.f))));
+        pos = mk(clampv(st.pos.x + xdir.x, P.lo[0], P.hi[0]), clampv(st.pos.y + xdir.y, P.lo[1], P.hi[1]),
+                 clampv(st.pos.z + xdir.z, P.lo[2], P.hi[2]));
+    } else {
+        pos = mk(clampv(st.pos.x + pv[0], P.lo[0], P.hi[0]), clampv(st.pos.y + pv[1], P.lo[1], P.hi[1]),
+                 clampv(st.pos.z + pv[2], P.lo[2], P.hi[2]));
+        q4 wq = w2quat(mk(pv[3], pv[4], pv[5]));
+        if (P.type == P_GRIPPER || P.type == P_GRIPPER2 || P.type == P_CHOPSTICKS) {
+            rot = qmul(st.rot, wq);
+            float g = tmax(st.gap - pv[6], P.min_gap);
+            gap = (P.type == P_CHOPSTICKS) ? g : tmin(g, P.max_gap);
+        } else {
+            rot = qmul(wq, st.rot);
+        }
+    }
+    o[0] = pos.x; o[1] = pos.y; o[2] = pos.z;
+    o[3] = rot.w; o[4] = rot.x; o[5] = rot.y; o[6] = rot.z;
+    o[7] = gap;
+}
+// reverse: go = adjoint of frame f+1 state; adds into gs (frame f) and gpv
+DEV void prim_fk_rev(const PrimC& P, const float* s, const float* pv, const float* go, float* gs, float* gpv) {
+    PS st = load_ps(s);
+    q4 gq = mkq(0.f, 0.f, 0.f, 0.f);
+    q4 gro = mkq(go[3], go[4], go[5], go[6]);
+    if (P.type == P_ROLLINGPIN || P.type == P_ROLLINGPIN_EXT) {
+        float dw = pv[0], dth = pv[1], dy = pv[2];
+        v3 ny = mk(0.f, -1.f, 0.f);
+        v3 ydir = qrot(st.rot, ny);
+        float k = (P.type == P_ROLLINGPIN) ? dw * 0.03f : dw * 0.03f + pv[3];
+        v3 xdir = mk(ydir.z * k, dy, -ydir.x * k);
+        float gx0 = go[0] * clampv_grad(st.pos.x + xdir.x, P.lo[0], P.hi[0]);
+        float gx1 = go[1] * clampv_grad(st.pos.y + xdir.y, P.lo[1], P.hi[1]);
+        float gx2 = go[2] * clampv_grad(st.pos.z + xdir.z, P.lo[2], P.hi[2]);
+        gs[0] += gx0; gs[1] += gx1; gs[2] += gx2;
+        float gk = gx0 * ydir.z - gx2 * ydir.x;
+        gpv[2] += gx1;
+        qrot_rev(st.rot, ny, mk(-gx2 * k, 0.f, gx0 * k), &gq);
+        gpv[0] += gk * 0.03f;
+        if (P.type == P_ROLLINGPIN_EXT) gpv[3] += gk;
+        v3 a1 = mk(0.f, -dth, 0.f), a2 = mk(0.f, dw, 0.f);
+        q4 q1 = w2quat(a1), q2 = w2quat(a2);
+        q4 t = qmul(st.rot, q2);
+        q4 gq1 = mkq(0.f, 0.f, 0.f, 0.f), gt = mkq(0.f, 0.f, 0.f, 0.f), gq2 = mkq(0.f, 0.f, 0.f, 0.f);
+        qmul_rev(q1, t, gro, &gq1, &gt);
+        qmul_rev(st.rot, q2, gt, &gq, &gq2);
+        v3 ga1 = w2quat_rev(a1, gq1), ga2 = w2quat_rev(a2, gq2);
+        gpv[1] -= ga1.y;
+        gpv[0] += ga2.y;
+        gs[7] += go[7];
+    } else {
+        float g0 = go[0] * clampv_grad(st.pos.x + pv[0], P.lo[0], P.hi[0]);
+        float g1 = go[1] * clampv_grad(st.pos.y + pv[1], P.lo[1], P.hi[1]);
+        float g2 = go[2] * clampv_grad(st.pos.z + pv[2], P.lo[2], P.hi[2]);
+        gs[0] += g0; gs[1] += g1; gs[2] += g2;
+        gpv[0] += g0; gpv[1] += g1; gpv[2] += g2;
+        v3 w = mk(pv[3], pv[4], pv[5]);
+        q4 wq = w2quat(w), gwq = mkq(0.f, 0.f, 0.f, 0.f);
+        if (P.type == P_GRIPPER || P.type == P_GRIPPER2 || P.type == P_CHOPSTICKS) {
+            qmul_rev(st.rot, wq, gro, &gq, &gwq);
+            float a = st.gap - pv[6];
+            float g1g = tmax(a, P.min_gap);
+            float gg = go[7];
+            if (P.type != P_CHOPSTICKS) gg = (g1g < P.max_gap) ? gg : 0.f;
+            gg = (P.min_gap < a) ? gg : 0.f;
+            gs[7] += gg;
+            gpv[6] -= gg;
+        } else {
+            qmul_rev(wq, st.rot, gro, &gwq, &gq);
+            gs[7] += go[7];
+        }
+        v3 gw = w2quat_rev(w, gwq);
+        gpv[3] += gw.x; gpv[4] += gw.y; gpv[5] += gw.z;
+    }
+    gs[3] += gq.w; gs[4] += gq.x; gs[5] += gq.y; gs[6] += gq.z;
+}
+
+}  // namespace dsk

@@ -1,0 +1,1328 @@
+// MI355X (gfx950) differentiable MLS-MPM engine: kernels and the C ABI.
+//
+// One context packs any number of scenes (each with its own grid and tool
+// primitives) into shared device arrays, so every phase of a substep is ONE
+// launch over all scenes:
+//   forward  : mark(+primitive FK) -> compact -> p2g -> grid_op -> g2p
+//   adjoint  : mark -> compact -> p2g -> grid_op(keep) -> g2p_bwd ->
+//              grid_op_bwd -> p2g_bwd(+primitive FK adjoint)
+// The reference runs the same sequence as ~7 + n_prim Taichi launches per
+// scene and substep (plb/engine/mpm_simulator.py:307-345).
+//
+// Layouts in HBM:
+//   particles  SoA, 24 rows of `stride` floats per frame (x, v, C, F), each
+//              scene a 64-aligned column range; frames live in checkpoint
+//              slots (every K-th frame) plus a ring of K-1 live frames.
+//   grid       block-major: 4x4x4 node blocks of 64 float4 (1 KiB); only
+//              blocks touched by particles ("active") are visited.
+//   primitives [frame][prim][8] (pos, quat wxyz, gap), velocities [frame][prim][8].
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/diffskill_mpm.h"
+#include "mpm_device.h"
+
+using namespace dsk;
+
+namespace {
+
+thread_local std::string g_err;
+int fail(const std::string& m) { g_err = m; return -1; }
+#define HIPCHK(x)                                                              \
+    do {                                                                       \
+        hipError_t e_ = (x);                                                   \
+        if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int NST = 24;      // particle state rows
+constexpr int PPB = 256;     // particles per chunk / threads per particle block
+constexpr int NODES_PER_BLK = 64;
+
+// ------------------------------------------------------------------ device helpers
+DEV long long node_id(const SceneC& S, int i, int j, int k) {
+    int b = ((i >> 2) * S.nb + (j >> 2)) * S.nb + (k >> 2);
+    return S.g_off + (long long)b * NODES_PER_BLK + (((i & 3) << 4) | ((j & 3) << 2) | (k & 3));
+}
+DEV bool in_grid(const SceneC& S, int i, int j, int k) {
+    return (unsigned)i < (unsigned)S.n_grid && (unsigned)j < (unsigned)S.n_grid && (unsigned)k < (unsigned)S.n_grid;
+}
+struct Stencil {
+    int base[3];
+    float fx[3];
+    float w[3][3];   // w[offset][dim]
+    float dw[3][3];
+};
+// quadratic B-spline weights; base uses truncation (cast(int)), mpm_simulator.py:201-204
+DEV void stencil(const SceneC& S, v3 x, Stencil& st) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        float xs = comp(x, d) * S.inv_dx;
+        int b = (int)(xs - 0.5f);
+        float f = xs - (float)b;
+        st.base[d] = b;
+        st.fx[d] = f;
+        st.w[0][d] = 0.5f * (1.5f - f) * (1.5f - f);
+        st.w[1][d] = 0.75f - (f - 1.f) * (f - 1.f);
+        st.w[2][d] = 0.5f * (f - 0.5f) * (f - 0.5f);
+        st.dw[0][d] = f - 1.5f;
+        st.dw[1][d] = -2.f * (f - 1.f);
+        st.dw[2][d] = f - 0.5f;
+    }
+}
+DEV v3 load3(const float* st, int stride, int row, int slot) {
+    return mk(st[(size_t)row * stride + slot], st[(size_t)(row + 1) * stride + slot], st[(size_t)(row + 2) * stride + slot]);
+}
+DEV m3 load9(const float* st, int stride, int row, int slot) {
+    m3 m;
+#pragma unroll
+    for (int i = 0; i < 9; i++) m.a[i] = st[(size_t)(row + i) * stride + slot];
+    return m;
+}
+DEV void store3(float* st, int stride, int row, int slot, v3 v) {
+    st[(size_t)row * stride + slot] = v.x;
+    st[(size_t)(row + 1) * stride + slot] = v.y;
+    st[(size_t)(row + 2) * stride + slot] = v.z;
+}
+DEV void store9(float* st, int stride, int row, int slot, const m3& m) {
+#pragma unroll
+    for (int i = 0; i < 9; i++) st[(size_t)(row + i) * stride + slot] = m.a[i];
+}
+
+// per-particle constitutive update: F_tmp, SVD, von Mises return map, stress
+// (mpm_simulator.py:121-129,165-182,205-216)
+struct PTrace {
+    m3 Ftmp, U, V, Fn, R, affine;
+    float sig[3], sc[3], eh[3], ehn, dg, e[3], J;
+    bool yield;
+};
+DEV void particle_stress(const SceneC& S, const m3& C, const m3& F, PTrace& t) {
+    m3 A;
+#pragma unroll
+    for (int i = 0; i < 9; i++) A.a[i] = ((i % 4 == 0) ? 1.f : 0.f) + S.dt * C.a[i];
+    t.Ftmp = mmul(A, F);
+    svd3(t.Ftmp, t.U, t.sig, t.V);
+    float eps[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) { t.sc[i] = tmax(t.sig[i], 0.05f); eps[i] = logf(t.sc[i]); }
+    float mean = (eps[0] + eps[1] + eps[2]) / 3.f;
+#pragma unroll
+    for (int i = 0; i < 3; i++) t.eh[i] = eps[i] - mean;
+    t.ehn = sqrtf(t.eh[0] * t.eh[0] + t.eh[1] * t.eh[1] + t.eh[2] * t.eh[2] + 1e-8f);
+    t.dg = t.ehn - S.ys / (2.f * S.mu);
+    t.yield = t.dg > 0.f;
+    if (t.yield) {
+        m3 D = mzero();
+#pragma unroll
+        for (int i = 0; i < 3; i++) { t.e[i] = expf(eps[i] - (t.dg / t.ehn) * t.eh[i]); D.a[i * 4] = t.e[i]; }
+        t.Fn = mmul_nt(mmul(t.U, D), t.V);
+    } else {
+        t.Fn = t.Ftmp;
+    }
+    t.J = mdet(t.Fn);
+    t.R = mmul_nt(t.U, t.V);
+    m3 FmR;
+#pragma unroll
+    for (int i = 0; i < 9; i++) FmR.a[i] = t.Fn.a[i] - t.R.a[i];
+    m3 P1 = mmul_nt(FmR, t.Fn);
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        float s = 2.f * S.mu * P1.a[i] + ((i % 4 == 0) ? S.lam * t.J * (t.J - 1.f) : 0.f);
+        t.affine.a[i] = S.stress_k * s + S.p_mass * C.a[i];
+    }
+}
+
+// grid_op for one node with m > 1e-12 (mpm_simulator.py:230-262)
+constexpr int MAX_UNITS = 8;
+struct NodeTr { v3 vu[MAX_UNITS]; v3 vw[3]; };
+DEV v3 grid_node(const SceneC& S, const PrimC* prims, const float* ps0, const float* ps1, int I0, int I1, int I2,
+                 float4 pm, NodeTr* tr) {
+    float inv_m = 1.f / pm.w;
+    v3 v = mk(inv_m * pm.x, inv_m * pm.y, inv_m * pm.z);
+    v += mk(S.g30[0], S.g30[1], S.g30[2]);
+    v3 x = mk((float)I0 * S.dx, (float)I1 * S.dx, (float)I2 * S.dx);
+    int nu = 0;
+    for (int q = 0; q < S.n_prims; q++) {
+        const PrimC& P = prims[q];
+        PS s0 = load_ps(ps0 + q * 8), s1 = load_ps(ps1 + q * 8);
+        Unit u[2];
+        int m = prim_units(P, u);
+        for (int j = 0; j < m; j++) {
+            if (tr && nu < MAX_UNITS) tr->vu[nu] = v;
+            v = unit_collide(P, u[j], s0, s1, x, S.dt, v, nullptr);
+            nu++;
+        }
+    }
+    int I[3] = {I0, I1, I2};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (tr) tr->vw[d] = v;
+        if (I[d] < 3 && comp(v, d) < 0.f) {
+            if (d != 1 || S.gf_mode == 0) setc(v, d, 0.f);
+            else if (S.gf_mode == 1) {
+                float lin = v.y + 1e-30f;
+                v3 vit = mk(v.x - (float)I0 * 1e-30f, v.y - lin - (float)I1 * 1e-30f, v.z - (float)I2 * 1e-30f);
+                float lit = sqrtf(dot(vit, vit) + 1e-8f);
+                float c = tmax(1.f + S.gf * lin / lit, 0.f);
+                v = mk(c * (vit.x + (float)I0 * 1e-30f), 0.f, c * (vit.z + (float)I2 * 1e-30f));
+            } else {
+                v = mk(0.f, 0.f, 0.f);
+            }
+        }
+        if (I[d] > S.n_grid - 3 && comp(v, d) > 0.f) setc(v, d, 0.f);
+    }
+    return v;
+}
+
+// ------------------------------------------------------------------ kernels
+struct ChunkArgs {
+    const SceneC* scenes;
+    const int4* chunks;   // (scene, first slot, count, -)
+    int n_chunks;
+    int stride;
+};
+
+// mark active grid blocks of frame f's particles; optionally run primitive
+// forward kinematics f -> f+1 (forward_kinematics, primive_base.py:152-156)
+__global__ void __launch_bounds__(PPB) k_mark(ChunkArgs ca, const float* __restrict__ st, int* __restrict__ blk_flag,
+                                              int* __restrict__ counters, int do_fk, const PrimC* __restrict__ prims,
+                                              int n_prims_total, float* __restrict__ pstate, const float* __restrict__ pvel, int f) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[0] = 0;
+    if (do_fk && blockIdx.x == 0 && (int)threadIdx.x < n_prims_total) {
+        int q = threadIdx.x;
+        prim_fk(prims[q], pstate + ((size_t)f * n_prims_total + q) * 8, pvel + ((size_t)f * n_prims_total + q) * 8,
+                pstate + ((size_t)(f + 1) * n_prims_total + q) * 8);
+    }
+    int4 ch = ca.chunks[blockIdx.x];
+    int i = threadIdx.x;
+    if (i >= ch.z) return;
+    const SceneC& S = ca.scenes[ch.x];
+    int slot = ch.y + i;
+    v3 x = load3(st, ca.stride, 0, slot);
+    int lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        int b = (int)(comp(x, d) * S.inv_dx - 0.5f);
+        int a0 = max(b, 0), a1 = min(b + 2, S.n_grid - 1);
+        lo[d] = a0 >> 2;
+        hi[d] = a1 >> 2;
+    }
+    for (int bi = lo[0]; bi <= hi[0]; bi++)
+        for (int bj = lo[1]; bj <= hi[1]; bj++)
+            for (int bk = lo[2]; bk <= hi[2]; bk++)
+                blk_flag[S.blk_off + (bi * S.nb + bj) * S.nb + bk] = 1;
+}
+
+// compact flagged grid blocks into the active list (wave-aggregated atomics)
+__global__ void __launch_bounds__(256) k_compact(int n_blocks, int* __restrict__ blk_flag, int* __restrict__ active,
+                                                 int* __restrict__ counters) {
+    for (int b0 = blockIdx.x * blockDim.x; b0 < n_blocks; b0 += gridDim.x * blockDim.x) {
+        int b = b0 + threadIdx.x;
+        int fl = (b < n_blocks) ? blk_flag[b] : 0;
+        unsigned long long mask = __ballot(fl != 0);
+        if (mask == 0ull) continue;
+        int lane = threadIdx.x & 63;
+        int leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counters[0], __popcll(mask));
+        base = __shfl(base, leader);
+        if (fl) {
+            int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            active[base + rank] = b;
+            blk_flag[b] = 0;
+        }
+    }
+}
+
+// p2g: constitutive update + 27-node scatter of momentum and mass
+// (mpm_simulator.py:198-225).  st_out != null also stores F of frame f+1.
+__global__ void __launch_bounds__(PPB) k_p2g(ChunkArgs ca, const float* __restrict__ st, float* __restrict__ st_out,
+                                             float4* __restrict__ g_pm) {
+    int4 ch = ca.chunks[blockIdx.x];
+    int i = threadIdx.x;
+    if (i >= ch.z) return;
+    const SceneC& S = ca.scenes[ch.x];
+    int slot = ch.y + i;
+    int stride = ca.stride;
+    v3 x = load3(st, stride, 0, slot);
+    v3 v = load3(st, stride, 3, slot);
+    m3 C = load9(st, stride, 6, slot);
+    m3 F = load9(st, stride, 15, slot);
+    PTrace t;
+    particle_stress(S, C, F, t);
+    if (st_out) store9(st_out, stride, 15, slot, t.Fn);
+    Stencil sw;
+    stencil(S, x, sw);
+    v3 mv = S.p_mass * v;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                int I0 = sw.base[0] + a, I1 = sw.base[1] + b, I2 = sw.base[2] + c;
+                if (!in_grid(S, I0, I1, I2)) continue;
+                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
+                float d0 = ((float)a - sw.fx[0]) * S.dx, d1 = ((float)b - sw.fx[1]) * S.dx, d2 = ((float)c - sw.fx[2]) * S.dx;
+                float4* g = g_pm + node_id(S, I0, I1, I2);
+                const float* A = t.affine.a;
+                unsafeAtomicAdd(&g->x, wt * (mv.x + A[0] * d0 + A[1] * d1 + A[2] * d2));
+                unsafeAtomicAdd(&g->y, wt * (mv.y + A[3] * d0 + A[4] * d1 + A[5] * d2));
+                unsafeAtomicAdd(&g->z, wt * (mv.z + A[6] * d0 + A[7] * d1 + A[8] * d2));
+                unsafeAtomicAdd(&g->w, wt * S.p_mass);
+            }
+}
+
+struct GridArgs {
+    const SceneC* scenes;
+    const PrimC* prims;
+    const int* blk_scene;
+    const int* active;
+    const int* counters;
+    const float* pstate;   // [frame][n_prims_total][8]
+    int n_prims_total;
+    int f;
+};
+
+DEV void block_coords(const SceneC& S, int gb, int lane, int& I0, int& I1, int& I2, long long& node) {
+    int lb = gb - S.blk_off;
+    int bk = lb % S.nb, t = lb / S.nb, bj = t % S.nb, bi = t / S.nb;
+    I0 = bi * 4 + (lane >> 4);
+    I1 = bj * 4 + ((lane >> 2) & 3);
+    I2 = bk * 4 + (lane & 3);
+    node = S.g_off + (long long)lb * NODES_PER_BLK + lane;
+}
+
+// grid_op over the active blocks; one wave per 4x4x4 block.  clear_pm zeroes
+// the accumulated momentum/mass after use (forward), leaving the grid clean
+// for the next substep.
+__global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, float4* __restrict__ g_pm, float4* __restrict__ g_v, int clear_pm) {
+    int n_active = ga.counters[0];
+    int lane = threadIdx.x & 63;
+    int wave = threadIdx.x >> 6;
+    for (int e = blockIdx.x * 4 + wave; e < n_active; e += gridDim.x * 4) {
+        int gb = ga.active[e];
+        const SceneC& S = ga.scenes[ga.blk_scene[gb]];
+        int I0, I1, I2;
+        long long node;
+        block_coords(S, gb, lane, I0, I1, I2, node);
+        float4 pm = g_pm[node];
+        v3 v = mk(0.f, 0.f, 0.f);
+        if (pm.w > 1e-12f && in_grid(S, I0, I1, I2)) {
+            const float* ps0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
+            const float* ps1 = ga.pstate + ((size_t)(ga.f + 1) * ga.n_prims_total + S.prim_off) * 8;
+            v = grid_node(S, ga.prims + S.prim_off, ps0, ps1, I0, I1, I2, pm, nullptr);
+        }
+        g_v[node] = make_float4(v.x, v.y, v.z, 0.f);
+        if (clear_pm) g_pm[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// g2p: 27-node gather, APIC C, advection with clamp (mpm_simulator.py:264-283)
+__global__ void __launch_bounds__(PPB) k_g2p(ChunkArgs ca, const float* __restrict__ st, float* __restrict__ st_out,
+                                             const float4* __restrict__ g_v) {
+    int4 ch = ca.chunks[blockIdx.x];
+    int i = threadIdx.x;
+    if (i >= ch.z) return;
+    const SceneC& S = ca.scenes[ch.x];
+    int slot = ch.y + i;
+    int stride = ca.stride;
+    v3 x = load3(st, stride, 0, slot);
+    Stencil sw;
+    stencil(S, x, sw);
+    v3 nv = mk(0.f, 0.f, 0.f);
+    m3 nC = mzero();
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                int I0 = sw.base[0] + a, I1 = sw.base[1] + b, I2 = sw.base[2] + c;
+                if (!in_grid(S, I0, I1, I2)) continue;
+                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
+                float4 g = g_v[node_id(S, I0, I1, I2)];
+                float dp[3] = {(float)a - sw.fx[0], (float)b - sw.fx[1], (float)c - sw.fx[2]};
+                nv += wt * mk(g.x, g.y, g.z);
+                float cw = S.c4 * wt;
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    nC.a[j] += cw * g.x * dp[j];
+                    nC.a[3 + j] += cw * g.y * dp[j];
+                    nC.a[6 + j] += cw * g.z * dp[j];
+                }
+            }
+    store3(st_out, stride, 3, slot, nv);
+    store9(st_out, stride, 6, slot, nC);
+    v3 xn = mk(tmax(tmin(x.x + S.dt * nv.x, S.upper), S.lower), tmax(tmin(x.y + S.dt * nv.y, S.upper), S.lower),
+               tmax(tmin(x.z + S.dt * nv.z, S.upper), S.lower));
+    store3(st_out, stride, 0, slot, xn);
+}
+
+// ---- adjoint kernels ------------------------------------------------------
+
+// reverse of g2p: scatter the adjoint of grid_v_out, write the first part of
+// the adjoint of x (frame f).  a1 = adjoint of frame f+1, st1 = frame f+1.
+__global__ void __launch_bounds__(PPB) k_g2p_bwd(ChunkArgs ca, const float* __restrict__ st, const float* __restrict__ st1,
+                                                 const float* __restrict__ a1, float* __restrict__ a0,
+                                                 const float4* __restrict__ g_v, float4* __restrict__ g_adjv) {
+    int4 ch = ca.chunks[blockIdx.x];
+    int i = threadIdx.x;
+    if (i >= ch.z) return;
+    const SceneC& S = ca.scenes[ch.x];
+    int slot = ch.y + i;
+    int stride = ca.stride;
+    v3 x = load3(st, stride, 0, slot);
+    v3 x1 = load3(st1, stride, 0, slot);
+    v3 gx1 = load3(a1, stride, 0, slot);
+    v3 gv1 = load3(a1, stride, 3, slot);
+    m3 gC = load9(a1, stride, 6, slot);
+    // clamp mask from the stored output position: the adjoint passes iff lower < x' < upper
+    v3 gxn = mk((S.lower < x1.x && x1.x < S.upper) ? gx1.x : 0.f, (S.lower < x1.y && x1.y < S.upper) ? gx1.y : 0.f,
+                (S.lower < x1.z && x1.z < S.upper) ? gx1.z : 0.f);
+    v3 gnv = gv1 + S.dt * gxn;
+    Stencil sw;
+    stencil(S, x, sw);
+    float gfx[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                int I0 = sw.base[0] + a, I1 = sw.base[1] + b, I2 = sw.base[2] + c;
+                if (!in_grid(S, I0, I1, I2)) continue;
+                long long nid = node_id(S, I0, I1, I2);
+                float4 g4 = g_v[nid];
+                v3 g = mk(g4.x, g4.y, g4.z);
+                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
+                float dp[3] = {(float)a - sw.fx[0], (float)b - sw.fx[1], (float)c - sw.fx[2]};
+                v3 Cd = mk(gC.a[0] * dp[0] + gC.a[1] * dp[1] + gC.a[2] * dp[2], gC.a[3] * dp[0] + gC.a[4] * dp[1] + gC.a[5] * dp[2],
+                           gC.a[6] * dp[0] + gC.a[7] * dp[1] + gC.a[8] * dp[2]);
+                v3 CTg = mk(gC.a[0] * g.x + gC.a[3] * g.y + gC.a[6] * g.z, gC.a[1] * g.x + gC.a[4] * g.y + gC.a[7] * g.z,
+                            gC.a[2] * g.x + gC.a[5] * g.y + gC.a[8] * g.z);
+                float gw = dot(gnv, g) + S.c4 * dot(g, Cd);
+                v3 contrib = wt * gnv + (S.c4 * wt) * Cd;
+                float4* ga = g_adjv + nid;
+                unsafeAtomicAdd(&ga->x, contrib.x);
+                unsafeAtomicAdd(&ga->y, contrib.y);
+                unsafeAtomicAdd(&ga->z, contrib.z);
+                int off[3] = {a, b, c};
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    float dwd = sw.dw[off[d]][d];
+#pragma unroll
+                    for (int e2 = 0; e2 < 3; e2++)
+                        if (e2 != d) dwd *= sw.w[off[e2]][e2];
+                    gfx[d] += gw * dwd - S.c4 * wt * comp(CTg, d);
+                }
+            }
+    store3(a0, stride, 0, slot, gxn + S.inv_dx * mk(gfx[0], gfx[1], gfx[2]));
+}
+
+DEV float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// reverse of grid_op: adjoint of (momentum, mass) per node, plus the adjoint
+// of the primitive states of frames f and f+1 (wave-reduced, then atomics).
+// Clears g_pm and g_adjv of the visited blocks.
+__global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, float4* __restrict__ g_pm, float4* __restrict__ g_adjv,
+                                                     float4* __restrict__ g_adjpm, float* __restrict__ gpstate) {
+    int n_active = ga.counters[0];
+    int lane = threadIdx.x & 63;
+    int wave = threadIdx.x >> 6;
+    for (int e = blockIdx.x * 4 + wave; e < n_active; e += gridDim.x * 4) {
+        int gb = ga.active[e];
+        const SceneC& S = ga.scenes[ga.blk_scene[gb]];
+        int I0, I1, I2;
+        long long node;
+        block_coords(S, gb, lane, I0, I1, I2, node);
+        float4 pm = g_pm[node];
+        float4 gv4 = g_adjv[node];
+        g_pm[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+        g_adjv[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+        bool act = pm.w > 1e-12f && in_grid(S, I0, I1, I2);
+        const float* ps0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
+        const float* ps1 = ga.pstate + ((size_t)(ga.f + 1) * ga.n_prims_total + S.prim_off) * 8;
+        float* gps0 = gpstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
+        float* gps1 = gpstate + ((size_t)(ga.f + 1) * ga.n_prims_total + S.prim_off) * 8;
+        NodeTr tr;
+        v3 g = mk(gv4.x, gv4.y, gv4.z);
+        if (act) {
+            grid_node(S, ga.prims + S.prim_off, ps0, ps1, I0, I1, I2, pm, &tr);
+            int Iv[3] = {I0, I1, I2};
+            for (int d = 2; d >= 0; d--) {
+                v3 v = tr.vw[d];
+                v3 vmid = v;
+                int mode = 0;
+                float lin = 0.f, lit = 1.f, c = 0.f, sarg = 0.f;
+                v3 vit = mk(0.f, 0.f, 0.f);
+                if (Iv[d] < 3 && comp(v, d) < 0.f) {
+                    if (d != 1 || S.gf_mode == 0) { mode = 1; setc(vmid, d, 0.f); }
+                    else if (S.gf_mode == 1) {
+                        mode = 2;
+                        lin = v.y + 1e-30f;
+                        vit = mk(v.x - (float)I0 * 1e-30f, v.y - lin - (float)I1 * 1e-30f, v.z - (float)I2 * 1e-30f);
+                        lit = sqrtf(dot(vit, vit) + 1e-8f);
+                        sarg = 1.f + S.gf * lin / lit;
+                        c = tmax(sarg, 0.f);
+                        vmid = mk(c * (vit.x + (float)I0 * 1e-30f), 0.f, c * (vit.z + (float)I2 * 1e-30f));
+                    } else { mode = 3; vmid = mk(0.f, 0.f, 0.f); }
+                }
+                if (Iv[d] > S.n_grid - 3 && comp(vmid, d) > 0.f) setc(g, d, 0.f);
+                if (mode == 1) setc(g, d, 0.f);
+                else if (mode == 3) g = mk(0.f, 0.f, 0.f);
+                else if (mode == 2) {
+                    g.y = 0.f;
+                    v3 uu = mk(vit.x + (float)I0 * 1e-30f, vit.y + (float)I1 * 1e-30f, vit.z + (float)I2 * 1e-30f);
+                    float gc = dot(g, uu);
+                    v3 gvit = c * g;
+                    float gs = (0.f < sarg) ? gc : 0.f;
+                    float glin = gs * S.gf / lit;
+                    float glit = -gs * S.gf * lin / (lit * lit);
+                    gvit += (glit / lit) * vit;
+                    glin -= gvit.y;
+                    g = gvit;
+                    g.y += glin;
+                }
+            }
+        }
+        // collision units in reverse order; each unit's primitive adjoint is wave-reduced
+        v3 x = mk((float)I0 * S.dx, (float)I1 * S.dx, (float)I2 * S.dx);
+        int uidx = 0;
+        for (int q = 0; q < S.n_prims; q++) { Unit u[2]; uidx += prim_units(ga.prims[S.prim_off + q], u); }
+        for (int q = S.n_prims - 1; q >= 0; q--) {
+            const PrimC& P = ga.prims[S.prim_off + q];
+            PS s0 = load_ps(ps0 + q * 8), s1 = load_ps(ps1 + q * 8);
+            Unit u[2];
+            int m = prim_units(P, u);
+            for (int j = m - 1; j >= 0; j--) {
+                uidx--;
+                PSg g0 = zero_psg(), g1 = zero_psg();
+                if (act && uidx < MAX_UNITS) g = unit_collide_rev(P, u[j], s0, s1, x, S.dt, tr.vu[uidx], g, &g0, &g1);
+                float vals[16] = {g0.pos.x, g0.pos.y, g0.pos.z, g0.rot.w, g0.rot.x, g0.rot.y, g0.rot.z, g0.gap,
+                                  g1.pos.x, g1.pos.y, g1.pos.z, g1.rot.w, g1.rot.x, g1.rot.y, g1.rot.z, g1.gap};
+                bool any = false;
+#pragma unroll
+                for (int k = 0; k < 16; k++) any |= (vals[k] != 0.f);
+                if (__ballot(any) == 0ull) continue;
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    float s = wave_sum(vals[k]);
+                    if (lane == 0 && s != 0.f) unsafeAtomicAdd((k < 8 ? gps0 : gps1) + q * 8 + (k & 7), s);
+                }
+            }
+        }
+        float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (act) {
+            float inv_m = 1.f / pm.w;
+            out.x = inv_m * g.x; out.y = inv_m * g.y; out.z = inv_m * g.z;
+            out.w = -(g.x * pm.x + g.y * pm.y + g.z * pm.z) / (pm.w * pm.w);
+        }
+        g_adjpm[node] = out;
+    }
+}
+
+// backward_svd (mpm_simulator.py:136-156) with the gap clamp of :184-192
+DEV float gap_clamp(float a) { return a >= 0.f ? tmax(a, 1e-6f) : tmin(a, -1e-6f); }
+DEV m3 backward_svd(const m3& gu, const float gs[3], const m3& gv, const m3& u, const float sig[3], const m3& v) {
+    float s[3] = {sig[0] * sig[0], sig[1] * sig[1], sig[2] * sig[2]};
+    m3 Fm;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) Fm.a[i * 3 + j] = (i == j) ? 0.f : 1.f / gap_clamp(s[j] - s[i]);
+    m3 S = mzero(), GS = mzero();
+#pragma unroll
+    for (int i = 0; i < 3; i++) { S.a[i * 4] = sig[i]; GS.a[i * 4] = gs[i]; }
+    m3 sigma_term = mmul_nt(mmul(u, GS), v);
+    m3 t1 = mmul_tn(u, gu), t2 = mmul_tn(gu, u), M;
+#pragma unroll
+    for (int i = 0; i < 9; i++) M.a[i] = Fm.a[i] * (t1.a[i] - t2.a[i]);
+    m3 u_term = mmul_nt(mmul(u, mmul(M, S)), v);
+    t1 = mmul_tn(v, gv);
+    t2 = mmul_tn(gv, v);
+#pragma unroll
+    for (int i = 0; i < 9; i++) M.a[i] = Fm.a[i] * (t1.a[i] - t2.a[i]);
+    m3 v_term = mmul(u, mmul(S, mmul_nt(M, v)));
+    m3 out;
+#pragma unroll
+    for (int i = 0; i < 9; i++) out.a[i] = u_term.a[i] + v_term.a[i] + sigma_term.a[i];
+    return out;
+}
+
+// reverse of p2g + svd_grad + compute_F_tmp.grad (mpm_simulator.py:121-156,198-225);
+// block 0 also runs the primitive FK adjoint of substep f.
+__global__ void __launch_bounds__(PPB) k_p2g_bwd(ChunkArgs ca, const float* __restrict__ st, const float* __restrict__ a1,
+                                                 float* __restrict__ a0, const float4* __restrict__ g_adjpm,
+                                                 const PrimC* __restrict__ prims, int n_prims_total, const float* __restrict__ pstate,
+                                                 const float* __restrict__ pvel, float* __restrict__ gpstate,
+                                                 float* __restrict__ gpvel, int f) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < n_prims_total) {
+        int q = threadIdx.x;
+        size_t o0 = ((size_t)f * n_prims_total + q) * 8, o1 = ((size_t)(f + 1) * n_prims_total + q) * 8;
+        prim_fk_rev(prims[q], pstate + o0, pvel + o0, gpstate + o1, gpstate + o0, gpvel + o0);
+    }
+    int4 ch = ca.chunks[blockIdx.x];
+    int i = threadIdx.x;
+    if (i >= ch.z) return;
+    const SceneC& S = ca.scenes[ch.x];
+    int slot = ch.y + i;
+    int stride = ca.stride;
+    v3 x = load3(st, stride, 0, slot);
+    v3 v = load3(st, stride, 3, slot);
+    m3 C = load9(st, stride, 6, slot);
+    m3 F = load9(st, stride, 15, slot);
+    PTrace t;
+    particle_stress(S, C, F, t);
+    Stencil sw;
+    stencil(S, x, sw);
+    v3 gv = mk(0.f, 0.f, 0.f);
+    m3 gaff = mzero();
+    float gfx[3] = {0.f, 0.f, 0.f};
+    v3 mv = S.p_mass * v;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                int I0 = sw.base[0] + a, I1 = sw.base[1] + b, I2 = sw.base[2] + c;
+                if (!in_grid(S, I0, I1, I2)) continue;
+                float4 gg = g_adjpm[node_id(S, I0, I1, I2)];
+                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
+                float dp[3] = {((float)a - sw.fx[0]) * S.dx, ((float)b - sw.fx[1]) * S.dx, ((float)c - sw.fx[2]) * S.dx};
+                const float* A = t.affine.a;
+                v3 contrib = mk(mv.x + A[0] * dp[0] + A[1] * dp[1] + A[2] * dp[2], mv.y + A[3] * dp[0] + A[4] * dp[1] + A[5] * dp[2],
+                                mv.z + A[6] * dp[0] + A[7] * dp[1] + A[8] * dp[2]);
+                v3 g = mk(gg.x, gg.y, gg.z);
+                float gw = dot(g, contrib) + gg.w * S.p_mass;
+                gv += (wt * S.p_mass) * g;
+                v3 wg = wt * g;
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    gaff.a[j] += wg.x * dp[j];
+                    gaff.a[3 + j] += wg.y * dp[j];
+                    gaff.a[6 + j] += wg.z * dp[j];
+                }
+                v3 gdpos = mk(wg.x * A[0] + wg.y * A[3] + wg.z * A[6], wg.x * A[1] + wg.y * A[4] + wg.z * A[7],
+                              wg.x * A[2] + wg.y * A[5] + wg.z * A[8]);
+                int off[3] = {a, b, c};
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    float dwd = sw.dw[off[d]][d];
+#pragma unroll
+                    for (int e2 = 0; e2 < 3; e2++)
+                        if (e2 != d) dwd *= sw.w[off[e2]][e2];
+                    gfx[d] += gw * dwd - comp(gdpos, d) * S.dx;
+                }
+            }
+    m3 gC, gst;
+#pragma unroll
+    for (int k = 0; k < 9; k++) { gC.a[k] = S.p_mass * gaff.a[k]; gst.a[k] = S.stress_k * gaff.a[k]; }
+    m3 FmR;
+#pragma unroll
+    for (int k = 0; k < 9; k++) FmR.a[k] = t.Fn.a[k] - t.R.a[k];
+    m3 sF = mmul(gst, t.Fn), sT = mmul_tn(gst, FmR);
+    m3 gF1 = load9(a1, stride, 15, slot);
+    m3 gF, gR;
+#pragma unroll
+    for (int k = 0; k < 9; k++) { gF.a[k] = 2.f * S.mu * (sF.a[k] + sT.a[k]) + gF1.a[k]; gR.a[k] = -2.f * S.mu * sF.a[k]; }
+    float gJ = S.lam * (2.f * t.J - 1.f) * (gst.a[0] + gst.a[4] + gst.a[8]);
+    m3 cf = mcof(t.Fn);
+#pragma unroll
+    for (int k = 0; k < 9; k++) gF.a[k] += gJ * cf.a[k];
+    m3 gU = mmul(gR, t.V), gV = mmul_tn(gR, t.U), gFt = mzero();
+    float gS[3] = {0.f, 0.f, 0.f};
+    if (t.yield) {
+        m3 D = mzero();
+#pragma unroll
+        for (int k = 0; k < 3; k++) D.a[k * 4] = t.e[k];
+        m3 a = mmul(mmul(gF, t.V), D), b = mmul(mmul_tn(gF, t.U), D);
+#pragma unroll
+        for (int k = 0; k < 9; k++) { gU.a[k] += a.a[k]; gV.a[k] += b.a[k]; }
+        m3 M = mmul(mmul_tn(t.U, gF), t.V);
+        float gep[3], geps[3], geh[3];
+        float r = t.dg / t.ehn, gr = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { gep[k] = M.a[k * 4] * t.e[k]; geps[k] = gep[k]; geh[k] = -r * gep[k]; gr -= gep[k] * t.eh[k]; }
+        float gdg = gr / t.ehn;
+        float gehn = -gr * t.dg / (t.ehn * t.ehn) + gdg;
+#pragma unroll
+        for (int k = 0; k < 3; k++) geh[k] += gehn * t.eh[k] / t.ehn;
+        float mg = (geh[0] + geh[1] + geh[2]) / 3.f;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            geps[k] += geh[k] - mg;
+            if (0.05f < t.sig[k]) gS[k] += geps[k] / t.sc[k];
+        }
+    } else {
+        gFt = gF;
+    }
+    m3 bs = backward_svd(gU, gS, gV, t.U, t.sig, t.V);
+#pragma unroll
+    for (int k = 0; k < 9; k++) gFt.a[k] += bs.a[k];
+    m3 Am;
+#pragma unroll
+    for (int k = 0; k < 9; k++) Am.a[k] = ((k % 4 == 0) ? 1.f : 0.f) + S.dt * C.a[k];
+    m3 gCa = mmul_nt(gFt, F), gF0 = mmul_tn(Am, gFt);
+#pragma unroll
+    for (int k = 0; k < 9; k++) gC.a[k] += S.dt * gCa.a[k];
+    v3 gx0 = load3(a0, stride, 0, slot) + S.inv_dx * mk(gfx[0], gfx[1], gfx[2]);
+    store3(a0, stride, 0, slot, gx0);
+    store3(a0, stride, 3, slot, gv);
+    store9(a0, stride, 6, slot, gC);
+    store9(a0, stride, 15, slot, gF0);
+}
+
+// compute_grid_m_kernel (mpm_simulator.py:456-466): dense per-scene output
+__global__ void __launch_bounds__(PPB) k_grid_mass(ChunkArgs ca, const float* __restrict__ st, float* __restrict__ out) {
+    int4 ch = ca.chunks[blockIdx.x];
+    int i = threadIdx.x;
+    if (i >= ch.z) return;
+    const SceneC& S = ca.scenes[ch.x];
+    int slot = ch.y + i;
+    v3 x = load3(st, ca.stride, 0, slot);
+    Stencil sw;
+    stencil(S, x, sw);
+    float* o = out + (size_t)S.m_off;
+    int n = S.n_grid;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                int I0 = sw.base[0] + a, I1 = sw.base[1] + b, I2 = sw.base[2] + c;
+                if (!in_grid(S, I0, I1, I2)) continue;
+                unsafeAtomicAdd(o + ((size_t)I0 * n + I1) * n + I2, sw.w[a][0] * sw.w[b][1] * sw.w[c][2] * S.p_mass);
+            }
+}
+__global__ void __launch_bounds__(PPB) k_grid_mass_bwd(ChunkArgs ca, const float* __restrict__ st, const float* __restrict__ gm,
+                                                       float* __restrict__ adj) {
+    int4 ch = ca.chunks[blockIdx.x];
+    int i = threadIdx.x;
+    if (i >= ch.z) return;
+    const SceneC& S = ca.scenes[ch.x];
+    int slot = ch.y + i;
+    v3 x = load3(st, ca.stride, 0, slot);
+    Stencil sw;
+    stencil(S, x, sw);
+    const float* g = gm + (size_t)S.m_off;
+    int n = S.n_grid;
+    float gfx[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                int I0 = sw.base[0] + a, I1 = sw.base[1] + b, I2 = sw.base[2] + c;
+                if (!in_grid(S, I0, I1, I2)) continue;
+                float gg = g[((size_t)I0 * n + I1) * n + I2] * S.p_mass;
+                int off[3] = {a, b, c};
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    float dwd = sw.dw[off[d]][d];
+#pragma unroll
+                    for (int e2 = 0; e2 < 3; e2++)
+                        if (e2 != d) dwd *= sw.w[off[e2]][e2];
+                    gfx[d] += gg * dwd;
+                }
+            }
+    v3 gx = load3(adj, ca.stride, 0, slot) + S.inv_dx * mk(gfx[0], gfx[1], gfx[2]);
+    store3(adj, ca.stride, 0, slot, gx);
+}
+
+// reset_kernel (mpm_simulator.py:402-410)
+__global__ void k_reset_frame0(int stride, const float* __restrict__ x, float* __restrict__ st) {
+    int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= stride) return;
+    for (int r = 0; r < 3; r++) st[(size_t)r * stride + slot] = x[(size_t)r * stride + slot];
+    for (int r = 3; r < NST; r++) {
+        int k = r - 15;
+        st[(size_t)r * stride + slot] = (r >= 15 && (k == 0 || k == 4 || k == 8)) ? 1.f : 0.f;
+    }
+}
+// GradModel._set_obs_grad (function.py:130-142)
+__global__ void k_obs_grad(int stride, int n_prims_total, const float* __restrict__ xv, const float* __restrict__ pg,
+                           float* __restrict__ adj, float* __restrict__ gps) {
+    int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot < stride && xv) {
+        for (int r = 0; r < 6; r++) adj[(size_t)r * stride + slot] += xv[(size_t)slot * 6 + r];
+    }
+    if (pg && slot < n_prims_total * 8) gps[slot] += pg[slot];
+}
+// set_action + set_velocity for env step s (primive_base.py:260-274)
+__global__ void k_set_actions(const PrimC* __restrict__ prims, int n_prims_total, int s, int n_sub, const float* __restrict__ act,
+                              float* __restrict__ pvel) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int q = t % max(n_prims_total, 1), j = t / max(n_prims_total, 1);
+    if (n_prims_total == 0 || j >= n_sub) return;
+    const PrimC& P = prims[q];
+    float* o = pvel + ((size_t)(s * n_sub + j) * n_prims_total + q) * 8;
+    if (P.action_dim <= 0) return;
+    const float* a = act + q * 7;
+    float ns = (float)n_sub;
+    for (int k = 0; k < 3; k++) o[k] = a[k] * P.scale[k] / ns;
+    if (P.action_dim > 3)
+        for (int k = 3; k < 6; k++) o[k] = a[k] * P.scale[k] / ns;
+    if (P.action_dim > 6) o[6] = a[6] * P.scale[6] / ns;
+}
+// set_velocity.grad + get_action_grad (function.py:180-190)
+__global__ void k_action_grad(const PrimC* __restrict__ prims, int n_prims_total, int s, int n_sub, const float* __restrict__ gpvel,
+                              float* __restrict__ out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_prims_total * 7) return;
+    int q = t / 7, k = t % 7;
+    const PrimC& P = prims[q];
+    float acc = 0.f;
+    bool used = (k < 3) || (k < 6 && P.action_dim > 3) || (k == 6 && P.action_dim > 6);
+    if (P.action_dim > 0 && used)
+        for (int j = 0; j < n_sub; j++) acc += gpvel[((size_t)(s * n_sub + j) * n_prims_total + q) * 8 + k];
+    out[t] = (P.action_dim > 0 && used) ? acc * P.scale[k] / (float)n_sub : 0.f;
+}
+__global__ void k_fill(float* __restrict__ p, size_t n, float v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+}  // namespace
+
+// ==================================================================== host side
+struct mpm_ctx {
+    int n_scenes = 0, n_prims = 0, max_frames = 0, K = 0, n_ckpt = 0, ring = 0;
+    int stride = 0, n_chunks = 0, n_blocks = 0;
+    long long n_nodes = 0;
+    std::vector<SceneC> scenes;
+    std::vector<PrimC> prims;
+    std::vector<int> p_off;
+    SceneC* d_scenes = nullptr;
+    PrimC* d_prims = nullptr;
+    int4* d_chunks = nullptr;
+    int* d_blk_scene = nullptr;
+    float* ckpt = nullptr;
+    float* ringbuf = nullptr;
+    float* adj[2] = {nullptr, nullptr};
+    float4 *g_pm = nullptr, *g_v = nullptr, *g_adjv = nullptr, *g_adjpm = nullptr;
+    int *blk_flag = nullptr, *active = nullptr, *counters = nullptr;
+    float *pstate = nullptr, *pstate_g = nullptr, *pvel = nullptr, *pvel_g = nullptr;
+    int valid_hi = -1;    // frames > valid_hi are stale
+    int seg_valid = -1;   // segment whose ring frames are resident
+    int seg_lo = -1;      // lowest resident ring frame of that segment
+    int seg_hi = -1;      // highest resident ring frame of that segment
+    int adj_top = -1;     // frame whose adjoint is current
+    int grid_ctas = 1024; // persistent grid-phase workgroups
+
+    size_t frame_floats() const { return (size_t)NST * stride; }
+    float* frame_ptr(int f) const {
+        if (f % K == 0) return ckpt + (size_t)(f / K) * frame_floats();
+        return ringbuf + (size_t)((f % K) - 1) * frame_floats();
+    }
+    float* adj_ptr(int f) const { return adj[f & 1]; }
+    ChunkArgs cargs() const { ChunkArgs a; a.scenes = d_scenes; a.chunks = d_chunks; a.n_chunks = n_chunks; a.stride = stride; return a; }
+    GridArgs gargs(int f) const {
+        GridArgs g;
+        g.scenes = d_scenes; g.prims = d_prims; g.blk_scene = d_blk_scene; g.active = active; g.counters = counters;
+        g.pstate = pstate; g.n_prims_total = n_prims; g.f = f;
+        return g;
+    }
+};
+
+namespace {
+
+struct Layout {
+    size_t off_scenes, off_prims, off_chunks, off_blk_scene, off_ckpt, off_ring, off_adj, off_grid, off_flags, off_active,
+        off_counters, off_prim_bufs, total;
+    int stride, n_chunks, n_blocks, n_ckpt, ring, K;
+    long long n_nodes;
+    int n_prims;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k) {
+    Layout L;
+    memset(&L, 0, sizeof(L));
+    int stride = 0, n_chunks = 0, n_blocks = 0, n_prims = 0;
+    long long nodes = 0;
+    for (int i = 0; i < n; i++) {
+        int np = std::max(sc[i].n_particles, 0);
+        stride += (np + 63) & ~63;
+        n_chunks += (np + PPB - 1) / PPB;
+        int nb = (sc[i].n_grid + 3) / 4;
+        n_blocks += nb * nb * nb;
+        nodes += (long long)nb * nb * nb * NODES_PER_BLK;
+        n_prims += sc[i].n_prims;
+    }
+    stride = std::max(64, (stride + 255) & ~255);
+    int K = (ckpt_k <= 0 || ckpt_k >= max_frames) ? max_frames : ckpt_k;
+    L.K = K;
+    L.n_ckpt = (max_frames - 1) / K + 1;
+    L.ring = std::max(1, std::min(K - 1, max_frames - 1));
+    L.stride = stride;
+    L.n_chunks = std::max(1, n_chunks);
+    L.n_blocks = std::max(1, n_blocks);
+    L.n_nodes = std::max(64LL, nodes);
+    L.n_prims = n_prims;
+    size_t o = 0;
+    L.off_scenes = o; o = align256(o + sizeof(SceneC) * std::max(1, n));
+    L.off_prims = o; o = align256(o + sizeof(PrimC) * std::max(1, n_prims));
+    L.off_chunks = o; o = align256(o + sizeof(int4) * L.n_chunks);
+    L.off_blk_scene = o; o = align256(o + sizeof(int) * L.n_blocks);
+    size_t ff = (size_t)NST * stride * sizeof(float);
+    L.off_ckpt = o; o = align256(o + ff * L.n_ckpt);
+    L.off_ring = o; o = align256(o + ff * L.ring);
+    L.off_adj = o; o = align256(o + ff * 2);
+    L.off_grid = o; o = align256(o + sizeof(float4) * (size_t)L.n_nodes * 4);
+    L.off_flags = o; o = align256(o + sizeof(int) * L.n_blocks);
+    L.off_active = o; o = align256(o + sizeof(int) * L.n_blocks);
+    L.off_counters = o; o = align256(o + sizeof(int) * 64);
+    L.off_prim_bufs = o; o = align256(o + sizeof(float) * 8 * (size_t)max_frames * std::max(1, n_prims) * 4);
+    L.total = o;
+    return L;
+}
+
+hipStream_t S_(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int launch_fill(float* p, size_t n, hipStream_t st) {
+    if (n == 0) return 0;
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, st, p, n, 0.f);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// forward substep f -> f+1 (writes frame f+1 wherever its slot is)
+int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
+    const float* in = c->frame_ptr(f);
+    float* out = c->frame_ptr(f + 1);
+    ChunkArgs ca = c->cargs();
+    hipLaunchKernelGGL(k_mark, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, c->blk_flag, c->counters, 1, c->d_prims, c->n_prims,
+                       c->pstate, c->pvel, f);
+    hipLaunchKernelGGL(k_compact, dim3(std::min(4096, (c->n_blocks + 255) / 256)), dim3(256), 0, st, c->n_blocks, c->blk_flag,
+                       c->active, c->counters);
+    hipLaunchKernelGGL(k_p2g, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, out, c->g_pm);
+    hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->g_v, 1);
+    hipLaunchKernelGGL(k_g2p, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, out, c->g_v);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int ensure_frame(mpm_ctx* c, int f, hipStream_t st) {
+    if (f < 0 || f >= c->max_frames) return fail("frame " + std::to_string(f) + " outside [0, max_frames)");
+    if (f > c->valid_hi) return fail("frame " + std::to_string(f) + " has not been simulated");
+    if (f % c->K == 0) return 0;
+    int a = f / c->K;
+    if (c->seg_valid == a && c->seg_lo <= f && f <= c->seg_hi) return 0;
+    // recompute the segment from its checkpoint (never rewrites a checkpoint)
+    int start = a * c->K;
+    for (int s = start; s < f; s++)
+        if (run_fwd(c, s, st)) return -1;
+    c->seg_valid = a;
+    c->seg_lo = start + 1;
+    c->seg_hi = f;
+    return 0;
+}
+
+int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
+    if (ensure_frame(c, f + 1, st)) return -1;
+    if (ensure_frame(c, f, st)) return -1;
+    const float* in = c->frame_ptr(f);
+    const float* in1 = c->frame_ptr(f + 1);
+    const float* a1 = c->adj_ptr(f + 1);
+    float* a0 = c->adj_ptr(f);
+    ChunkArgs ca = c->cargs();
+    hipLaunchKernelGGL(k_mark, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, c->blk_flag, c->counters, 0, c->d_prims, c->n_prims,
+                       c->pstate, c->pvel, f);
+    hipLaunchKernelGGL(k_compact, dim3(std::min(4096, (c->n_blocks + 255) / 256)), dim3(256), 0, st, c->n_blocks, c->blk_flag,
+                       c->active, c->counters);
+    hipLaunchKernelGGL(k_p2g, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, (float*)nullptr, c->g_pm);
+    hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->g_v, 0);
+    hipLaunchKernelGGL(k_g2p_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, in1, a1, a0, c->g_v, c->g_adjv);
+    hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->g_adjv, c->g_adjpm, c->pstate_g);
+    hipLaunchKernelGGL(k_p2g_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, a1, a0, c->g_adjpm, c->d_prims, c->n_prims, c->pstate,
+                       c->pvel, c->pstate_g, c->pvel_g, f);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+PrimC to_primc(const mpm_prim_desc& d) {
+    PrimC p;
+    memset(&p, 0, sizeof(p));
+    p.type = d.type;
+    p.action_dim = d.action_dim;
+    switch (d.type) {
+    case MPM_SPHERE: p.unit_shape = SH_SPHERE; break;
+    case MPM_CAPSULE: case MPM_ROLLINGPIN: case MPM_ROLLINGPIN_EXT: case MPM_GRIPPER2: p.unit_shape = SH_CAPSULE; break;
+    case MPM_BOX: case MPM_GRIPPER: p.unit_shape = SH_BOX; break;
+    case MPM_PRISM: p.unit_shape = SH_PRISM; break;
+    case MPM_KNIFE: p.unit_shape = SH_KNIFE; break;
+    case MPM_CHOPSTICKS: p.unit_shape = SH_CHOP; break;
+    default: p.unit_shape = SH_BOX;
+    }
+    p.n_units = (d.type == MPM_GRIPPER || d.type == MPM_GRIPPER2) ? 2 : 1;
+    p.friction = (float)d.friction;
+    p.softness = (float)d.softness;
+    p.h = (float)d.h;
+    p.r = (float)d.r;
+    for (int i = 0; i < 3; i++) { p.size[i] = (float)d.size[i]; p.lo[i] = (float)d.lo[i]; p.hi[i] = (float)d.hi[i]; }
+    for (int i = 0; i < 2; i++) p.ph[i] = (float)d.ph[i];
+    for (int i = 0; i < 4; i++) p.prot[i] = (float)d.prot[i];
+    p.min_gap = (float)d.min_gap;
+    p.max_gap = (float)d.max_gap;
+    for (int i = 0; i < 7; i++) p.scale[i] = (float)d.action_scale[i];
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mpm_last_error(void) { return g_err.c_str(); }
+
+int64_t mpm_workspace_bytes(const mpm_scene_desc* scenes, int n_scenes, int max_frames, int ckpt_k) {
+    if (!scenes || n_scenes <= 0 || max_frames < 2) return -1;
+    return (int64_t)make_layout(scenes, n_scenes, max_frames, ckpt_k).total;
+}
+
+int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* prims, int max_frames, int ckpt_k, void* workspace,
+               int64_t workspace_bytes, void* stream, mpm_ctx** out) {
+    if (!out) return fail("mpm_create: out is null");
+    *out = nullptr;
+    if (!scenes || n_scenes <= 0) return fail("mpm_create: no scenes");
+    if (max_frames < 2) return fail("mpm_create: max_frames must be >= 2");
+    for (int i = 0; i < n_scenes; i++) {
+        if (scenes[i].n_grid < 8 || scenes[i].n_grid > 1024) return fail("mpm_create: n_grid outside [8, 1024]");
+        if (scenes[i].n_particles < 0) return fail("mpm_create: negative particle count");
+        if (scenes[i].n_prims < 0 || scenes[i].n_prims > 4) return fail("mpm_create: at most 4 primitives per scene");
+    }
+    Layout L = make_layout(scenes, n_scenes, max_frames, ckpt_k);
+    if (!workspace || workspace_bytes < (int64_t)L.total)
+        return fail("mpm_create: workspace smaller than mpm_workspace_bytes()");
+    hipStream_t st = S_(stream);
+    mpm_ctx* c = new mpm_ctx();
+    c->n_scenes = n_scenes;
+    c->n_prims = L.n_prims;
+    c->max_frames = max_frames;
+    c->K = L.K;
+    c->n_ckpt = L.n_ckpt;
+    c->ring = L.ring;
+    c->stride = L.stride;
+    c->n_chunks = L.n_chunks;
+    c->n_blocks = L.n_blocks;
+    c->n_nodes = L.n_nodes;
+    char* w = static_cast<char*>(workspace);
+    c->d_scenes = (SceneC*)(w + L.off_scenes);
+    c->d_prims = (PrimC*)(w + L.off_prims);
+    c->d_chunks = (int4*)(w + L.off_chunks);
+    c->d_blk_scene = (int*)(w + L.off_blk_scene);
+    c->ckpt = (float*)(w + L.off_ckpt);
+    c->ringbuf = (float*)(w + L.off_ring);
+    c->adj[0] = (float*)(w + L.off_adj);
+    c->adj[1] = c->adj[0] + c->frame_floats();
+    float4* g = (float4*)(w + L.off_grid);
+    c->g_pm = g;
+    c->g_v = g + L.n_nodes;
+    c->g_adjv = g + 2 * L.n_nodes;
+    c->g_adjpm = g + 3 * L.n_nodes;
+    c->blk_flag = (int*)(w + L.off_flags);
+    c->active = (int*)(w + L.off_active);
+    c->counters = (int*)(w + L.off_counters);
+    size_t pb = (size_t)8 * max_frames * std::max(1, L.n_prims);
+    c->pstate = (float*)(w + L.off_prim_bufs);
+    c->pstate_g = c->pstate + pb;
+    c->pvel = c->pstate + 2 * pb;
+    c->pvel_g = c->pstate + 3 * pb;
+
+    // host tables
+    std::vector<int4> chunks;
+    std::vector<int> blk_scene(L.n_blocks, 0);
+    int poff = 0, boff = 0, prim_off = 0, moff = 0;
+    long long goff = 0;
+    for (int i = 0; i < n_scenes; i++) {
+        const mpm_scene_desc& d = scenes[i];
+        SceneC s;
+        memset(&s, 0, sizeof(s));
+        s.n_grid = d.n_grid;
+        s.nb = (d.n_grid + 3) / 4;
+        s.p_off = poff;
+        s.p_cnt = d.n_particles;
+        s.blk_off = boff;
+        s.prim_off = prim_off;
+        s.n_prims = d.n_prims;
+        s.gf_mode = d.ground_friction == 0 ? 0 : (d.ground_friction < 10 ? 1 : 2);
+        s.m_off = moff;
+        s.chunk_off = (int)chunks.size();
+        s.g_off = goff;
+        s.dx = (float)d.dx;
+        s.inv_dx = (float)d.inv_dx;
+        s.dt = (float)d.dt;
+        s.p_mass = (float)d.p_mass;
+        s.mu = (float)d.mu;
+        s.lam = (float)d.lam;
+        s.ys = (float)d.yield_stress;
+        s.stress_k = (float)(-d.dt * d.p_vol * 4 * d.inv_dx * d.inv_dx);
+        s.c4 = (float)(4 * d.inv_dx);
+        s.lower = (float)(d.lower_bound * d.dx);
+        s.upper = (float)(1. - 3 * d.dx);
+        s.gf = (float)d.ground_friction;
+        for (int k = 0; k < 3; k++) s.g30[k] = (float)d.dt * (float)d.gravity[k] * 30.f;
+        for (int p = 0; p < d.n_particles; p += PPB) {
+            int4 ch;
+            ch.x = i; ch.y = poff + p; ch.z = std::min(PPB, d.n_particles - p); ch.w = 0;
+            chunks.push_back(ch);
+        }
+        s.n_chunks = (int)chunks.size() - s.chunk_off;
+        int nbk = s.nb * s.nb * s.nb;
+        for (int b = 0; b < nbk; b++) blk_scene[boff + b] = i;
+        c->scenes.push_back(s);
+        c->p_off.push_back(poff);
+        poff += (d.n_particles + 63) & ~63;
+        boff += nbk;
+        goff += (long long)nbk * NODES_PER_BLK;
+        prim_off += d.n_prims;
+        moff += d.n_grid * d.n_grid * d.n_grid;
+    }
+    if (chunks.empty()) { int4 z; z.x = 0; z.y = 0; z.z = 0; z.w = 0; chunks.push_back(z); }
+    for (int q = 0; q < L.n_prims; q++) c->prims.push_back(to_primc(prims[q]));
+    if (c->prims.empty()) { PrimC z; memset(&z, 0, sizeof(z)); c->prims.push_back(z); }
+    c->n_chunks = (int)chunks.size();
+    hipError_t e = hipSuccess;
+    e = (e == hipSuccess) ? hipMemsetAsync(workspace, 0, L.total, st) : e;
+    e = (e == hipSuccess) ? hipMemcpyAsync(c->d_scenes, c->scenes.data(), sizeof(SceneC) * c->scenes.size(), hipMemcpyHostToDevice, st) : e;
+    e = (e == hipSuccess) ? hipMemcpyAsync(c->d_prims, c->prims.data(), sizeof(PrimC) * c->prims.size(), hipMemcpyHostToDevice, st) : e;
+    e = (e == hipSuccess) ? hipMemcpyAsync(c->d_chunks, chunks.data(), sizeof(int4) * chunks.size(), hipMemcpyHostToDevice, st) : e;
+    e = (e == hipSuccess) ? hipMemcpyAsync(c->d_blk_scene, blk_scene.data(), sizeof(int) * blk_scene.size(), hipMemcpyHostToDevice, st) : e;
+    e = (e == hipSuccess) ? hipStreamSynchronize(st) : e;
+    if (e != hipSuccess) {
+        delete c;
+        return fail(std::string("mpm_create: ") + hipGetErrorString(e));
+    }
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+    }
+    c->grid_ctas = std::max(64, ncu * 8);
+    *out = c;
+    return 0;
+}
+
+int mpm_destroy(mpm_ctx* ctx) {
+    delete ctx;
+    return 0;
+}
+
+int mpm_stride(mpm_ctx* ctx) { return ctx ? ctx->stride : -1; }
+int mpm_particle_offset(mpm_ctx* ctx, int scene) {
+    if (!ctx || scene < 0 || scene >= ctx->n_scenes) return fail("mpm_particle_offset: bad scene");
+    return ctx->p_off[scene];
+}
+
+int mpm_frame(mpm_ctx* ctx, int f, void* stream, float** ptr) {
+    if (!ctx || !ptr) return fail("mpm_frame: null argument");
+    if (ensure_frame(ctx, f, S_(stream))) return -1;
+    *ptr = ctx->frame_ptr(f);
+    return 0;
+}
+
+int mpm_adjoint(mpm_ctx* ctx, int f, float** ptr) {
+    if (!ctx || !ptr) return fail("mpm_adjoint: null argument");
+    if (f != ctx->adj_top) return fail("mpm_adjoint: frame " + std::to_string(f) + " is not the current adjoint frame " + std::to_string(ctx->adj_top));
+    *ptr = ctx->adj_ptr(f);
+    return 0;
+}
+
+int mpm_prim_buffers(mpm_ctx* ctx, float** state, float** state_grad, float** vel, float** vel_grad) {
+    if (!ctx) return fail("mpm_prim_buffers: null context");
+    if (state) *state = ctx->pstate;
+    if (state_grad) *state_grad = ctx->pstate_g;
+    if (vel) *vel = ctx->pvel;
+    if (vel_grad) *vel_grad = ctx->pvel_g;
+    return 0;
+}
+
+// frame f was produced by a forward substep from frame f - 1
+static void mark_simulated(mpm_ctx* c, int f) {
+    c->valid_hi = f;
+    if (f % c->K == 0) return;
+    if (c->seg_valid == f / c->K && c->seg_hi == f - 1) c->seg_hi = f;
+    else { c->seg_valid = f / c->K; c->seg_lo = f; c->seg_hi = f; }
+}
+// frame f was written directly: the trajectory restarts there
+static void mark_set(mpm_ctx* c, int f) {
+    c->valid_hi = f;
+    if (f % c->K == 0) { c->seg_valid = -1; c->seg_lo = c->seg_hi = -1; }
+    else { c->seg_valid = f / c->K; c->seg_lo = f; c->seg_hi = f; }
+}
+
+int mpm_set_frame(mpm_ctx* ctx, int f, const float* src, void* stream) {
+    if (!ctx || !src) return fail("mpm_set_frame: null argument");
+    if (f < 0 || f >= ctx->max_frames) return fail("mpm_set_frame: frame out of range");
+    HIPCHK(hipMemcpyAsync(ctx->frame_ptr(f), src, ctx->frame_floats() * sizeof(float), hipMemcpyDeviceToDevice, S_(stream)));
+    mark_set(ctx, f);
+    return 0;
+}
+
+int mpm_get_frame(mpm_ctx* ctx, int f, float* dst, void* stream) {
+    if (!ctx || !dst) return fail("mpm_get_frame: null argument");
+    if (ensure_frame(ctx, f, S_(stream))) return -1;
+    HIPCHK(hipMemcpyAsync(dst, ctx->frame_ptr(f), ctx->frame_floats() * sizeof(float), hipMemcpyDeviceToDevice, S_(stream)));
+    return 0;
+}
+
+int mpm_reset_frame0(mpm_ctx* ctx, const float* x, void* stream) {
+    if (!ctx || !x) return fail("mpm_reset_frame0: null argument");
+    hipLaunchKernelGGL(k_reset_frame0, dim3((ctx->stride + 255) / 256), dim3(256), 0, S_(stream), ctx->stride, x, ctx->frame_ptr(0));
+    HIPCHK(hipGetLastError());
+    mark_set(ctx, 0);
+    return 0;
+}
+
+int mpm_set_prim_frame(mpm_ctx* ctx, int f, const float* src, void* stream) {
+    if (!ctx || !src) return fail("mpm_set_prim_frame: null argument");
+    if (f < 0 || f >= ctx->max_frames) return fail("mpm_set_prim_frame: frame out of range");
+    if (ctx->n_prims == 0) return 0;
+    HIPCHK(hipMemcpyAsync(ctx->pstate + (size_t)f * ctx->n_prims * 8, src, sizeof(float) * 8 * ctx->n_prims, hipMemcpyDeviceToDevice,
+                          S_(stream)));
+    return 0;
+}
+
+int mpm_get_prim_frame(mpm_ctx* ctx, int f, float* dst, void* stream) {
+    if (!ctx || !dst) return fail("mpm_get_prim_frame: null argument");
+    if (f < 0 || f >= ctx->max_frames) return fail("mpm_get_prim_frame: frame out of range");
+    if (ctx->n_prims == 0) return 0;
+    HIPCHK(hipMemcpyAsync(dst, ctx->pstate + (size_t)f * ctx->n_prims * 8, sizeof(float) * 8 * ctx->n_prims, hipMemcpyDeviceToDevice,
+                          S_(stream)));
+    return 0;
+}
+
+int mpm_set_actions(mpm_ctx* ctx, int s, int n_substeps, const float* actions, void* stream) {
+    if (!ctx || !actions) return fail("mpm_set_actions: null argument");
+    if (n_substeps <= 0 || s < 0 || (long long)(s + 1) * n_substeps >= ctx->max_frames)
+        return fail("mpm_set_actions: env step outside the frame capacity");
+    if (ctx->n_prims == 0) return 0;
+    int n = ctx->n_prims * n_substeps;
+    hipLaunchKernelGGL(k_set_actions, dim3((n + 63) / 64), dim3(64), 0, S_(stream), ctx->d_prims, ctx->n_prims, s, n_substeps, actions,
+                       ctx->pvel);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mpm_substep_fwd(mpm_ctx* ctx, int s, int count, void* stream) {
+    if (!ctx) return fail("mpm_substep_fwd: null context");
+    if (count < 0 || s < 0 || s + count >= ctx->max_frames) return fail("mpm_substep_fwd: frames outside [0, max_frames)");
+    hipStream_t st = S_(stream);
+    if (count == 0) return 0;
+    if (ensure_frame(ctx, s, st)) return -1;
+    for (int f = s; f < s + count; f++) {
+        if (run_fwd(ctx, f, st)) return -1;
+        mark_simulated(ctx, f + 1);
+    }
+    return 0;
+}
+
+int mpm_substep_bwd(mpm_ctx* ctx, int s, int count, void* stream) {
+    if (!ctx) return fail("mpm_substep_bwd: null context");
+    if (count <= 0) return 0;
+    if (s < 0 || s + count > ctx->valid_hi) return fail("mpm_substep_bwd: substeps not simulated");
+    if (ctx->adj_top < 0) ctx->adj_top = s + count;
+    if (ctx->adj_top != s + count)
+        return fail("mpm_substep_bwd: adjoint order violated (current adjoint frame " + std::to_string(ctx->adj_top) + ", expected " +
+                    std::to_string(s + count) + ")");
+    hipStream_t st = S_(stream);
+    for (int f = s + count - 1; f >= s; f--) {
+        if (run_bwd(ctx, f, st)) return -1;
+        ctx->adj_top = f;
+    }
+    return 0;
+}
+
+int mpm_grid_mass(mpm_ctx* ctx, int f, float* out, void* stream) {
+    if (!ctx || !out) return fail("mpm_grid_mass: null argument");
+    hipStream_t st = S_(stream);
+    if (ensure_frame(ctx, f, st)) return -1;
+    hipLaunchKernelGGL(k_grid_mass, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mpm_grid_mass_bwd(mpm_ctx* ctx, int f, const float* g, void* stream) {
+    if (!ctx || !g) return fail("mpm_grid_mass_bwd: null argument");
+    if (ctx->adj_top < 0) ctx->adj_top = f;
+    if (f != ctx->adj_top) return fail("mpm_grid_mass_bwd: frame is not the current adjoint frame");
+    hipStream_t st = S_(stream);
+    if (ensure_frame(ctx, f, st)) return -1;
+    hipLaunchKernelGGL(k_grid_mass_bwd, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), g, ctx->adj_ptr(f));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mpm_add_obs_grad(mpm_ctx* ctx, int f, const float* xv_grad, const float* prim_grad, void* stream) {
+    if (!ctx) return fail("mpm_add_obs_grad: null context");
+    if (ctx->adj_top < 0) ctx->adj_top = f;
+    if (f != ctx->adj_top) return fail("mpm_add_obs_grad: frame is not the current adjoint frame");
+    int n = std::max(ctx->stride, ctx->n_prims * 8);
+    hipLaunchKernelGGL(k_obs_grad, dim3((n + 255) / 256), dim3(256), 0, S_(stream), ctx->stride, ctx->n_prims, xv_grad, prim_grad,
+                       ctx->adj_ptr(f), ctx->pstate_g + (size_t)f * ctx->n_prims * 8);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mpm_action_grad(mpm_ctx* ctx, int s, int n_substeps, float* out, void* stream) {
+    if (!ctx || !out) return fail("mpm_action_grad: null argument");
+    if (n_substeps <= 0 || s < 0 || (long long)(s + 1) * n_substeps >= ctx->max_frames) return fail("mpm_action_grad: bad env step");
+    if (ctx->n_prims == 0) return 0;
+    int n = ctx->n_prims * 7;
+    hipLaunchKernelGGL(k_action_grad, dim3((n + 63) / 64), dim3(64), 0, S_(stream), ctx->d_prims, ctx->n_prims, s, n_substeps,
+                       ctx->pvel_g, out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mpm_update_prims(mpm_ctx* ctx, const mpm_prim_desc* prims, void* stream) {
+    if (!ctx || (!prims && ctx->n_prims > 0)) return fail("mpm_update_prims: null argument");
+    for (int q = 0; q < ctx->n_prims; q++) {
+        if (prims[q].type != ctx->prims[q].type) return fail("mpm_update_prims: primitive types may not change");
+        ctx->prims[q] = to_primc(prims[q]);
+    }
+    if (ctx->n_prims == 0) return 0;
+    HIPCHK(hipMemcpyAsync(ctx->d_prims, ctx->prims.data(), sizeof(PrimC) * ctx->n_prims, hipMemcpyHostToDevice, S_(stream)));
+    HIPCHK(hipStreamSynchronize(S_(stream)));
+    return 0;
+}
+
+int mpm_copy_frame(mpm_ctx* ctx, int src, int dst, void* stream) {
+    if (!ctx) return fail("mpm_copy_frame: null context");
+    if (dst < 0 || dst >= ctx->max_frames) return fail("mpm_copy_frame: frame out of range");
+    hipStream_t st = S_(stream);
+    if (ensure_frame(ctx, src, st)) return -1;
+    if (src != dst) {
+        HIPCHK(hipMemcpyAsync(ctx->frame_ptr(dst), ctx->frame_ptr(src), ctx->frame_floats() * sizeof(float), hipMemcpyDeviceToDevice, st));
+        if (ctx->n_prims > 0)
+            HIPCHK(hipMemcpyAsync(ctx->pstate + (size_t)dst * ctx->n_prims * 8, ctx->pstate + (size_t)src * ctx->n_prims * 8,
+                                  sizeof(float) * 8 * ctx->n_prims, hipMemcpyDeviceToDevice, st));
+    }
+    mark_set(ctx, dst);
+    return 0;
+}
+
+int mpm_clear_grads(mpm_ctx* ctx, int f, void* stream) {
+    if (!ctx) return fail("mpm_clear_grads: null context");
+    if (f < -1 || f >= ctx->max_frames) return fail("mpm_clear_grads: frame out of range");
+    hipStream_t st = S_(stream);
+    if (launch_fill(ctx->adj[0], 2 * ctx->frame_floats(), st)) return -1;
+    size_t pb = (size_t)8 * ctx->max_frames * std::max(1, ctx->n_prims);
+    if (launch_fill(ctx->pstate_g, pb, st)) return -1;
+    if (launch_fill(ctx->pvel_g, pb, st)) return -1;
+    ctx->adj_top = f;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+"""Engine context: one C-ABI context over a torch-owned device workspace.
+
+The workspace is a single uint8 CUDA (HIP) tensor of ``mpm_workspace_bytes``;
+every buffer the context carves out of it is exposed here as a zero-copy
+torch view (frames, adjoints, primitive frames).  All work is enqueued on
+torch's current stream.
+"""
+import ctypes
+
+import torch
+
+from .. import _lib
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def scene_desc(c, n_particles, n_prims):
+    """mpm_scene_desc from the constants of ``config.derive``."""
+    d = _lib.SceneDesc()
+    d.n_grid = int(c["n_grid"])
+    d.n_particles = int(n_particles)
+    d.n_prims = int(n_prims)
+    for k in ("dx", "inv_dx", "dt", "p_vol", "p_mass", "mu", "lam", "yield_stress", "ground_friction", "lower_bound"):
+        setattr(d, k, float(c[k]))
+    for i in range(3):
+        d.gravity[i] = float(c["gravity"][i])
+    return d
+
+
+class Engine:
+    """A packed multi-scene context (scene k: constants, particle count, primitives)."""
+
+    def __init__(self, scene_descs, prim_descs, max_frames, ckpt_k=0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("the MI355X engine needs a HIP device (torch.cuda.is_available() is False)")
+        self.L = _lib.load()
+        self.n_scenes = len(scene_descs)
+        self.max_frames = int(max_frames)
+        self.ckpt_k = int(ckpt_k or 0)
+        self._scenes = (_lib.SceneDesc * self.n_scenes)(*scene_descs)
+        self.n_prims = len(prim_descs)
+        self._prims = (_lib.PrimDesc * max(1, self.n_prims))(*prim_descs)
+        nbytes = self.L.mpm_workspace_bytes(self._scenes, self.n_scenes, self.max_frames, self.ckpt_k)
+        if nbytes <= 0:
+            raise RuntimeError("mpm_workspace_bytes failed")
+        self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device="cuda")
+        ctx = ctypes.c_void_p()
+        _lib.check(self.L.mpm_create(self._scenes, self.n_scenes, self._prims, self.max_frames, self.ckpt_k,
+                                     ctypes.c_void_p(self.ws.data_ptr()), int(nbytes), _stream(), ctypes.byref(ctx)),
+                   "mpm_create")
+        self.ctx = ctx
+        self.stride = self.L.mpm_stride(ctx)
+        self.offsets = [self.L.mpm_particle_offset(ctx, k) for k in range(self.n_scenes)]
+        self.counts = [int(d.n_particles) for d in scene_descs]
+        self.n_grids = [int(d.n_grid) for d in scene_descs]
+        ps, psg, pv, pvg = (ctypes.c_void_p() for _ in range(4))
+        _lib.check(self.L.mpm_prim_buffers(ctx, ctypes.byref(ps), ctypes.byref(psg), ctypes.byref(pv), ctypes.byref(pvg)),
+                   "mpm_prim_buffers")
+        np_ = max(1, self.n_prims)
+        shape = (self.max_frames, np_, 8)
+        self.prim_state = self._view(ps.value, shape)
+        self.prim_state_grad = self._view(psg.value, shape)
+        self.prim_vel = self._view(pv.value, shape)
+        self.prim_vel_grad = self._view(pvg.value, shape)
+
+    def __del__(self):
+        ctx = getattr(self, "ctx", None)
+        if ctx is not None and ctx.value:
+            try:
+                self.L.mpm_destroy(ctx)
+            except Exception:
+                pass
+            self.ctx = None
+
+    # ---------------------------------------------------------------- views
+    def _view(self, ptr, shape):
+        n = 1
+        for s in shape:
+            n *= s
+        off = int(ptr) - self.ws.data_ptr()
+        assert 0 <= off and off + 4 * n <= self.ws.numel(), "buffer outside the workspace"
+        return self.ws[off:off + 4 * n].view(torch.float32).view(*shape)
+
+    def frame(self, f):
+        """(24, stride) view of frame f (recomputed from its checkpoint if needed)."""
+        p = ctypes.c_void_p()
+        _lib.check(self.L.mpm_frame(self.ctx, int(f), _stream(), ctypes.byref(p)), "mpm_frame")
+        return self._view(p.value, (24, self.stride))
+
+    def adjoint(self, f):
+        p = ctypes.c_void_p()
+        _lib.check(self.L.mpm_adjoint(self.ctx, int(f), ctypes.byref(p)), "mpm_adjoint")
+        return self._view(p.value, (24, self.stride))
+
+    def scene_cols(self, k):
+        o = self.offsets[k]
+        return slice(o, o + self.counts[k])
+
+    # ---------------------------------------------------------------- calls
+    def set_frame(self, f, soa):
+        soa = soa.contiguous()
+        assert soa.shape == (24, self.stride) and soa.dtype == torch.float32 and soa.is_cuda
+        _lib.check(self.L.mpm_set_frame(self.ctx, int(f), ctypes.c_void_p(soa.data_ptr()), _stream()), "mpm_set_frame")
+
+    def reset_frame0(self, x_soa):
+        x_soa = x_soa.contiguous()
+        assert x_soa.shape == (3, self.stride)
+        _lib.check(self.L.mpm_reset_frame0(self.ctx, ctypes.c_void_p(x_soa.data_ptr()), _stream()), "mpm_reset_frame0")
+
+    def copy_frame(self, src, dst):
+        _lib.check(self.L.mpm_copy_frame(self.ctx, int(src), int(dst), _stream()), "mpm_copy_frame")
+
+    def set_actions(self, s, n_substeps, table):
+        table = table.contiguous().float()
+        assert table.is_cuda and table.shape[-1] == 7
+        _lib.check(self.L.mpm_set_actions(self.ctx, int(s), int(n_substeps), ctypes.c_void_p(table.data_ptr()), _stream()),
+                   "mpm_set_actions")
+
+    def substep_fwd(self, s, count=1):
+        _lib.check(self.L.mpm_substep_fwd(self.ctx, int(s), int(count), _stream()), "mpm_substep_fwd")
+
+    def substep_bwd(self, s, count=1):
+        _lib.check(self.L.mpm_substep_bwd(self.ctx, int(s), int(count), _stream()), "mpm_substep_bwd")
+
+    def grid_mass(self, f, out):
+        assert out.is_cuda and out.dtype == torch.float32 and out.is_contiguous()
+        _lib.check(self.L.mpm_grid_mass(self.ctx, int(f), ctypes.c_void_p(out.data_ptr()), _stream()), "mpm_grid_mass")
+
+    def grid_mass_bwd(self, f, g):
+        g = g.contiguous().float()
+        _lib.check(self.L.mpm_grid_mass_bwd(self.ctx, int(f), ctypes.c_void_p(g.data_ptr()), _stream()), "mpm_grid_mass_bwd")
+
+    def add_obs_grad(self, f, xv_slots=None, prim_grad=None):
+        a = ctypes.c_void_p(xv_slots.data_ptr()) if xv_slots is not None else None
+        b = ctypes.c_void_p(prim_grad.data_ptr()) if prim_grad is not None else None
+        _lib.check(self.L.mpm_add_obs_grad(self.ctx, int(f), a, b, _stream()), "mpm_add_obs_grad")
+
+    def action_grad(self, s, n_substeps):
+        out = torch.zeros((max(1, self.n_prims), 7), dtype=torch.float32, device="cuda")
+        _lib.check(self.L.mpm_action_grad(self.ctx, int(s), int(n_substeps), ctypes.c_void_p(out.data_ptr()), _stream()),
+                   "mpm_action_grad")
+        return out
+
+    def clear_grads(self, f=-1):
+        _lib.check(self.L.mpm_clear_grads(self.ctx, int(f), _stream()), "mpm_clear_grads")
+
+    def update_prims(self, prim_descs):
+        self._prims = (_lib.PrimDesc * max(1, self.n_prims))(*prim_descs)
+        _lib.check(self.L.mpm_update_prims(self.ctx, self._prims, _stream()), "mpm_update_prims")

@@ -1,0 +1,157 @@
+/*
+ * diffskill_mpm.h -- C ABI of the MI355X (gfx950) differentiable MLS-MPM engine.
+ *
+ * Plain pointers and sizes only.  All device pointers are HIP device memory;
+ * every call that launches work takes the hipStream_t (as void*) to run on.
+ *
+ * Each entry point replaces a piece of the reference's Taichi path
+ * (Xingyu-Lin/DiffSkill, plb/engine):
+ *
+ *   mpm_create / mpm_workspace_bytes   MPMSimulator.__init__ field allocation
+ *                                      (mpm_simulator.py:8-67) and
+ *                                      Primitives(...) (primitives.py:821-838)
+ *   mpm_set_frame / mpm_get_frame      setframe / readframe, set_state /
+ *                                      get_state (mpm_simulator.py:348-396)
+ *   mpm_reset_frame0                   reset_kernel (mpm_simulator.py:402-413)
+ *   mpm_set_prim_frame / mpm_get_prim_frame
+ *                                      Primitive.set_state / get_state
+ *                                      (primive_base.py:182-207)
+ *   mpm_set_actions                    Primitive.set_action + set_velocity
+ *                                      (primive_base.py:260-274)
+ *   mpm_substep_fwd                    MPMSimulator.substep (mpm_simulator.py:307-323)
+ *   mpm_substep_bwd                    MPMSimulator.substep_grad (mpm_simulator.py:325-345)
+ *   mpm_grid_mass / mpm_grid_mass_bwd  compute_grid_m_kernel and its .grad
+ *                                      (mpm_simulator.py:456-466, function.py:147-152)
+ *   mpm_add_obs_grad                   GradModel._set_obs_grad (function.py:130-142)
+ *   mpm_action_grad                    set_velocity.grad + get_action_grad
+ *                                      (function.py:180-190, primive_base.py:276-282)
+ *   mpm_clear_grads                    GradModel.reset grad clearing (function.py:44-61)
+ *   mpm_last_error                     Python exceptions of the reference
+ *
+ * Threading: one context per stream; no internal locking (the reference is
+ * single-threaded per process).  Every function returns 0 on success and a
+ * negative code on error; mpm_last_error() describes the last error of the
+ * calling thread.
+ */
+#ifndef DISKILL_MPM_H
+#define DISKILL_MPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Primitive shape codes (class names of plb/engine/primitive/primitives.py). */
+enum {
+    MPM_SPHERE = 0, MPM_CAPSULE = 1, MPM_ROLLINGPIN = 2, MPM_ROLLINGPIN_EXT = 3,
+    MPM_BOX = 4, MPM_GRIPPER = 5, MPM_PRISM = 6, MPM_KNIFE = 7,
+    MPM_CHOPSTICKS = 8, MPM_GRIPPER2 = 9
+};
+
+/* One scene: the SIMULATOR constants of MPMSimulator.__init__
+ * (mpm_simulator.py:8-39), derived on the host in double precision. */
+typedef struct {
+    int n_grid;            /* nodes per axis */
+    int n_particles;       /* particles of this scene */
+    int n_prims;           /* primitives of this scene */
+    int pad0;
+    double dx, inv_dx, dt, p_vol, p_mass;
+    double mu, lam, yield_stress;
+    double gravity[3];
+    double ground_friction;
+    double lower_bound;
+} mpm_scene_desc;
+
+/* One primitive (Primitive cfg + per-class parameters). */
+typedef struct {
+    int type;              /* MPM_* code */
+    int action_dim;
+    double friction, softness;
+    double size[3];        /* Box/Gripper half extents; Knife box size */
+    double h, r;           /* Capsule family h (segment length), r; Sphere radius in r */
+    double ph[2];          /* Prism / Knife prism h */
+    double prot[4];        /* Prism / Knife prism rotation (w, x, y, z) */
+    double lo[3], hi[3];   /* xyz_limit */
+    double min_gap, max_gap;
+    double action_scale[7];
+} mpm_prim_desc;
+
+typedef struct mpm_ctx mpm_ctx;
+
+/* Bytes of device workspace a context needs: particle frames (checkpoint
+ * every ckpt_k frames plus one live segment of ckpt_k - 1 frames; ckpt_k >=
+ * max_frames means store-all), two adjoint frames, grids, primitive frames. */
+int64_t mpm_workspace_bytes(const mpm_scene_desc *scenes, int n_scenes, int max_frames, int ckpt_k);
+
+/* Create a context over a caller-owned workspace (device pointer, >=
+ * mpm_workspace_bytes).  prims: concatenated primitives of all scenes, in
+ * scene order.  The workspace must stay alive until mpm_destroy. */
+int mpm_create(const mpm_scene_desc *scenes, int n_scenes, const mpm_prim_desc *prims,
+               int max_frames, int ckpt_k, void *workspace, int64_t workspace_bytes,
+               void *stream, mpm_ctx **out);
+int mpm_destroy(mpm_ctx *ctx);
+
+/* Device pointer of frame f's particle state (SoA, 24 rows of `stride`
+ * floats: x0 x1 x2 v0 v1 v2 C00..C22 F00..F22; scene k owns columns
+ * [particle_offset(k), +n_particles)).  Recomputes the frame from its
+ * checkpoint when it is not resident (forward substeps on `stream`). */
+int mpm_frame(mpm_ctx *ctx, int f, void *stream, float **ptr);
+int mpm_stride(mpm_ctx *ctx);
+int mpm_particle_offset(mpm_ctx *ctx, int scene);
+/* Adjoint of frame f (valid between backward substeps, same layout). */
+int mpm_adjoint(mpm_ctx *ctx, int f, float **ptr);
+/* Primitive frame state [max_frames][n_prims_total][8] (pos, quat wxyz,
+ * gap), its adjoint, per-substep velocity [max_frames][n][8] (v, w,
+ * gap_vel), its adjoint. */
+int mpm_prim_buffers(mpm_ctx *ctx, float **state, float **state_grad, float **vel, float **vel_grad);
+
+/* Copy a whole frame (all scenes, SoA layout above) from/to device memory. */
+int mpm_set_frame(mpm_ctx *ctx, int f, const float *src, void *stream);
+int mpm_get_frame(mpm_ctx *ctx, int f, float *dst, void *stream);
+/* reset_kernel: frame 0 from positions x (3 rows of `stride` floats, the
+ * SoA layout above), v = 0, F = I, C = 0. */
+int mpm_reset_frame0(mpm_ctx *ctx, const float *x, void *stream);
+/* Primitive state of frame f for every primitive ([n_prims_total][8], device). */
+int mpm_set_prim_frame(mpm_ctx *ctx, int f, const float *src, void *stream);
+int mpm_get_prim_frame(mpm_ctx *ctx, int f, float *dst, void *stream);
+
+/* Set the actions of env step s for every primitive (device
+ * [n_prims_total][7], already clipped), filling the per-substep velocity of
+ * substeps [s*n_substeps, (s+1)*n_substeps). */
+int mpm_set_actions(mpm_ctx *ctx, int s, int n_substeps, const float *actions, void *stream);
+
+/* `count` forward substeps starting at frame s (all scenes). */
+int mpm_substep_fwd(mpm_ctx *ctx, int s, int count, void *stream);
+/* `count` adjoint substeps: frames s+count-1 down to s (LIFO).  Returns an
+ * error when the adjoint of frame s+count is not the current one. */
+int mpm_substep_bwd(mpm_ctx *ctx, int s, int count, void *stream);
+
+/* Particle mass grid of frame f for every scene, ADDED into out
+ * [n_scenes][n^3] (i, j, k row-major, node I at world position I*dx). */
+int mpm_grid_mass(mpm_ctx *ctx, int f, float *out, void *stream);
+/* Adds d(sum g*m)/dx into the adjoint of frame f. g: [n_scenes][n^3]. */
+int mpm_grid_mass_bwd(mpm_ctx *ctx, int f, const float *g, void *stream);
+/* Adds observation adjoints at frame f: xv_grad [stride][6] (x, v of every
+ * particle slot; may be NULL), prim_grad [n_prims_total][8] (may be NULL). */
+int mpm_add_obs_grad(mpm_ctx *ctx, int f, const float *xv_grad, const float *prim_grad, void *stream);
+/* Action adjoint of env step s: out [n_prims_total][7]. */
+int mpm_action_grad(mpm_ctx *ctx, int s, int n_substeps, float *out, void *stream);
+/* Zero every adjoint (particles and primitives) and make frame f the top
+ * of the adjoint chain; f = -1 lets the first adjoint call choose it. */
+int mpm_clear_grads(mpm_ctx *ctx, int f, void *stream);
+/* Re-upload the primitive table (softness / friction / limits; types fixed).
+ * Replaces Primitives.set_softness (primitives.py:876-878). Synchronizes `stream`. */
+int mpm_update_prims(mpm_ctx *ctx, const mpm_prim_desc *prims, void *stream);
+/* Copy frame src (particles and primitive state) to frame dst and restart
+ * the trajectory there: copyframe (mpm_simulator.py:369-378), used by
+ * MPMSimulator.step in copy mode (mpm_simulator.py:449-451). */
+int mpm_copy_frame(mpm_ctx *ctx, int src, int dst, void *stream);
+
+const char *mpm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
