@@ -46,8 +46,10 @@ EXPORTS = [
     "mpm_adjoint", "mpm_prim_buffers", "mpm_set_frame", "mpm_get_frame", "mpm_reset_frame0",
     "mpm_set_prim_frame", "mpm_get_prim_frame", "mpm_set_actions", "mpm_substep_fwd", "mpm_substep_bwd",
     "mpm_grid_mass", "mpm_grid_mass_bwd", "mpm_add_obs_grad", "mpm_action_grad", "mpm_clear_grads",
-    "mpm_update_prims", "mpm_copy_frame", "mpm_last_error",
+    "mpm_update_prims", "mpm_copy_frame", "mpm_set_profiling", "mpm_profile_read", "mpm_last_error",
 ]
+
+PHASES = ["mark", "compact", "p2g", "grid_op", "g2p", "p2g_grid", "grid_op_keep", "g2p_bwd", "grid_op_bwd", "p2g_bwd"]
 
 _LIB = None
 
@@ -87,6 +89,8 @@ def load():
         "mpm_clear_grads": (i32, [vp, i32, vp]),
         "mpm_update_prims": (i32, [vp, vp, vp]),
         "mpm_copy_frame": (i32, [vp, i32, i32, vp]),
+        "mpm_set_profiling": (i32, [vp, i32]),
+        "mpm_profile_read": (i32, [vp, vp, vp, i32]),
         "mpm_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():

@@ -818,6 +818,13 @@ struct mpm_ctx {
     int seg_hi = -1;      // highest resident ring frame of that segment
     int adj_top = -1;     // frame whose adjoint is current
     int grid_ctas = 1024; // persistent grid-phase workgroups
+    // per-kernel timing with HIP events on the launch stream (mpm_set_profiling)
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Rec { int phase; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    double prof_ms[MPM_N_PHASES] = {0};
+    long long prof_n[MPM_N_PHASES] = {0};
 
     size_t frame_floats() const { return (size_t)NST * stride; }
     float* frame_ptr(int f) const {
@@ -835,6 +842,23 @@ struct mpm_ctx {
 };
 
 namespace {
+
+hipEvent_t take_event(mpm_ctx* c) {
+    if (!c->ev_pool.empty()) { hipEvent_t e = c->ev_pool.back(); c->ev_pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+// wraps one kernel launch in a pair of events when profiling is on
+struct PhaseTimer {
+    mpm_ctx* c; int phase; hipStream_t st; hipEvent_t a = nullptr, b = nullptr;
+    PhaseTimer(mpm_ctx* c_, int ph, hipStream_t s) : c(c_), phase(ph), st(s) {
+        if (c->prof) { a = take_event(c); b = take_event(c); if (a) hipEventRecord(a, st); }
+    }
+    ~PhaseTimer() {
+        if (c->prof && a && b) { hipEventRecord(b, st); c->recs.push_back({phase, a, b}); }
+    }
+};
 
 struct Layout {
     size_t off_scenes, off_prims, off_chunks, off_blk_scene, off_ckpt, off_ring, off_adj, off_grid, off_flags, off_active,
@@ -903,13 +927,18 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
     const float* in = c->frame_ptr(f);
     float* out = c->frame_ptr(f + 1);
     ChunkArgs ca = c->cargs();
-    hipLaunchKernelGGL(k_mark, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, c->blk_flag, c->counters, 1, c->d_prims, c->n_prims,
-                       c->pstate, c->pvel, f);
-    hipLaunchKernelGGL(k_compact, dim3(std::min(4096, (c->n_blocks + 255) / 256)), dim3(256), 0, st, c->n_blocks, c->blk_flag,
-                       c->active, c->counters);
-    hipLaunchKernelGGL(k_p2g, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, out, c->g_pm);
-    hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->g_v, 1);
-    hipLaunchKernelGGL(k_g2p, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, out, c->g_v);
+    { PhaseTimer t(c, MPM_PH_MARK, st);
+      hipLaunchKernelGGL(k_mark, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, c->blk_flag, c->counters, 1, c->d_prims, c->n_prims,
+                         c->pstate, c->pvel, f); }
+    { PhaseTimer t(c, MPM_PH_COMPACT, st);
+      hipLaunchKernelGGL(k_compact, dim3(std::min(4096, (c->n_blocks + 255) / 256)), dim3(256), 0, st, c->n_blocks, c->blk_flag,
+                         c->active, c->counters); }
+    { PhaseTimer t(c, MPM_PH_P2G, st);
+      hipLaunchKernelGGL(k_p2g, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, out, c->g_pm); }
+    { PhaseTimer t(c, MPM_PH_GRID, st);
+      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->g_v, 1); }
+    { PhaseTimer t(c, MPM_PH_G2P, st);
+      hipLaunchKernelGGL(k_g2p, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, out, c->g_v); }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -938,16 +967,23 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     const float* a1 = c->adj_ptr(f + 1);
     float* a0 = c->adj_ptr(f);
     ChunkArgs ca = c->cargs();
-    hipLaunchKernelGGL(k_mark, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, c->blk_flag, c->counters, 0, c->d_prims, c->n_prims,
-                       c->pstate, c->pvel, f);
-    hipLaunchKernelGGL(k_compact, dim3(std::min(4096, (c->n_blocks + 255) / 256)), dim3(256), 0, st, c->n_blocks, c->blk_flag,
-                       c->active, c->counters);
-    hipLaunchKernelGGL(k_p2g, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, (float*)nullptr, c->g_pm);
-    hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->g_v, 0);
-    hipLaunchKernelGGL(k_g2p_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, in1, a1, a0, c->g_v, c->g_adjv);
-    hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->g_adjv, c->g_adjpm, c->pstate_g);
-    hipLaunchKernelGGL(k_p2g_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, a1, a0, c->g_adjpm, c->d_prims, c->n_prims, c->pstate,
-                       c->pvel, c->pstate_g, c->pvel_g, f);
+    { PhaseTimer t(c, MPM_PH_MARK, st);
+      hipLaunchKernelGGL(k_mark, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, c->blk_flag, c->counters, 0, c->d_prims, c->n_prims,
+                         c->pstate, c->pvel, f); }
+    { PhaseTimer t(c, MPM_PH_COMPACT, st);
+      hipLaunchKernelGGL(k_compact, dim3(std::min(4096, (c->n_blocks + 255) / 256)), dim3(256), 0, st, c->n_blocks, c->blk_flag,
+                         c->active, c->counters); }
+    { PhaseTimer t(c, MPM_PH_P2G_GRID, st);
+      hipLaunchKernelGGL(k_p2g, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, (float*)nullptr, c->g_pm); }
+    { PhaseTimer t(c, MPM_PH_GRID_KEEP, st);
+      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->g_v, 0); }
+    { PhaseTimer t(c, MPM_PH_G2P_BWD, st);
+      hipLaunchKernelGGL(k_g2p_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, in1, a1, a0, c->g_v, c->g_adjv); }
+    { PhaseTimer t(c, MPM_PH_GRID_BWD, st);
+      hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->g_adjv, c->g_adjpm, c->pstate_g); }
+    { PhaseTimer t(c, MPM_PH_P2G_BWD, st);
+      hipLaunchKernelGGL(k_p2g_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, a1, a0, c->g_adjpm, c->d_prims, c->n_prims, c->pstate,
+                         c->pvel, c->pstate_g, c->pvel_g, f); }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1115,7 +1151,36 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
 }
 
 int mpm_destroy(mpm_ctx* ctx) {
+    if (!ctx) return 0;
+    for (auto& r : ctx->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+    for (auto e : ctx->ev_pool) hipEventDestroy(e);
     delete ctx;
+    return 0;
+}
+
+int mpm_set_profiling(mpm_ctx* ctx, int on) {
+    if (!ctx) return fail("mpm_set_profiling: null context");
+    ctx->prof = on != 0;
+    return 0;
+}
+
+int mpm_profile_read(mpm_ctx* ctx, double* ms, long long* counts, int reset) {
+    if (!ctx) return fail("mpm_profile_read: null context");
+    for (auto& r : ctx->recs) {
+        HIPCHK(hipEventSynchronize(r.b));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+        ctx->prof_ms[r.phase] += t;
+        ctx->prof_n[r.phase] += 1;
+        ctx->ev_pool.push_back(r.a);
+        ctx->ev_pool.push_back(r.b);
+    }
+    ctx->recs.clear();
+    for (int i = 0; i < MPM_N_PHASES; i++) {
+        if (ms) ms[i] = ctx->prof_ms[i];
+        if (counts) counts[i] = ctx->prof_n[i];
+        if (reset) { ctx->prof_ms[i] = 0; ctx->prof_n[i] = 0; }
+    }
     return 0;
 }
 

@@ -143,8 +143,25 @@ class Engine:
                    "mpm_action_grad")
         return out
 
+    def action_grad_into(self, s, n_substeps, out):
+        """Action adjoint of env step s written into ``out`` ([n_prims, 7] float32, contiguous)."""
+        assert out.is_cuda and out.dtype == torch.float32 and out.is_contiguous() and out.shape[-1] == 7
+        _lib.check(self.L.mpm_action_grad(self.ctx, int(s), int(n_substeps), ctypes.c_void_p(out.data_ptr()), _stream()),
+                   "mpm_action_grad")
+
     def clear_grads(self, f=-1):
         _lib.check(self.L.mpm_clear_grads(self.ctx, int(f), _stream()), "mpm_clear_grads")
+
+    def set_profiling(self, on=True):
+        _lib.check(self.L.mpm_set_profiling(self.ctx, 1 if on else 0), "mpm_set_profiling")
+
+    def profile_read(self, reset=True):
+        """{phase: (total_ms, launches)} from the HIP events recorded around each kernel."""
+        n = len(_lib.PHASES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_longlong * n)()
+        _lib.check(self.L.mpm_profile_read(self.ctx, ms, cnt, 1 if reset else 0), "mpm_profile_read")
+        return {p: (ms[i], cnt[i]) for i, p in enumerate(_lib.PHASES)}
 
     def update_prims(self, prim_descs):
         self._prims = (_lib.PrimDesc * max(1, self.n_prims))(*prim_descs)

@@ -149,6 +149,19 @@ int mpm_update_prims(mpm_ctx *ctx, const mpm_prim_desc *prims, void *stream);
  * MPMSimulator.step in copy mode (mpm_simulator.py:449-451). */
 int mpm_copy_frame(mpm_ctx *ctx, int src, int dst, void *stream);
 
+/* Per-kernel timing with HIP events recorded on the launch stream around
+ * every engine kernel (no counterpart in the reference; measurement only).
+ * Phases: */
+enum {
+    MPM_PH_MARK = 0, MPM_PH_COMPACT = 1, MPM_PH_P2G = 2, MPM_PH_GRID = 3, MPM_PH_G2P = 4,
+    MPM_PH_P2G_GRID = 5, MPM_PH_GRID_KEEP = 6, MPM_PH_G2P_BWD = 7, MPM_PH_GRID_BWD = 8,
+    MPM_PH_P2G_BWD = 9, MPM_N_PHASES = 10
+};
+int mpm_set_profiling(mpm_ctx *ctx, int on);
+/* Accumulated milliseconds and launch counts per phase (synchronizes the
+ * recorded events); reset != 0 clears the accumulators. */
+int mpm_profile_read(mpm_ctx *ctx, double *ms, long long *counts, int reset);
+
 const char *mpm_last_error(void);
 
 #ifdef __cplusplus

@@ -115,6 +115,87 @@ def make_prims(plist):
     return arr
 
 
+# per-class primitive defaults (Primitive.default_config subclasses, primitives.py)
+PRIM_DEFAULTS = {
+    "Sphere": dict(radius=1.0),
+    "Capsule": dict(h=0.06, r=0.03),
+    "RollingPin": dict(h=0.06, r=0.03),
+    "RollingPinExt": dict(h=0.06, r=0.03),
+    "Box": dict(size=(0.1, 0.1, 0.1)),
+    "Gripper": dict(size=(0.03, 0.06, 0.03), minimal_gap=0.06, maximal_gap=1.0, init_gap=0.06),
+    "Gripper2": dict(h=0.06, r=0.015, minimal_gap=0.06, maximal_gap=1.0, init_gap=0.06),
+    "Chopsticks": dict(h=0.06, r=0.03, minimal_gap=0.06, init_gap=0.06),
+    "Prism": dict(h=(0.1, 0.1), prot=(1.0, 0.0, 0.0, 0.0)),
+    "Knife": dict(h=(0.1, 0.1), size=(0.1, 0.1, 0.1), prot=(1.0, 0.0, 0.0, 0.0)),
+}
+
+
+def sim_from_cfg(cfg):
+    """SIMULATOR cfg -> oracle constants, the arithmetic of MPMSimulator.__init__
+    (plb/engine/mpm_simulator.py:19-39) with the defaults of
+    plb/config/default_config.py:11-22; optional overrides n_grid / dt / substeps."""
+    d = dict(quality=1, quality_multiplier=1.0, yield_stress=50.0, E=5e3, nu=0.15, ground_friction=1.5,
+             gravity=(0, -1, 0), lower_bound=0.0)
+    d.update(cfg)
+    q = d["quality"] * d["quality_multiplier"] * 0.5
+    n_grid = int(d.get("n_grid") or int(128 * q))
+    dx = 1 / n_grid
+    dt = float(d.get("dt") or 0.5e-4 / q)
+    p_vol = (dx * 0.5) ** 2
+    E, nu = d["E"], d["nu"]
+    sim = dict(n_grid=n_grid, dx=dx, inv_dx=float(n_grid), dt=dt, p_vol=p_vol, p_mass=p_vol,
+               mu=E / (2 * (1 + nu)), lam=E * nu / ((1 + nu) * (1 - 2 * nu)), yield_stress=float(d["yield_stress"]),
+               gravity=tuple(float(g) for g in d["gravity"]), ground_friction=float(d["ground_friction"]),
+               lower_bound=float(d["lower_bound"]))
+    substeps = int(d.get("substeps") or int(2e-3 // dt))
+    return sim, substeps
+
+
+def prim_from_cfg(cfg, softness=666.0):
+    """PRIMITIVES entry -> oracle primitive parameters."""
+    shape = cfg["shape"]
+    d = dict(PRIM_DEFAULTS.get(shape, {}))
+    d.update(cfg)
+    act = cfg.get("action", {}) or {}
+    o = dict(type=shape, action_dim=int(act.get("dim", 0)), friction=float(cfg.get("friction", 0.9)),
+             softness=softness, lo=tuple(cfg.get("lower_bound", (0, 0, 0))), hi=tuple(cfg.get("upper_bound", (1, 1, 1))),
+             action_scale=tuple(act.get("scale", ())))
+    if shape == "Sphere":
+        o["r"] = d["radius"]
+    if shape in ("Capsule", "RollingPin", "RollingPinExt", "Gripper2", "Chopsticks"):
+        o["h"], o["r"] = d["h"], d["r"]
+    if shape in ("Box", "Gripper", "Knife"):
+        o["size"] = tuple(d["size"])
+    if shape in ("Prism", "Knife"):
+        o["ph"], o["prot"] = tuple(d["h"]), tuple(d["prot"])
+    if shape in ("Gripper", "Gripper2", "Chopsticks"):
+        o["min_gap"] = d["minimal_gap"]
+        o["max_gap"] = d.get("maximal_gap", 1.0)
+    return o
+
+
+def prim_state_from_cfg(cfg):
+    """Initial primitive frame state: init_pos + init_rot (+ init_gap)."""
+    shape = cfg["shape"]
+    d = dict(PRIM_DEFAULTS.get(shape, {}))
+    d.update(cfg)
+    s = np.zeros(8)
+    s[:3] = cfg.get("init_pos", (0.3, 0.3, 0.3))
+    s[3:7] = cfg.get("init_rot", (1.0, 0.0, 0.0, 0.0))
+    if shape in ("Gripper", "Gripper2", "Chopsticks"):
+        s[7] = d["init_gap"]
+    return s
+
+
+def reset_state(x):
+    """reset_kernel: v = 0, F = I, C = 0 (mpm_simulator.py:402-410)."""
+    x = np.asarray(x, np.float64)
+    st = np.zeros((x.shape[0], 24))
+    st[:, 0:3] = x
+    st[:, 15:24] = np.eye(3).reshape(-1)
+    return st
+
+
 class Oracle:
     """One scene advanced and differentiated by the C restatement."""
 

@@ -7,57 +7,24 @@ import numpy as np
 
 from diffskill_amd.config import derive
 
-PRIM_DEFAULTS = {
-    "Sphere": dict(radius=1.0),
-    "Capsule": dict(h=0.06, r=0.03),
-    "RollingPin": dict(h=0.06, r=0.03),
-    "RollingPinExt": dict(h=0.06, r=0.03),
-    "Box": dict(size=(0.1, 0.1, 0.1)),
-    "Gripper": dict(size=(0.03, 0.06, 0.03), minimal_gap=0.06, maximal_gap=1.0, init_gap=0.06),
-    "Gripper2": dict(h=0.06, r=0.015, minimal_gap=0.06, maximal_gap=1.0, init_gap=0.06),
-    "Chopsticks": dict(h=0.06, r=0.03, minimal_gap=0.06, init_gap=0.06),
-    "Prism": dict(h=(0.1, 0.1), prot=(1.0, 0.0, 0.0, 0.0)),
-    "Knife": dict(h=(0.1, 0.1), size=(0.1, 0.1, 0.1), prot=(1.0, 0.0, 0.0, 0.0)),
-}
+from oracle import prim_from_cfg, prim_state_from_cfg, sim_from_cfg
 
 
 def oracle_prim(cfg, softness=666.0):
-    shape = cfg["shape"]
-    d = dict(PRIM_DEFAULTS.get(shape, {}))
-    d.update(cfg)
-    o = dict(type=shape, action_dim=int(cfg.get("action", {}).get("dim", 0)), friction=float(cfg.get("friction", 0.9)),
-             softness=softness, lo=tuple(cfg.get("lower_bound", (0, 0, 0))), hi=tuple(cfg.get("upper_bound", (1, 1, 1))),
-             action_scale=tuple(cfg.get("action", {}).get("scale", ())))
-    if shape == "Sphere":
-        o["r"] = d["radius"]
-    if shape in ("Capsule", "RollingPin", "RollingPinExt", "Gripper2", "Chopsticks"):
-        o["h"], o["r"] = d["h"], d["r"]
-    if shape in ("Box", "Gripper", "Knife"):
-        o["size"] = tuple(d["size"])
-    if shape in ("Prism", "Knife"):
-        o["ph"], o["prot"] = tuple(d["h"]), tuple(d["prot"])
-    if shape in ("Gripper", "Gripper2", "Chopsticks"):
-        o["min_gap"] = d["minimal_gap"]
-        o["max_gap"] = d.get("maximal_gap", 1.0)
-    return o
+    return prim_from_cfg(cfg, softness)
 
 
 def oracle_sim(sim_cfg):
+    """(oracle constants, product-derived constants); the two derivations are
+    independent and must agree."""
+    o, substeps = sim_from_cfg(sim_cfg)
     c = derive(sim_cfg)
-    return {k: c[k] for k in ("n_grid", "dx", "inv_dx", "dt", "p_vol", "p_mass", "mu", "lam", "yield_stress",
-                              "gravity", "ground_friction", "lower_bound")}, c
+    assert c["substeps"] == substeps and c["n_grid"] == o["n_grid"] and abs(c["dt"] - o["dt"]) < 1e-15
+    return o, c
 
 
 def prim_state0(cfg):
-    shape = cfg["shape"]
-    d = dict(PRIM_DEFAULTS.get(shape, {}))
-    d.update(cfg)
-    s = np.zeros(8)
-    s[:3] = cfg.get("init_pos", (0.3, 0.3, 0.3))
-    s[3:7] = cfg.get("init_rot", (1.0, 0.0, 0.0, 0.0))
-    if shape in ("Gripper", "Gripper2", "Chopsticks"):
-        s[7] = d["init_gap"]
-    return s
+    return prim_state_from_cfg(cfg)
 
 
 def particles_box(n, lo, hi, seed=0, v_sigma=0.0, F_jitter=0.0):
