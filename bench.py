@@ -31,7 +31,7 @@ ALGO_BYTES = {
     "p2g_grid": 96,            # read x v C F
     "g2p_bwd": 12 + 12 + 60 + 12,    # read x, x', adj x' v' C', write adj x
     "p2g_bwd": 96 + 36 + 12 + 96,    # read x v C F, adj F', adj x; write adj x v C F
-    "mark": 12,
+    "bin": 12,
 }
 
 

@@ -12,7 +12,7 @@ import numpy as np
 import torch
 
 from .config import derive
-from .engine.core import Engine, scene_desc
+from .engine.core import Engine, cell_sort_perm, scene_desc
 from .engine.primitive.primitives import CLASSES
 
 
@@ -48,9 +48,13 @@ class SceneBatch:
         frame0 = torch.zeros((24, e.stride), dtype=torch.float32, device="cuda")
         eye = torch.tensor([1., 0, 0, 0, 1, 0, 0, 0, 1], device="cuda")
         ps0 = torch.zeros((max(1, self.n_prims), 8), dtype=torch.float32, device="cuda")
+        # particles stored in cell order; perm[k][i] = caller's index of slot i of scene k
+        self.perm = []
         for k, s in enumerate(scenes):
             cols = e.scene_cols(k)
-            frame0[0:3, cols] = torch.as_tensor(np.asarray(s["x0"], np.float32), device="cuda").t()
+            perm = cell_sort_perm(s["x0"], self.consts[k]["n_grid"])
+            self.perm.append(perm)
+            frame0[0:3, cols] = torch.as_tensor(np.asarray(s["x0"], np.float32)[perm], device="cuda").t()
             frame0[15:24, cols] = eye[:, None]
             states = s.get("prim_states") or [np.asarray(p.init_state + ((0.0,) if p.state_dim == 7 else ()), np.float64)
                                              for p in self.prims[k]]

@@ -404,6 +404,106 @@ DEV float local_sdf(const PrimC& P, int sh, float gap, v3 p, v3* g, float* ggap)
     }
     }
 }
+// ---- fp64 evaluation of the central-difference normals (Box / Prism / Knife).
+// In fp32 the shifted positions p +- 1e-4 are rounded to ~3e-8, a ~3e-4
+// relative error in the difference quotient that dominates the adjoint error
+// of tools with FD normals; the quotient and its adjoint are therefore
+// evaluated in fp64 (per grid node only).
+struct d3 { double x, y, z; };
+DEV double dcomp(const d3& a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+DEV void dsetc(d3& a, int i, double v) { if (i == 0) a.x = v; else if (i == 1) a.y = v; else a.z = v; }
+DEV double dmax(double a, double b) { return b < a ? a : b; }
+DEV double dmin(double a, double b) { return a < b ? a : b; }
+DEV double dsgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+DEV double box_sdf_d(const PrimC& P, d3 p, d3* g) {
+    double q0 = fabs(p.x) - (double)P.size[0], q1 = fabs(p.y) - (double)P.size[1], q2 = fabs(p.z) - (double)P.size[2];
+    double a0 = dmax(q0, 0.0), a1 = dmax(q1, 0.0), a2 = dmax(q2, 0.0);
+    double L = sqrt(a0 * a0 + a1 * a1 + a2 * a2 + (double)LEN_EPS_SHAPE);
+    double m12 = dmax(q1, q2), m = dmax(q0, m12);
+    if (g) {
+        double g0 = (0.0 < q0) ? a0 / L : 0.0, g1 = (0.0 < q1) ? a1 / L : 0.0, g2 = (0.0 < q2) ? a2 / L : 0.0;
+        if (m < 0.0) {
+            if (m12 < q0) g0 += 1.0;
+            else if (q2 < q1) g1 += 1.0;
+            else g2 += 1.0;
+        }
+        g->x = dsgn(p.x) * g0; g->y = dsgn(p.y) * g1; g->z = dsgn(p.z) * g2;
+    }
+    return L + dmin(m, 0.0);
+}
+// qrot with the (fp32) prism rotation, in fp64
+DEV d3 qrot_d(double w, double qx, double qy, double qz, d3 v) {
+    d3 uv = {qy * v.z - qz * v.y, qz * v.x - qx * v.z, qx * v.y - qy * v.x};
+    d3 uuv = {qy * uv.z - qz * uv.y, qz * uv.x - qx * uv.z, qx * uv.y - qy * uv.x};
+    d3 o = {v.x + 2.0 * (w * uv.x + uuv.x), v.y + 2.0 * (w * uv.y + uuv.y), v.z + 2.0 * (w * uv.z + uuv.z)};
+    return o;
+}
+DEV double prism_sdf_d(const PrimC& P, d3 p, d3* g) {
+    q4 iq = conj_n(mkq(P.prot[0], P.prot[1], P.prot[2], P.prot[3]));
+    d3 gg = qrot_d(iq.w, iq.x, iq.y, iq.z, p);
+    double q0 = fabs(gg.x), q2 = fabs(gg.z);
+    double t1 = q2 - (double)P.ph[1];
+    double ta = q0 * (double)0.866025f + gg.y * 0.5;
+    double tb = -gg.y;
+    double t2 = dmax(ta, tb);
+    double t3 = t2 - (double)P.ph[0] * 0.5;
+    if (g) {
+        d3 ggg = {0.0, 0.0, 0.0};
+        if (t3 < t1) ggg.z = dsgn(gg.z);
+        else if (tb < ta) { ggg.x = dsgn(gg.x) * (double)0.866025f; ggg.y = 0.5; }
+        else ggg.y = -1.0;
+        // transpose of the rotation: qrot by the conjugate of the (unit) quaternion
+        *g = qrot_d(iq.w, -iq.x, -iq.y, -iq.z, ggg);
+    }
+    return dmax(t1, t3);
+}
+DEV double fd_sdf_d(const PrimC& P, int sh, d3 p, d3* g) {
+    if (sh == SH_BOX) return box_sdf_d(P, p, g);
+    if (sh == SH_PRISM) return prism_sdf_d(P, p, g);
+    d3 g1, g2;  // SH_KNIFE: max(prism, box)
+    double a = prism_sdf_d(P, p, g ? &g1 : nullptr), b = box_sdf_d(P, p, g ? &g2 : nullptr);
+    if (g) *g = (b < a) ? g1 : g2;
+    return dmax(a, b);
+}
+DEV v3 fd_normal_d(const PrimC& P, int sh, v3 pf) {
+    const double d = (double)1e-4f;
+    d3 p = {pf.x, pf.y, pf.z}, m;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        d3 inc = p, dec = p;
+        dsetc(inc, i, dcomp(p, i) + d);
+        dsetc(dec, i, dcomp(p, i) - d);
+        dsetc(m, i, (0.5 / d) * (fd_sdf_d(P, sh, inc, nullptr) - fd_sdf_d(P, sh, dec, nullptr)));
+    }
+    double L = sqrt(m.x * m.x + m.y * m.y + m.z * m.z + (double)LEN_EPS_SHAPE);
+    return mk((float)(m.x / L), (float)(m.y / L), (float)(m.z / L));
+}
+DEV v3 fd_normal_rev_d(const PrimC& P, int sh, v3 pf, v3 gnf) {
+    const double d = (double)1e-4f;
+    d3 p = {pf.x, pf.y, pf.z}, m, gi[3], gd[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        d3 inc = p, dec = p;
+        dsetc(inc, i, dcomp(p, i) + d);
+        dsetc(dec, i, dcomp(p, i) - d);
+        double a = fd_sdf_d(P, sh, inc, &gi[i]);
+        double b = fd_sdf_d(P, sh, dec, &gd[i]);
+        dsetc(m, i, (0.5 / d) * (a - b));
+    }
+    double gn[3] = {gnf.x, gnf.y, gnf.z};
+    double L = sqrt(m.x * m.x + m.y * m.y + m.z * m.z + (double)LEN_EPS_SHAPE);
+    double dd = m.x * gn[0] + m.y * gn[1] + m.z * gn[2], L3 = L * L * L;
+    double gp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double gm = (gn[i] / L - dcomp(m, i) * dd / L3) * (0.5 / d);
+        gp[0] += gm * (gi[i].x - gd[i].x);
+        gp[1] += gm * (gi[i].y - gd[i].y);
+        gp[2] += gm * (gi[i].z - gd[i].z);
+    }
+    return mk((float)gp[0], (float)gp[1], (float)gp[2]);
+}
+
 // normals: analytic for sphere / capsule / chopsticks, central FD with the
 // float32 step 1e-4 for box / prism / knife (primitives.py:382-393,720-731,796-807)
 DEV v3 local_normal(const PrimC& P, int sh, float gap, v3 p) {
@@ -415,17 +515,7 @@ DEV v3 local_normal(const PrimC& P, int sh, float gap, v3 p) {
         float a = capsule_sdf(P.h, P.r, pa, nullptr), b = capsule_sdf(P.h, P.r, pb, nullptr);
         return (a <= b) ? capsule_normal(P.h, pa) : capsule_normal(P.h, pb);
     }
-    const float d = 1e-4f;
-    v3 m;
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        v3 inc = p, dec = p;
-        setc(inc, i, comp(p, i) + d);
-        setc(dec, i, comp(p, i) - d);
-        setc(m, i, (0.5f / d) * (local_sdf(P, sh, gap, inc, nullptr, nullptr) - local_sdf(P, sh, gap, dec, nullptr, nullptr)));
-    }
-    float L = sqrtf(dot(m, m) + LEN_EPS_SHAPE);
-    return mk(m.x / L, m.y / L, m.z / L);
+    return fd_normal_d(P, sh, p);
 }
 DEV v3 local_normal_rev(const PrimC& P, int sh, float gap, v3 p, v3 gn, float* ggap) {
     if (sh == SH_CAPSULE) return capsule_normal_rev(P.h, p, gn);
@@ -442,25 +532,7 @@ DEV v3 local_normal_rev(const PrimC& P, int sh, float gap, v3 p, v3 gn, float* g
         else { g3 = capsule_normal_rev(P.h, pb, gn); *ggap += g3.x / 2.f; }
         return g3;
     }
-    const float d = 1e-4f;
-    v3 m, gi[3], gd[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        v3 inc = p, dec = p;
-        setc(inc, i, comp(p, i) + d);
-        setc(dec, i, comp(p, i) - d);
-        float a = local_sdf(P, sh, gap, inc, &gi[i], nullptr);
-        float b = local_sdf(P, sh, gap, dec, &gd[i], nullptr);
-        setc(m, i, (0.5f / d) * (a - b));
-    }
-    float L = sqrtf(dot(m, m) + LEN_EPS_SHAPE), dd = dot(m, gn), L3 = L * L * L;
-    v3 gp = mk(0.f, 0.f, 0.f);
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        float gm = comp(gn, i) / L - comp(m, i) * dd / L3;
-        gp += (gm * (0.5f / d)) * (gi[i] - gd[i]);
-    }
-    return gp;
+    return fd_normal_rev_d(P, sh, p, gn);
 }
 
 // ------------------------------------------------------------------ primitive frames

@@ -41,6 +41,7 @@ int fail(const std::string& m) { g_err = m; return -1; }
 constexpr int NST = 24;      // particle state rows
 constexpr int PPB = 256;     // particles per chunk / threads per particle block
 constexpr int NODES_PER_BLK = 64;
+constexpr int MAX_UNITS = 8;
 
 // ------------------------------------------------------------------ device helpers
 DEV long long node_id(const SceneC& S, int i, int j, int k) {
@@ -136,7 +137,6 @@ DEV void particle_stress(const SceneC& S, const m3& C, const m3& F, PTrace& t) {
 }
 
 // grid_op for one node with m > 1e-12 (mpm_simulator.py:230-262)
-constexpr int MAX_UNITS = 8;
 struct NodeTr { v3 vu[MAX_UNITS]; v3 vw[3]; };
 DEV v3 grid_node(const SceneC& S, const PrimC* prims, const float* ps0, const float* ps1, int I0, int I1, int I2,
                  float4 pm, NodeTr* tr) {
@@ -185,242 +185,516 @@ struct ChunkArgs {
     int stride;
 };
 
-// mark active grid blocks of frame f's particles; optionally run primitive
-// forward kinematics f -> f+1 (forward_kinematics, primive_base.py:152-156)
-__global__ void __launch_bounds__(PPB) k_mark(ChunkArgs ca, const float* __restrict__ st, int* __restrict__ blk_flag,
-                                              int* __restrict__ counters, int do_fk, const PrimC* __restrict__ prims,
-                                              int n_prims_total, float* __restrict__ pstate, const float* __restrict__ pvel, int f) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[0] = 0;
+// ------------------------------------------------------------------ binning
+// Each substep the particles of a frame are counting-sorted by (4x4x4 cell
+// block, cell): per-cell counts with wave-aggregated atomics, per-block scans,
+// then an index scatter.  A block's particles form one or more work items of
+// at most WI particles; the particle phases run one workgroup per work item
+// over a 6x6x6-node LDS tile (the stencil footprint of a cell block).
+constexpr int WI = 1024;     // particles per work item
+constexpr int TILE = 216;    // 6x6x6 nodes
+constexpr int WG = 256;      // threads per work-item workgroup
+constexpr int MAXSEG = 4;    // distinct cells per wave reduced through the transpose tree
+
+enum { CNT_PBLK = 0, CNT_NBLK = 1, CNT_PART = 2, CNT_ITEMS = 3, CNT_STAMP = 4 };
+
+struct BinArgs {
+    const SceneC* scenes;
+    const int4* chunks;   // (scene, first slot, count, -) per PPB particles
+    const int* blk_scene;
+    int stride;
+    int* cell_count;
+    int* cell_start;
+    int* bin_q;
+    int* bin_rank;
+    int* blk_flag;
+    int* node_flag;
+    int* counters;
+    int* pblocks;
+    int* blk_stamp;
+    int* blk_item0;
+    int* blk_nitems;
+    int4* items;
+    int* order;
+};
+
+DEV unsigned long long lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+
+DEV void block_origin(const SceneC& S, int gb, int& bi, int& bj, int& bk) {
+    int lb = gb - S.blk_off;
+    bk = lb % S.nb;
+    int t = lb / S.nb;
+    bj = t % S.nb;
+    bi = t / S.nb;
+}
+
+// bin frame f's particles by cell; optionally run primitive forward kinematics
+// f -> f+1 (forward_kinematics, primive_base.py:152-156)
+__global__ void __launch_bounds__(PPB) k_bin(BinArgs a, const float* __restrict__ st, int do_fk, const PrimC* __restrict__ prims,
+                                             int n_prims_total, float* __restrict__ pstate, const float* __restrict__ pvel, int f) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.counters[CNT_PBLK] = 0;
+        a.counters[CNT_NBLK] = 0;
+        a.counters[CNT_PART] = 0;
+        a.counters[CNT_ITEMS] = 0;
+        a.counters[CNT_STAMP] += 1;
+    }
     if (do_fk && blockIdx.x == 0 && (int)threadIdx.x < n_prims_total) {
         int q = threadIdx.x;
         prim_fk(prims[q], pstate + ((size_t)f * n_prims_total + q) * 8, pvel + ((size_t)f * n_prims_total + q) * 8,
                 pstate + ((size_t)(f + 1) * n_prims_total + q) * 8);
     }
-    int4 ch = ca.chunks[blockIdx.x];
+    int4 ch = a.chunks[blockIdx.x];
     int i = threadIdx.x;
-    if (i >= ch.z) return;
-    const SceneC& S = ca.scenes[ch.x];
+    int lane = i & 63;
+    bool valid = i < ch.z;
     int slot = ch.y + i;
-    v3 x = load3(st, ca.stride, 0, slot);
-    int lo[3], hi[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        int b = (int)(comp(x, d) * S.inv_dx - 0.5f);
-        int a0 = max(b, 0), a1 = min(b + 2, S.n_grid - 1);
-        lo[d] = a0 >> 2;
-        hi[d] = a1 >> 2;
+    int q = -1;
+    if (valid) {
+        const SceneC& S = a.scenes[ch.x];
+        v3 x = load3(st, a.stride, 0, slot);
+        int b0 = min(max((int)(x.x * S.inv_dx - 0.5f), 0), S.n_grid - 1);
+        int b1 = min(max((int)(x.y * S.inv_dx - 0.5f), 0), S.n_grid - 1);
+        int b2 = min(max((int)(x.z * S.inv_dx - 0.5f), 0), S.n_grid - 1);
+        int gb = S.blk_off + ((b0 >> 2) * S.nb + (b1 >> 2)) * S.nb + (b2 >> 2);
+        q = gb * 64 + (((b0 & 3) << 4) | ((b1 & 3) << 2) | (b2 & 3));
+        a.blk_flag[gb] = 1;
     }
-    for (int bi = lo[0]; bi <= hi[0]; bi++)
-        for (int bj = lo[1]; bj <= hi[1]; bj++)
-            for (int bk = lo[2]; bk <= hi[2]; bk++)
-                blk_flag[S.blk_off + (bi * S.nb + bj) * S.nb + bk] = 1;
+    unsigned long long act = __ballot(valid);
+    int rank = 0;
+    while (act) {
+        int lead = __ffsll((long long)act) - 1;
+        int q0 = __shfl(q, lead);
+        unsigned long long m = __ballot(valid && q == q0);
+        int base = 0;
+        if (lane == lead) base = atomicAdd(&a.cell_count[q0], __popcll(m));
+        base = __shfl(base, lead);
+        if (valid && q == q0) rank = base + __popcll(m & lanemask_lt(lane));
+        act &= ~m;
+    }
+    if (valid) {
+        a.bin_q[slot] = q;
+        a.bin_rank[slot] = rank;
+    }
 }
 
-// compact flagged grid blocks into the active list (wave-aggregated atomics)
-__global__ void __launch_bounds__(256) k_compact(int n_blocks, int* __restrict__ blk_flag, int* __restrict__ active,
-                                                 int* __restrict__ counters) {
+// compact flagged blocks into a list (wave-aggregated atomics); clears the flags
+__global__ void __launch_bounds__(256) k_compact(int n_blocks, int* __restrict__ flag, int* __restrict__ list,
+                                                 int* __restrict__ counter) {
     for (int b0 = blockIdx.x * blockDim.x; b0 < n_blocks; b0 += gridDim.x * blockDim.x) {
         int b = b0 + threadIdx.x;
-        int fl = (b < n_blocks) ? blk_flag[b] : 0;
+        int fl = (b < n_blocks) ? flag[b] : 0;
         unsigned long long mask = __ballot(fl != 0);
         if (mask == 0ull) continue;
         int lane = threadIdx.x & 63;
         int leader = __ffsll((long long)mask) - 1;
         int base = 0;
-        if (lane == leader) base = atomicAdd(&counters[0], __popcll(mask));
+        if (lane == leader) base = atomicAdd(counter, __popcll(mask));
         base = __shfl(base, leader);
         if (fl) {
-            int rank = __popcll(mask & ((1ull << lane) - 1ull));
-            active[base + rank] = b;
-            blk_flag[b] = 0;
+            list[base + __popcll(mask & lanemask_lt(lane))] = b;
+            flag[b] = 0;
         }
     }
 }
 
-// p2g: constitutive update + 27-node scatter of momentum and mass
-// (mpm_simulator.py:198-225).  st_out != null also stores F of frame f+1.
-__global__ void __launch_bounds__(PPB) k_p2g(ChunkArgs ca, const float* __restrict__ st, float* __restrict__ st_out,
-                                             float4* __restrict__ g_pm) {
-    int4 ch = ca.chunks[blockIdx.x];
+// per occupied cell block (one wave, lane = cell): scan the cell counts, claim
+// a particle range and work items, stamp the block, mark the node blocks its
+// 6x6x6 footprint covers
+__global__ void __launch_bounds__(256) k_blocks(BinArgs a) {
+    int lane = threadIdx.x & 63;
+    int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    int nw = (gridDim.x * blockDim.x) >> 6;
+    int n = a.counters[CNT_PBLK];
+    int stamp = a.counters[CNT_STAMP];
+    for (int e = wave; e < n; e += nw) {
+        int gb = a.pblocks[e];
+        int cnt = a.cell_count[gb * 64 + lane];
+        a.cell_count[gb * 64 + lane] = 0;
+        int incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            int t = __shfl_up(incl, d);
+            if (lane >= d) incl += t;
+        }
+        int total = __shfl(incl, 63);
+        int nit = (total + WI - 1) / WI;
+        int start = 0, ib = 0;
+        if (lane == 0) {
+            start = atomicAdd(&a.counters[CNT_PART], total);
+            ib = atomicAdd(&a.counters[CNT_ITEMS], nit);
+        }
+        start = __shfl(start, 0);
+        ib = __shfl(ib, 0);
+        a.cell_start[gb * 64 + lane] = start + incl - cnt;
+        for (int it = lane; it < nit; it += 64) a.items[ib + it] = make_int4(gb, start + it * WI, min(WI, total - it * WI), 0);
+        if (lane == 0) {
+            a.blk_stamp[gb] = stamp;
+            a.blk_item0[gb] = ib;
+            a.blk_nitems[gb] = nit;
+        }
+        if (lane < 8) {
+            const SceneC& S = a.scenes[a.blk_scene[gb]];
+            int bi, bj, bk;
+            block_origin(S, gb, bi, bj, bk);
+            int ni = bi + ((lane >> 2) & 1), nj = bj + ((lane >> 1) & 1), nk = bk + (lane & 1);
+            if (ni < S.nb && nj < S.nb && nk < S.nb) a.node_flag[S.blk_off + (ni * S.nb + nj) * S.nb + nk] = 1;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(PPB) k_scatter(BinArgs a) {
+    int4 ch = a.chunks[blockIdx.x];
     int i = threadIdx.x;
     if (i >= ch.z) return;
-    const SceneC& S = ca.scenes[ch.x];
     int slot = ch.y + i;
-    int stride = ca.stride;
-    v3 x = load3(st, stride, 0, slot);
-    v3 v = load3(st, stride, 3, slot);
-    m3 C = load9(st, stride, 6, slot);
-    m3 F = load9(st, stride, 15, slot);
-    PTrace t;
-    particle_stress(S, C, F, t);
-    if (st_out) store9(st_out, stride, 15, slot, t.Fn);
-    Stencil sw;
-    stencil(S, x, sw);
-    v3 mv = S.p_mass * v;
+    int q = a.bin_q[slot];
+    a.order[a.cell_start[q] + a.bin_rank[slot]] = slot;
+}
+
+// ------------------------------------------------------------------ wave transpose reduction
+// Sums 32 per-lane values over the 64 lanes in 6 butterfly steps (31 + 1
+// shuffles instead of 32 x 6): after it, lanes 2j and 2j+1 hold the total of
+// value j.
+template <int H, int MASK>
+DEV void tstep(float* r, int lane) {
+    bool up = (lane & MASK) != 0;
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+    for (int j = 0; j < H; j++) {
+        float lo = r[j], hi = r[j + H];
+        float send = up ? lo : hi;
+        float keep = up ? hi : lo;
+        r[j] = keep + __shfl_xor(send, MASK);
+    }
+}
+DEV float tsum32(float* r, int lane) {
+    tstep<16, 32>(r, lane);
+    tstep<8, 16>(r, lane);
+    tstep<4, 8>(r, lane);
+    tstep<2, 4>(r, lane);
+    tstep<1, 2>(r, lane);
+    return r[0] + __shfl_xor(r[0], 1);
+}
+DEV int tile_index(int l0, int l1, int l2, int n) { return ((l0 + n / 9) * 6 + (l1 + (n / 3) % 3)) * 6 + (l2 + n % 3); }
+
+// Segmented scatter of 27 x 4 per-particle node values into an LDS tile
+// (component-major, TILE floats per component).  Lanes that share a cell are
+// summed through the transpose tree (at most MAXSEG distinct cells per wave),
+// the rest add directly.  VAL(lane_node, comp) gives a lane's value.
+template <int NCOMP, class ValFn>
+DEV void scatter_tile(float* tile, bool valid, int key, int l0, int l1, int l2, int lane, ValFn val) {
+    unsigned long long act = __ballot(valid);
+    int seg = 0;
+    while (act) {
+        if (seg == MAXSEG) {
+            if ((act >> lane) & 1ull) {
 #pragma unroll
-        for (int b = 0; b < 3; b++)
+                for (int n = 0; n < 27; n++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                int I0 = sw.base[0] + a, I1 = sw.base[1] + b, I2 = sw.base[2] + c;
-                if (!in_grid(S, I0, I1, I2)) continue;
-                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
-                float d0 = ((float)a - sw.fx[0]) * S.dx, d1 = ((float)b - sw.fx[1]) * S.dx, d2 = ((float)c - sw.fx[2]) * S.dx;
-                float4* g = g_pm + node_id(S, I0, I1, I2);
-                const float* A = t.affine.a;
-                unsafeAtomicAdd(&g->x, wt * (mv.x + A[0] * d0 + A[1] * d1 + A[2] * d2));
-                unsafeAtomicAdd(&g->y, wt * (mv.y + A[3] * d0 + A[4] * d1 + A[5] * d2));
-                unsafeAtomicAdd(&g->z, wt * (mv.z + A[6] * d0 + A[7] * d1 + A[8] * d2));
-                unsafeAtomicAdd(&g->w, wt * S.p_mass);
+                    for (int c = 0; c < NCOMP; c++) atomicAdd(&tile[c * TILE + tile_index(l0, l1, l2, n)], val(n, c));
             }
+            break;
+        }
+        int lead = __ffsll((long long)act) - 1;
+        int key0 = __shfl(key, lead);
+        unsigned long long m = __ballot(valid && key == key0);
+        bool in = (m >> lane) & 1ull;
+        int s0 = __shfl(l0, lead), s1 = __shfl(l1, lead), s2 = __shfl(l2, lead);
+#pragma unroll
+        for (int pass = 0; pass < 4; pass++) {
+            float r[32];
+#pragma unroll
+            for (int j = 0; j < 32; j++) {
+                int n = pass * 7 + j / 4, c = j & 3;
+                r[j] = (in && j < 28 && n < 27 && c < NCOMP) ? val(n, c) : 0.f;
+            }
+            float s = tsum32(r, lane);
+            int j = lane >> 1;
+            int n = pass * 7 + j / 4, c = j & 3;
+            if (!(lane & 1) && j < 28 && n < 27 && c < NCOMP) atomicAdd(&tile[c * TILE + tile_index(s0, s1, s2, n)], s);
+        }
+        act &= ~m;
+        seg++;
+    }
+}
+
+struct TileArgs {
+    const SceneC* scenes;
+    const int* blk_scene;
+    const int4* items;
+    const int* counters;
+    const int* order;
+    int stride;
+};
+
+// p2g over work items: constitutive update + scatter into the block's LDS tile,
+// flushed to the item's tile buffer (mpm_simulator.py:198-225).
+// st_out != null also stores F of frame f+1.
+__global__ void __launch_bounds__(WG) k_p2g_tile(TileArgs ta, const float* __restrict__ st, float* __restrict__ st_out,
+                                                 float4* __restrict__ tiles) {
+    __shared__ float tile[4 * TILE];
+    int n_items = ta.counters[CNT_ITEMS];
+    int lane = threadIdx.x & 63;
+    int stride = ta.stride;
+    for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
+        int4 it = ta.items[e];
+        const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
+        int bi, bj, bk;
+        block_origin(S, it.x, bi, bj, bk);
+        for (int k = threadIdx.x; k < 4 * TILE; k += WG) tile[k] = 0.f;
+        __syncthreads();
+        for (int k0 = 0; k0 < it.z; k0 += WG) {
+            int k = k0 + threadIdx.x;
+            bool valid = k < it.z;
+            int p = valid ? ta.order[it.y + k] : ta.order[it.y];
+            v3 x = load3(st, stride, 0, p);
+            v3 v = load3(st, stride, 3, p);
+            m3 C = load9(st, stride, 6, p);
+            m3 F = load9(st, stride, 15, p);
+            PTrace t;
+            particle_stress(S, C, F, t);
+            if (valid && st_out) store9(st_out, stride, 15, p, t.Fn);
+            Stencil sw;
+            stencil(S, x, sw);
+            int l0 = sw.base[0] - 4 * bi, l1 = sw.base[1] - 4 * bj, l2 = sw.base[2] - 4 * bk;
+            bool in_tile = (unsigned)l0 < 4u && (unsigned)l1 < 4u && (unsigned)l2 < 4u;
+            int key = (l0 << 4) | (l1 << 2) | l2;
+            v3 mv = S.p_mass * v;
+            const float* A = t.affine.a;
+            float dx = S.dx, pm = S.p_mass;
+            auto val = [&](int n, int c) -> float {
+                int a = n / 9, b = (n / 3) % 3, cc = n % 3;
+                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[cc][2];
+                if (c == 3) return wt * pm;
+                float d0 = ((float)a - sw.fx[0]) * dx, d1 = ((float)b - sw.fx[1]) * dx, d2 = ((float)cc - sw.fx[2]) * dx;
+                return wt * (comp(mv, c) + A[c * 3] * d0 + A[c * 3 + 1] * d1 + A[c * 3 + 2] * d2);
+            };
+            scatter_tile<4>(tile, valid && in_tile, key, l0, l1, l2, lane, val);
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < TILE; k += WG)
+            tiles[(size_t)e * TILE + k] = make_float4(tile[k], tile[TILE + k], tile[2 * TILE + k], tile[3 * TILE + k]);
+        __syncthreads();
+    }
 }
 
 struct GridArgs {
     const SceneC* scenes;
     const PrimC* prims;
     const int* blk_scene;
-    const int* active;
+    const int* nblocks;
     const int* counters;
+    const int* blk_stamp;
+    const int* blk_item0;
+    const int* blk_nitems;
     const float* pstate;   // [frame][n_prims_total][8]
     int n_prims_total;
     int f;
 };
 
-DEV void block_coords(const SceneC& S, int gb, int lane, int& I0, int& I1, int& I2, long long& node) {
-    int lb = gb - S.blk_off;
-    int bk = lb % S.nb, t = lb / S.nb, bj = t % S.nb, bi = t / S.nb;
-    I0 = bi * 4 + (lane >> 4);
-    I1 = bj * 4 + ((lane >> 2) & 3);
-    I2 = bk * 4 + (lane & 3);
-    node = S.g_off + (long long)lb * NODES_PER_BLK + lane;
+DEV void node_coords(const SceneC& S, int gb, int lane, int& B0, int& B1, int& B2, int& I0, int& I1, int& I2, long long& node) {
+    block_origin(S, gb, B0, B1, B2);
+    I0 = B0 * 4 + (lane >> 4);
+    I1 = B1 * 4 + ((lane >> 2) & 3);
+    I2 = B2 * 4 + (lane & 3);
+    node = S.g_off + (long long)(gb - S.blk_off) * NODES_PER_BLK + lane;
 }
 
-// grid_op over the active blocks; one wave per 4x4x4 block.  clear_pm zeroes
-// the accumulated momentum/mass after use (forward), leaving the grid clean
-// for the next substep.
-__global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, float4* __restrict__ g_pm, float4* __restrict__ g_v, int clear_pm) {
-    int n_active = ga.counters[0];
+// owner-computes gather: the node's value is the sum, in a fixed order, of the
+// tiles of the (up to 8) cell blocks whose footprint covers it
+DEV float4 gather_tiles(const GridArgs& ga, const SceneC& S, int B0, int B1, int B2, int lane, const float4* __restrict__ tiles,
+                        int stamp) {
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    int l0 = lane >> 4, l1 = (lane >> 2) & 3, l2 = lane & 3;
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+        int d0 = (d >> 2) & 1, d1 = (d >> 1) & 1, d2 = d & 1;
+        int P0 = B0 - d0, P1 = B1 - d1, P2 = B2 - d2;
+        if (P0 < 0 || P1 < 0 || P2 < 0) continue;
+        int pgb = S.blk_off + (P0 * S.nb + P1) * S.nb + P2;
+        if (ga.blk_stamp[pgb] != stamp) continue;
+        int t0 = l0 + 4 * d0, t1 = l1 + 4 * d1, t2 = l2 + 4 * d2;
+        if (t0 >= 6 || t1 >= 6 || t2 >= 6) continue;
+        int tidx = (t0 * 6 + t1) * 6 + t2;
+        int i0 = ga.blk_item0[pgb], ni = ga.blk_nitems[pgb];
+        for (int j = 0; j < ni; j++) {
+            float4 v = tiles[(size_t)(i0 + j) * TILE + tidx];
+            acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+        }
+    }
+    return acc;
+}
+
+// grid_op over the active node blocks (mpm_simulator.py:230-262); one wave per
+// 4x4x4 node block.  Writes the gathered momentum/mass (kept for the adjoint)
+// and the velocity.
+__global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
+                                                 float4* __restrict__ g_v) {
+    int n_active = ga.counters[CNT_NBLK];
+    int stamp = ga.counters[CNT_STAMP];
     int lane = threadIdx.x & 63;
     int wave = threadIdx.x >> 6;
     for (int e = blockIdx.x * 4 + wave; e < n_active; e += gridDim.x * 4) {
-        int gb = ga.active[e];
+        int gb = ga.nblocks[e];
         const SceneC& S = ga.scenes[ga.blk_scene[gb]];
-        int I0, I1, I2;
+        int B0, B1, B2, I0, I1, I2;
         long long node;
-        block_coords(S, gb, lane, I0, I1, I2, node);
-        float4 pm = g_pm[node];
+        node_coords(S, gb, lane, B0, B1, B2, I0, I1, I2, node);
+        float4 pm = gather_tiles(ga, S, B0, B1, B2, lane, tiles, stamp);
         v3 v = mk(0.f, 0.f, 0.f);
         if (pm.w > 1e-12f && in_grid(S, I0, I1, I2)) {
             const float* ps0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
             const float* ps1 = ga.pstate + ((size_t)(ga.f + 1) * ga.n_prims_total + S.prim_off) * 8;
             v = grid_node(S, ga.prims + S.prim_off, ps0, ps1, I0, I1, I2, pm, nullptr);
         }
+        g_pm[node] = pm;
         g_v[node] = make_float4(v.x, v.y, v.z, 0.f);
-        if (clear_pm) g_pm[node] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 
-// g2p: 27-node gather, APIC C, advection with clamp (mpm_simulator.py:264-283)
-__global__ void __launch_bounds__(PPB) k_g2p(ChunkArgs ca, const float* __restrict__ st, float* __restrict__ st_out,
-                                             const float4* __restrict__ g_v) {
-    int4 ch = ca.chunks[blockIdx.x];
-    int i = threadIdx.x;
-    if (i >= ch.z) return;
-    const SceneC& S = ca.scenes[ch.x];
-    int slot = ch.y + i;
-    int stride = ca.stride;
-    v3 x = load3(st, stride, 0, slot);
-    Stencil sw;
-    stencil(S, x, sw);
-    v3 nv = mk(0.f, 0.f, 0.f);
-    m3 nC = mzero();
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                int I0 = sw.base[0] + a, I1 = sw.base[1] + b, I2 = sw.base[2] + c;
-                if (!in_grid(S, I0, I1, I2)) continue;
-                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
-                float4 g = g_v[node_id(S, I0, I1, I2)];
-                float dp[3] = {(float)a - sw.fx[0], (float)b - sw.fx[1], (float)c - sw.fx[2]};
-                nv += wt * mk(g.x, g.y, g.z);
-                float cw = S.c4 * wt;
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    nC.a[j] += cw * g.x * dp[j];
-                    nC.a[3 + j] += cw * g.y * dp[j];
-                    nC.a[6 + j] += cw * g.z * dp[j];
-                }
-            }
-    store3(st_out, stride, 3, slot, nv);
-    store9(st_out, stride, 6, slot, nC);
-    v3 xn = mk(tmax(tmin(x.x + S.dt * nv.x, S.upper), S.lower), tmax(tmin(x.y + S.dt * nv.y, S.upper), S.lower),
-               tmax(tmin(x.z + S.dt * nv.z, S.upper), S.lower));
-    store3(st_out, stride, 0, slot, xn);
+// load the 6x6x6 node tile of a cell block from a node grid into LDS
+DEV void load_tile(float4* tv, const SceneC& S, int bi, int bj, int bk, const float4* __restrict__ g) {
+    for (int k = threadIdx.x; k < TILE; k += WG) {
+        int t0 = k / 36, t1 = (k / 6) % 6, t2 = k % 6;
+        int I0 = bi * 4 + t0, I1 = bj * 4 + t1, I2 = bk * 4 + t2;
+        tv[k] = in_grid(S, I0, I1, I2) ? g[node_id(S, I0, I1, I2)] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
 }
 
-// ---- adjoint kernels ------------------------------------------------------
-
-// reverse of g2p: scatter the adjoint of grid_v_out, write the first part of
-// the adjoint of x (frame f).  a1 = adjoint of frame f+1, st1 = frame f+1.
-__global__ void __launch_bounds__(PPB) k_g2p_bwd(ChunkArgs ca, const float* __restrict__ st, const float* __restrict__ st1,
-                                                 const float* __restrict__ a1, float* __restrict__ a0,
-                                                 const float4* __restrict__ g_v, float4* __restrict__ g_adjv) {
-    int4 ch = ca.chunks[blockIdx.x];
-    int i = threadIdx.x;
-    if (i >= ch.z) return;
-    const SceneC& S = ca.scenes[ch.x];
-    int slot = ch.y + i;
-    int stride = ca.stride;
-    v3 x = load3(st, stride, 0, slot);
-    v3 x1 = load3(st1, stride, 0, slot);
-    v3 gx1 = load3(a1, stride, 0, slot);
-    v3 gv1 = load3(a1, stride, 3, slot);
-    m3 gC = load9(a1, stride, 6, slot);
-    // clamp mask from the stored output position: the adjoint passes iff lower < x' < upper
-    v3 gxn = mk((S.lower < x1.x && x1.x < S.upper) ? gx1.x : 0.f, (S.lower < x1.y && x1.y < S.upper) ? gx1.y : 0.f,
-                (S.lower < x1.z && x1.z < S.upper) ? gx1.z : 0.f);
-    v3 gnv = gv1 + S.dt * gxn;
-    Stencil sw;
-    stencil(S, x, sw);
-    float gfx[3] = {0.f, 0.f, 0.f};
+// g2p over work items: 27-node gather from the LDS tile, APIC C, advection
+// with clamp (mpm_simulator.py:264-283)
+__global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __restrict__ st, float* __restrict__ st_out,
+                                                 const float4* __restrict__ g_v) {
+    __shared__ float4 tv[TILE];
+    int n_items = ta.counters[CNT_ITEMS];
+    int stride = ta.stride;
+    for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
+        int4 it = ta.items[e];
+        const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
+        int bi, bj, bk;
+        block_origin(S, it.x, bi, bj, bk);
+        load_tile(tv, S, bi, bj, bk, g_v);
+        __syncthreads();
+        for (int k = threadIdx.x; k < it.z; k += WG) {
+            int p = ta.order[it.y + k];
+            v3 x = load3(st, stride, 0, p);
+            Stencil sw;
+            stencil(S, x, sw);
+            int l0 = sw.base[0] - 4 * bi, l1 = sw.base[1] - 4 * bj, l2 = sw.base[2] - 4 * bk;
+            bool in_tile = (unsigned)l0 < 4u && (unsigned)l1 < 4u && (unsigned)l2 < 4u;
+            v3 nv = mk(0.f, 0.f, 0.f);
+            m3 nC = mzero();
+            if (in_tile) {
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+                for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++)
+                    for (int b = 0; b < 3; b++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                int I0 = sw.base[0] + a, I1 = sw.base[1] + b, I2 = sw.base[2] + c;
-                if (!in_grid(S, I0, I1, I2)) continue;
-                long long nid = node_id(S, I0, I1, I2);
-                float4 g4 = g_v[nid];
-                v3 g = mk(g4.x, g4.y, g4.z);
-                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
-                float dp[3] = {(float)a - sw.fx[0], (float)b - sw.fx[1], (float)c - sw.fx[2]};
-                v3 Cd = mk(gC.a[0] * dp[0] + gC.a[1] * dp[1] + gC.a[2] * dp[2], gC.a[3] * dp[0] + gC.a[4] * dp[1] + gC.a[5] * dp[2],
-                           gC.a[6] * dp[0] + gC.a[7] * dp[1] + gC.a[8] * dp[2]);
-                v3 CTg = mk(gC.a[0] * g.x + gC.a[3] * g.y + gC.a[6] * g.z, gC.a[1] * g.x + gC.a[4] * g.y + gC.a[7] * g.z,
-                            gC.a[2] * g.x + gC.a[5] * g.y + gC.a[8] * g.z);
-                float gw = dot(gnv, g) + S.c4 * dot(g, Cd);
-                v3 contrib = wt * gnv + (S.c4 * wt) * Cd;
-                float4* ga = g_adjv + nid;
-                unsafeAtomicAdd(&ga->x, contrib.x);
-                unsafeAtomicAdd(&ga->y, contrib.y);
-                unsafeAtomicAdd(&ga->z, contrib.z);
-                int off[3] = {a, b, c};
+                        for (int c = 0; c < 3; c++) {
+                            float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
+                            float4 g = tv[((l0 + a) * 6 + (l1 + b)) * 6 + (l2 + c)];
+                            float dp[3] = {(float)a - sw.fx[0], (float)b - sw.fx[1], (float)c - sw.fx[2]};
+                            nv += wt * mk(g.x, g.y, g.z);
+                            float cw = S.c4 * wt;
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    float dwd = sw.dw[off[d]][d];
-#pragma unroll
-                    for (int e2 = 0; e2 < 3; e2++)
-                        if (e2 != d) dwd *= sw.w[off[e2]][e2];
-                    gfx[d] += gw * dwd - S.c4 * wt * comp(CTg, d);
-                }
+                            for (int j = 0; j < 3; j++) {
+                                nC.a[j] += cw * g.x * dp[j];
+                                nC.a[3 + j] += cw * g.y * dp[j];
+                                nC.a[6 + j] += cw * g.z * dp[j];
+                            }
+                        }
             }
-    store3(a0, stride, 0, slot, gxn + S.inv_dx * mk(gfx[0], gfx[1], gfx[2]));
+            store3(st_out, stride, 3, p, nv);
+            store9(st_out, stride, 6, p, nC);
+            v3 xn = mk(tmax(tmin(x.x + S.dt * nv.x, S.upper), S.lower), tmax(tmin(x.y + S.dt * nv.y, S.upper), S.lower),
+                       tmax(tmin(x.z + S.dt * nv.z, S.upper), S.lower));
+            store3(st_out, stride, 0, p, xn);
+        }
+        __syncthreads();
+    }
+}
+
+// reverse of g2p over work items: scatter the adjoint of grid_v_out into the
+// item's tile buffer, write the first part of the adjoint of x (frame f).
+// a1 = adjoint of frame f+1, st1 = frame f+1.
+__global__ void __launch_bounds__(WG) k_g2p_bwd_tile(TileArgs ta, const float* __restrict__ st, const float* __restrict__ st1,
+                                                     const float* __restrict__ a1, float* __restrict__ a0,
+                                                     const float4* __restrict__ g_v, float4* __restrict__ tiles_v) {
+    __shared__ float4 tv[TILE];
+    __shared__ float acc[4 * TILE];
+    int n_items = ta.counters[CNT_ITEMS];
+    int lane = threadIdx.x & 63;
+    int stride = ta.stride;
+    for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
+        int4 it = ta.items[e];
+        const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
+        int bi, bj, bk;
+        block_origin(S, it.x, bi, bj, bk);
+        load_tile(tv, S, bi, bj, bk, g_v);
+        for (int k = threadIdx.x; k < 4 * TILE; k += WG) acc[k] = 0.f;
+        __syncthreads();
+        for (int k0 = 0; k0 < it.z; k0 += WG) {
+            int k = k0 + threadIdx.x;
+            bool valid = k < it.z;
+            int p = valid ? ta.order[it.y + k] : ta.order[it.y];
+            v3 x = load3(st, stride, 0, p);
+            v3 x1 = load3(st1, stride, 0, p);
+            v3 gx1 = load3(a1, stride, 0, p);
+            v3 gv1 = load3(a1, stride, 3, p);
+            m3 gC = load9(a1, stride, 6, p);
+            // clamp mask from the stored output position: the adjoint passes iff lower < x' < upper
+            v3 gxn = mk((S.lower < x1.x && x1.x < S.upper) ? gx1.x : 0.f, (S.lower < x1.y && x1.y < S.upper) ? gx1.y : 0.f,
+                        (S.lower < x1.z && x1.z < S.upper) ? gx1.z : 0.f);
+            v3 gnv = gv1 + S.dt * gxn;
+            Stencil sw;
+            stencil(S, x, sw);
+            int l0 = sw.base[0] - 4 * bi, l1 = sw.base[1] - 4 * bj, l2 = sw.base[2] - 4 * bk;
+            bool in_tile = (unsigned)l0 < 4u && (unsigned)l1 < 4u && (unsigned)l2 < 4u;
+            float gfx[3] = {0.f, 0.f, 0.f};
+            if (in_tile) {
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++)
+#pragma unroll
+                        for (int c = 0; c < 3; c++) {
+                            float4 g4 = tv[((l0 + a) * 6 + (l1 + b)) * 6 + (l2 + c)];
+                            v3 g = mk(g4.x, g4.y, g4.z);
+                            float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
+                            float dp[3] = {(float)a - sw.fx[0], (float)b - sw.fx[1], (float)c - sw.fx[2]};
+                            v3 Cd = mk(gC.a[0] * dp[0] + gC.a[1] * dp[1] + gC.a[2] * dp[2],
+                                       gC.a[3] * dp[0] + gC.a[4] * dp[1] + gC.a[5] * dp[2],
+                                       gC.a[6] * dp[0] + gC.a[7] * dp[1] + gC.a[8] * dp[2]);
+                            v3 CTg = mk(gC.a[0] * g.x + gC.a[3] * g.y + gC.a[6] * g.z, gC.a[1] * g.x + gC.a[4] * g.y + gC.a[7] * g.z,
+                                        gC.a[2] * g.x + gC.a[5] * g.y + gC.a[8] * g.z);
+                            float gw = dot(gnv, g) + S.c4 * dot(g, Cd);
+                            int off[3] = {a, b, c};
+#pragma unroll
+                            for (int d = 0; d < 3; d++) {
+                                float dwd = sw.dw[off[d]][d];
+#pragma unroll
+                                for (int e2 = 0; e2 < 3; e2++)
+                                    if (e2 != d) dwd *= sw.w[off[e2]][e2];
+                                gfx[d] += gw * dwd - S.c4 * wt * comp(CTg, d);
+                            }
+                        }
+            }
+            if (valid) store3(a0, stride, 0, p, gxn + S.inv_dx * mk(gfx[0], gfx[1], gfx[2]));
+            int key = (l0 << 4) | (l1 << 2) | l2;
+            float c4 = S.c4;
+            auto val = [&](int n, int c) -> float {
+                int a = n / 9, b = (n / 3) % 3, cc = n % 3;
+                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[cc][2];
+                float dp0 = (float)a - sw.fx[0], dp1 = (float)b - sw.fx[1], dp2 = (float)cc - sw.fx[2];
+                float Cd = gC.a[c * 3] * dp0 + gC.a[c * 3 + 1] * dp1 + gC.a[c * 3 + 2] * dp2;
+                return wt * comp(gnv, c) + (c4 * wt) * Cd;
+            };
+            scatter_tile<3>(acc, valid && in_tile, key, l0, l1, l2, lane, val);
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < TILE; k += WG) tiles_v[(size_t)e * TILE + k] = make_float4(acc[k], acc[TILE + k], acc[2 * TILE + k], 0.f);
+        __syncthreads();
+    }
 }
 
 DEV float wave_sum(float v) {
@@ -431,22 +705,20 @@ DEV float wave_sum(float v) {
 
 // reverse of grid_op: adjoint of (momentum, mass) per node, plus the adjoint
 // of the primitive states of frames f and f+1 (wave-reduced, then atomics).
-// Clears g_pm and g_adjv of the visited blocks.
-__global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, float4* __restrict__ g_pm, float4* __restrict__ g_adjv,
+__global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* __restrict__ g_pm, const float4* __restrict__ tiles_v,
                                                      float4* __restrict__ g_adjpm, float* __restrict__ gpstate) {
-    int n_active = ga.counters[0];
+    int n_active = ga.counters[CNT_NBLK];
+    int stamp = ga.counters[CNT_STAMP];
     int lane = threadIdx.x & 63;
     int wave = threadIdx.x >> 6;
     for (int e = blockIdx.x * 4 + wave; e < n_active; e += gridDim.x * 4) {
-        int gb = ga.active[e];
+        int gb = ga.nblocks[e];
         const SceneC& S = ga.scenes[ga.blk_scene[gb]];
-        int I0, I1, I2;
+        int B0, B1, B2, I0, I1, I2;
         long long node;
-        block_coords(S, gb, lane, I0, I1, I2, node);
+        node_coords(S, gb, lane, B0, B1, B2, I0, I1, I2, node);
         float4 pm = g_pm[node];
-        float4 gv4 = g_adjv[node];
-        g_pm[node] = make_float4(0.f, 0.f, 0.f, 0.f);
-        g_adjv[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 gv4 = gather_tiles(ga, S, B0, B1, B2, lane, tiles_v, stamp);
         bool act = pm.w > 1e-12f && in_grid(S, I0, I1, I2);
         const float* ps0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
         const float* ps1 = ga.pstate + ((size_t)(ga.f + 1) * ga.n_prims_total + S.prim_off) * 8;
@@ -557,128 +829,140 @@ DEV m3 backward_svd(const m3& gu, const float gs[3], const m3& gv, const m3& u, 
     return out;
 }
 
-// reverse of p2g + svd_grad + compute_F_tmp.grad (mpm_simulator.py:121-156,198-225);
-// block 0 also runs the primitive FK adjoint of substep f.
-__global__ void __launch_bounds__(PPB) k_p2g_bwd(ChunkArgs ca, const float* __restrict__ st, const float* __restrict__ a1,
-                                                 float* __restrict__ a0, const float4* __restrict__ g_adjpm,
-                                                 const PrimC* __restrict__ prims, int n_prims_total, const float* __restrict__ pstate,
-                                                 const float* __restrict__ pvel, float* __restrict__ gpstate,
-                                                 float* __restrict__ gpvel, int f) {
+// reverse of p2g + svd_grad + compute_F_tmp.grad over work items
+// (mpm_simulator.py:121-156,198-225), gathering the node adjoints from the
+// block's LDS tile; workgroup 0 also runs the primitive FK adjoint of substep f.
+__global__ void __launch_bounds__(WG) k_p2g_bwd_tile(TileArgs ta, const float* __restrict__ st, const float* __restrict__ a1,
+                                                     float* __restrict__ a0, const float4* __restrict__ g_adjpm,
+                                                     const PrimC* __restrict__ prims, int n_prims_total,
+                                                     const float* __restrict__ pstate, const float* __restrict__ pvel,
+                                                     float* __restrict__ gpstate, float* __restrict__ gpvel, int f) {
+    __shared__ float4 tg[TILE];
     if (blockIdx.x == 0 && (int)threadIdx.x < n_prims_total) {
         int q = threadIdx.x;
         size_t o0 = ((size_t)f * n_prims_total + q) * 8, o1 = ((size_t)(f + 1) * n_prims_total + q) * 8;
         prim_fk_rev(prims[q], pstate + o0, pvel + o0, gpstate + o1, gpstate + o0, gpvel + o0);
     }
-    int4 ch = ca.chunks[blockIdx.x];
-    int i = threadIdx.x;
-    if (i >= ch.z) return;
-    const SceneC& S = ca.scenes[ch.x];
-    int slot = ch.y + i;
-    int stride = ca.stride;
-    v3 x = load3(st, stride, 0, slot);
-    v3 v = load3(st, stride, 3, slot);
-    m3 C = load9(st, stride, 6, slot);
-    m3 F = load9(st, stride, 15, slot);
-    PTrace t;
-    particle_stress(S, C, F, t);
-    Stencil sw;
-    stencil(S, x, sw);
-    v3 gv = mk(0.f, 0.f, 0.f);
-    m3 gaff = mzero();
-    float gfx[3] = {0.f, 0.f, 0.f};
-    v3 mv = S.p_mass * v;
+    int n_items = ta.counters[CNT_ITEMS];
+    int stride = ta.stride;
+    for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
+        int4 it = ta.items[e];
+        const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
+        int bi, bj, bk;
+        block_origin(S, it.x, bi, bj, bk);
+        load_tile(tg, S, bi, bj, bk, g_adjpm);
+        __syncthreads();
+        for (int k = threadIdx.x; k < it.z; k += WG) {
+            int p = ta.order[it.y + k];
+            v3 x = load3(st, stride, 0, p);
+            v3 v = load3(st, stride, 3, p);
+            m3 C = load9(st, stride, 6, p);
+            m3 F = load9(st, stride, 15, p);
+            PTrace t;
+            particle_stress(S, C, F, t);
+            Stencil sw;
+            stencil(S, x, sw);
+            int l0 = sw.base[0] - 4 * bi, l1 = sw.base[1] - 4 * bj, l2 = sw.base[2] - 4 * bk;
+            bool in_tile = (unsigned)l0 < 4u && (unsigned)l1 < 4u && (unsigned)l2 < 4u;
+            v3 gv = mk(0.f, 0.f, 0.f);
+            m3 gaff = mzero();
+            float gfx[3] = {0.f, 0.f, 0.f};
+            v3 mv = S.p_mass * v;
+            const float* A = t.affine.a;
+            if (in_tile) {
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+                for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++)
+                    for (int b = 0; b < 3; b++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                int I0 = sw.base[0] + a, I1 = sw.base[1] + b, I2 = sw.base[2] + c;
-                if (!in_grid(S, I0, I1, I2)) continue;
-                float4 gg = g_adjpm[node_id(S, I0, I1, I2)];
-                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
-                float dp[3] = {((float)a - sw.fx[0]) * S.dx, ((float)b - sw.fx[1]) * S.dx, ((float)c - sw.fx[2]) * S.dx};
-                const float* A = t.affine.a;
-                v3 contrib = mk(mv.x + A[0] * dp[0] + A[1] * dp[1] + A[2] * dp[2], mv.y + A[3] * dp[0] + A[4] * dp[1] + A[5] * dp[2],
-                                mv.z + A[6] * dp[0] + A[7] * dp[1] + A[8] * dp[2]);
-                v3 g = mk(gg.x, gg.y, gg.z);
-                float gw = dot(g, contrib) + gg.w * S.p_mass;
-                gv += (wt * S.p_mass) * g;
-                v3 wg = wt * g;
+                        for (int c = 0; c < 3; c++) {
+                            float4 gg = tg[((l0 + a) * 6 + (l1 + b)) * 6 + (l2 + c)];
+                            float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
+                            float dp[3] = {((float)a - sw.fx[0]) * S.dx, ((float)b - sw.fx[1]) * S.dx, ((float)c - sw.fx[2]) * S.dx};
+                            v3 contrib = mk(mv.x + A[0] * dp[0] + A[1] * dp[1] + A[2] * dp[2], mv.y + A[3] * dp[0] + A[4] * dp[1] + A[5] * dp[2],
+                                            mv.z + A[6] * dp[0] + A[7] * dp[1] + A[8] * dp[2]);
+                            v3 g = mk(gg.x, gg.y, gg.z);
+                            float gw = dot(g, contrib) + gg.w * S.p_mass;
+                            gv += (wt * S.p_mass) * g;
+                            v3 wg = wt * g;
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    gaff.a[j] += wg.x * dp[j];
-                    gaff.a[3 + j] += wg.y * dp[j];
-                    gaff.a[6 + j] += wg.z * dp[j];
-                }
-                v3 gdpos = mk(wg.x * A[0] + wg.y * A[3] + wg.z * A[6], wg.x * A[1] + wg.y * A[4] + wg.z * A[7],
-                              wg.x * A[2] + wg.y * A[5] + wg.z * A[8]);
-                int off[3] = {a, b, c};
+                            for (int j = 0; j < 3; j++) {
+                                gaff.a[j] += wg.x * dp[j];
+                                gaff.a[3 + j] += wg.y * dp[j];
+                                gaff.a[6 + j] += wg.z * dp[j];
+                            }
+                            v3 gdpos = mk(wg.x * A[0] + wg.y * A[3] + wg.z * A[6], wg.x * A[1] + wg.y * A[4] + wg.z * A[7],
+                                          wg.x * A[2] + wg.y * A[5] + wg.z * A[8]);
+                            int off[3] = {a, b, c};
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    float dwd = sw.dw[off[d]][d];
+                            for (int d = 0; d < 3; d++) {
+                                float dwd = sw.dw[off[d]][d];
 #pragma unroll
-                    for (int e2 = 0; e2 < 3; e2++)
-                        if (e2 != d) dwd *= sw.w[off[e2]][e2];
-                    gfx[d] += gw * dwd - comp(gdpos, d) * S.dx;
-                }
+                                for (int e2 = 0; e2 < 3; e2++)
+                                    if (e2 != d) dwd *= sw.w[off[e2]][e2];
+                                gfx[d] += gw * dwd - comp(gdpos, d) * S.dx;
+                            }
+                        }
             }
-    m3 gC, gst;
+            m3 gC, gst;
 #pragma unroll
-    for (int k = 0; k < 9; k++) { gC.a[k] = S.p_mass * gaff.a[k]; gst.a[k] = S.stress_k * gaff.a[k]; }
-    m3 FmR;
+            for (int kk = 0; kk < 9; kk++) { gC.a[kk] = S.p_mass * gaff.a[kk]; gst.a[kk] = S.stress_k * gaff.a[kk]; }
+            m3 FmR;
 #pragma unroll
-    for (int k = 0; k < 9; k++) FmR.a[k] = t.Fn.a[k] - t.R.a[k];
-    m3 sF = mmul(gst, t.Fn), sT = mmul_tn(gst, FmR);
-    m3 gF1 = load9(a1, stride, 15, slot);
-    m3 gF, gR;
+            for (int kk = 0; kk < 9; kk++) FmR.a[kk] = t.Fn.a[kk] - t.R.a[kk];
+            m3 sF = mmul(gst, t.Fn), sT = mmul_tn(gst, FmR);
+            m3 gF1 = load9(a1, stride, 15, p);
+            m3 gF, gR;
 #pragma unroll
-    for (int k = 0; k < 9; k++) { gF.a[k] = 2.f * S.mu * (sF.a[k] + sT.a[k]) + gF1.a[k]; gR.a[k] = -2.f * S.mu * sF.a[k]; }
-    float gJ = S.lam * (2.f * t.J - 1.f) * (gst.a[0] + gst.a[4] + gst.a[8]);
-    m3 cf = mcof(t.Fn);
+            for (int kk = 0; kk < 9; kk++) { gF.a[kk] = 2.f * S.mu * (sF.a[kk] + sT.a[kk]) + gF1.a[kk]; gR.a[kk] = -2.f * S.mu * sF.a[kk]; }
+            float gJ = S.lam * (2.f * t.J - 1.f) * (gst.a[0] + gst.a[4] + gst.a[8]);
+            m3 cf = mcof(t.Fn);
 #pragma unroll
-    for (int k = 0; k < 9; k++) gF.a[k] += gJ * cf.a[k];
-    m3 gU = mmul(gR, t.V), gV = mmul_tn(gR, t.U), gFt = mzero();
-    float gS[3] = {0.f, 0.f, 0.f};
-    if (t.yield) {
-        m3 D = mzero();
+            for (int kk = 0; kk < 9; kk++) gF.a[kk] += gJ * cf.a[kk];
+            m3 gU = mmul(gR, t.V), gV = mmul_tn(gR, t.U), gFt = mzero();
+            float gS[3] = {0.f, 0.f, 0.f};
+            if (t.yield) {
+                m3 D = mzero();
 #pragma unroll
-        for (int k = 0; k < 3; k++) D.a[k * 4] = t.e[k];
-        m3 a = mmul(mmul(gF, t.V), D), b = mmul(mmul_tn(gF, t.U), D);
+                for (int kk = 0; kk < 3; kk++) D.a[kk * 4] = t.e[kk];
+                m3 aa = mmul(mmul(gF, t.V), D), bb = mmul(mmul_tn(gF, t.U), D);
 #pragma unroll
-        for (int k = 0; k < 9; k++) { gU.a[k] += a.a[k]; gV.a[k] += b.a[k]; }
-        m3 M = mmul(mmul_tn(t.U, gF), t.V);
-        float gep[3], geps[3], geh[3];
-        float r = t.dg / t.ehn, gr = 0.f;
+                for (int kk = 0; kk < 9; kk++) { gU.a[kk] += aa.a[kk]; gV.a[kk] += bb.a[kk]; }
+                m3 M = mmul(mmul_tn(t.U, gF), t.V);
+                float gep[3], geps[3], geh[3];
+                float r = t.dg / t.ehn, gr = 0.f;
 #pragma unroll
-        for (int k = 0; k < 3; k++) { gep[k] = M.a[k * 4] * t.e[k]; geps[k] = gep[k]; geh[k] = -r * gep[k]; gr -= gep[k] * t.eh[k]; }
-        float gdg = gr / t.ehn;
-        float gehn = -gr * t.dg / (t.ehn * t.ehn) + gdg;
+                for (int kk = 0; kk < 3; kk++) { gep[kk] = M.a[kk * 4] * t.e[kk]; geps[kk] = gep[kk]; geh[kk] = -r * gep[kk]; gr -= gep[kk] * t.eh[kk]; }
+                float gdg = gr / t.ehn;
+                float gehn = -gr * t.dg / (t.ehn * t.ehn) + gdg;
 #pragma unroll
-        for (int k = 0; k < 3; k++) geh[k] += gehn * t.eh[k] / t.ehn;
-        float mg = (geh[0] + geh[1] + geh[2]) / 3.f;
+                for (int kk = 0; kk < 3; kk++) geh[kk] += gehn * t.eh[kk] / t.ehn;
+                float mg = (geh[0] + geh[1] + geh[2]) / 3.f;
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            geps[k] += geh[k] - mg;
-            if (0.05f < t.sig[k]) gS[k] += geps[k] / t.sc[k];
+                for (int kk = 0; kk < 3; kk++) {
+                    geps[kk] += geh[kk] - mg;
+                    if (0.05f < t.sig[kk]) gS[kk] += geps[kk] / t.sc[kk];
+                }
+            } else {
+                gFt = gF;
+            }
+            m3 bs = backward_svd(gU, gS, gV, t.U, t.sig, t.V);
+#pragma unroll
+            for (int kk = 0; kk < 9; kk++) gFt.a[kk] += bs.a[kk];
+            m3 Am;
+#pragma unroll
+            for (int kk = 0; kk < 9; kk++) Am.a[kk] = ((kk % 4 == 0) ? 1.f : 0.f) + S.dt * C.a[kk];
+            m3 gCa = mmul_nt(gFt, F), gF0 = mmul_tn(Am, gFt);
+#pragma unroll
+            for (int kk = 0; kk < 9; kk++) gC.a[kk] += S.dt * gCa.a[kk];
+            v3 gx0 = load3(a0, stride, 0, p) + S.inv_dx * mk(gfx[0], gfx[1], gfx[2]);
+            store3(a0, stride, 0, p, gx0);
+            store3(a0, stride, 3, p, gv);
+            store9(a0, stride, 6, p, gC);
+            store9(a0, stride, 15, p, gF0);
         }
-    } else {
-        gFt = gF;
+        __syncthreads();
     }
-    m3 bs = backward_svd(gU, gS, gV, t.U, t.sig, t.V);
-#pragma unroll
-    for (int k = 0; k < 9; k++) gFt.a[k] += bs.a[k];
-    m3 Am;
-#pragma unroll
-    for (int k = 0; k < 9; k++) Am.a[k] = ((k % 4 == 0) ? 1.f : 0.f) + S.dt * C.a[k];
-    m3 gCa = mmul_nt(gFt, F), gF0 = mmul_tn(Am, gFt);
-#pragma unroll
-    for (int k = 0; k < 9; k++) gC.a[k] += S.dt * gCa.a[k];
-    v3 gx0 = load3(a0, stride, 0, slot) + S.inv_dx * mk(gfx[0], gfx[1], gfx[2]);
-    store3(a0, stride, 0, slot, gx0);
-    store3(a0, stride, 3, slot, gv);
-    store9(a0, stride, 6, slot, gC);
-    store9(a0, stride, 15, slot, gF0);
 }
 
 // compute_grid_m_kernel (mpm_simulator.py:456-466): dense per-scene output
@@ -809,8 +1093,14 @@ struct mpm_ctx {
     float* ckpt = nullptr;
     float* ringbuf = nullptr;
     float* adj[2] = {nullptr, nullptr};
-    float4 *g_pm = nullptr, *g_v = nullptr, *g_adjv = nullptr, *g_adjpm = nullptr;
-    int *blk_flag = nullptr, *active = nullptr, *counters = nullptr;
+    float4 *g_pm = nullptr, *g_v = nullptr, *g_adjpm = nullptr;
+    int *blk_flag = nullptr, *node_flag = nullptr, *pblocks = nullptr, *nblocks = nullptr, *counters = nullptr;
+    int *cell_count = nullptr, *cell_start = nullptr, *bin_q = nullptr, *bin_rank = nullptr, *order = nullptr;
+    int *blk_stamp = nullptr, *blk_item0 = nullptr, *blk_nitems = nullptr;
+    int4* items = nullptr;
+    float4 *tiles_pm = nullptr, *tiles_v = nullptr;
+    int max_items = 0;
+    int tile_ctas = 1024; // persistent work-item workgroups
     float *pstate = nullptr, *pstate_g = nullptr, *pvel = nullptr, *pvel_g = nullptr;
     int valid_hi = -1;    // frames > valid_hi are stale
     int seg_valid = -1;   // segment whose ring frames are resident
@@ -833,9 +1123,23 @@ struct mpm_ctx {
     }
     float* adj_ptr(int f) const { return adj[f & 1]; }
     ChunkArgs cargs() const { ChunkArgs a; a.scenes = d_scenes; a.chunks = d_chunks; a.n_chunks = n_chunks; a.stride = stride; return a; }
+    BinArgs bargs() const {
+        BinArgs a;
+        a.scenes = d_scenes; a.chunks = d_chunks; a.blk_scene = d_blk_scene; a.stride = stride;
+        a.cell_count = cell_count; a.cell_start = cell_start; a.bin_q = bin_q; a.bin_rank = bin_rank;
+        a.blk_flag = blk_flag; a.node_flag = node_flag; a.counters = counters; a.pblocks = pblocks;
+        a.blk_stamp = blk_stamp; a.blk_item0 = blk_item0; a.blk_nitems = blk_nitems; a.items = items; a.order = order;
+        return a;
+    }
+    TileArgs targs() const {
+        TileArgs t;
+        t.scenes = d_scenes; t.blk_scene = d_blk_scene; t.items = items; t.counters = counters; t.order = order; t.stride = stride;
+        return t;
+    }
     GridArgs gargs(int f) const {
         GridArgs g;
-        g.scenes = d_scenes; g.prims = d_prims; g.blk_scene = d_blk_scene; g.active = active; g.counters = counters;
+        g.scenes = d_scenes; g.prims = d_prims; g.blk_scene = d_blk_scene; g.nblocks = nblocks; g.counters = counters;
+        g.blk_stamp = blk_stamp; g.blk_item0 = blk_item0; g.blk_nitems = blk_nitems;
         g.pstate = pstate; g.n_prims_total = n_prims; g.f = f;
         return g;
     }
@@ -861,9 +1165,9 @@ struct PhaseTimer {
 };
 
 struct Layout {
-    size_t off_scenes, off_prims, off_chunks, off_blk_scene, off_ckpt, off_ring, off_adj, off_grid, off_flags, off_active,
-        off_counters, off_prim_bufs, total;
-    int stride, n_chunks, n_blocks, n_ckpt, ring, K;
+    size_t off_scenes, off_prims, off_chunks, off_blk_scene, off_ckpt, off_ring, off_adj, off_grid, off_blkints, off_cells,
+        off_pints, off_items, off_tiles, off_counters, off_prim_bufs, total;
+    int stride, n_chunks, n_blocks, n_ckpt, ring, K, max_items;
     long long n_nodes;
     int n_prims;
 };
@@ -903,9 +1207,14 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k) 
     L.off_ckpt = o; o = align256(o + ff * L.n_ckpt);
     L.off_ring = o; o = align256(o + ff * L.ring);
     L.off_adj = o; o = align256(o + ff * 2);
-    L.off_grid = o; o = align256(o + sizeof(float4) * (size_t)L.n_nodes * 4);
-    L.off_flags = o; o = align256(o + sizeof(int) * L.n_blocks);
-    L.off_active = o; o = align256(o + sizeof(int) * L.n_blocks);
+    L.off_grid = o; o = align256(o + sizeof(float4) * (size_t)L.n_nodes * 3);
+    L.off_blkints = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 7);   // flags, node flags, 2 lists, stamp, item0, nitems
+    L.off_cells = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 64 * 2); // cell counts, cell starts
+    L.off_pints = o; o = align256(o + sizeof(int) * (size_t)stride * 3);         // bin key, bin rank, order
+    // work items: at most one partial item per occupied cell block plus stride / WI full ones
+    L.max_items = stride / WI + std::min(L.n_blocks, stride) + 1;
+    L.off_items = o; o = align256(o + sizeof(int4) * (size_t)L.max_items);
+    L.off_tiles = o; o = align256(o + sizeof(float4) * (size_t)L.max_items * TILE * 2);
     L.off_counters = o; o = align256(o + sizeof(int) * 64);
     L.off_prim_bufs = o; o = align256(o + sizeof(float) * 8 * (size_t)max_frames * std::max(1, n_prims) * 4);
     L.total = o;
@@ -922,23 +1231,37 @@ int launch_fill(float* p, size_t n, hipStream_t st) {
     return 0;
 }
 
+// counting-sort frame f's particles into cell blocks and work items
+int run_bin(mpm_ctx* c, int f, int do_fk, hipStream_t st) {
+    const float* in = c->frame_ptr(f);
+    BinArgs ba = c->bargs();
+    int cblocks = std::min(4096, (c->n_blocks + 255) / 256);
+    { PhaseTimer t(c, MPM_PH_BIN, st);
+      hipLaunchKernelGGL(k_bin, dim3(c->n_chunks), dim3(PPB), 0, st, ba, in, do_fk, c->d_prims, c->n_prims, c->pstate, c->pvel, f); }
+    { PhaseTimer t(c, MPM_PH_COMPACT, st);
+      hipLaunchKernelGGL(k_compact, dim3(cblocks), dim3(256), 0, st, c->n_blocks, c->blk_flag, c->pblocks, c->counters + CNT_PBLK); }
+    { PhaseTimer t(c, MPM_PH_BLOCKS, st);
+      hipLaunchKernelGGL(k_blocks, dim3(c->grid_ctas), dim3(256), 0, st, ba); }
+    { PhaseTimer t(c, MPM_PH_COMPACT, st);
+      hipLaunchKernelGGL(k_compact, dim3(cblocks), dim3(256), 0, st, c->n_blocks, c->node_flag, c->nblocks, c->counters + CNT_NBLK); }
+    { PhaseTimer t(c, MPM_PH_SCATTER, st);
+      hipLaunchKernelGGL(k_scatter, dim3(c->n_chunks), dim3(PPB), 0, st, ba); }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // forward substep f -> f+1 (writes frame f+1 wherever its slot is)
 int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
     const float* in = c->frame_ptr(f);
     float* out = c->frame_ptr(f + 1);
-    ChunkArgs ca = c->cargs();
-    { PhaseTimer t(c, MPM_PH_MARK, st);
-      hipLaunchKernelGGL(k_mark, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, c->blk_flag, c->counters, 1, c->d_prims, c->n_prims,
-                         c->pstate, c->pvel, f); }
-    { PhaseTimer t(c, MPM_PH_COMPACT, st);
-      hipLaunchKernelGGL(k_compact, dim3(std::min(4096, (c->n_blocks + 255) / 256)), dim3(256), 0, st, c->n_blocks, c->blk_flag,
-                         c->active, c->counters); }
+    if (run_bin(c, f, 1, st)) return -1;
+    TileArgs ta = c->targs();
     { PhaseTimer t(c, MPM_PH_P2G, st);
-      hipLaunchKernelGGL(k_p2g, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, out, c->g_pm); }
+      hipLaunchKernelGGL(k_p2g_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, out, c->tiles_pm); }
     { PhaseTimer t(c, MPM_PH_GRID, st);
-      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->g_v, 1); }
+      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->tiles_pm, c->g_pm, c->g_v); }
     { PhaseTimer t(c, MPM_PH_G2P, st);
-      hipLaunchKernelGGL(k_g2p, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, out, c->g_v); }
+      hipLaunchKernelGGL(k_g2p_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, out, c->g_v); }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -966,24 +1289,19 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     const float* in1 = c->frame_ptr(f + 1);
     const float* a1 = c->adj_ptr(f + 1);
     float* a0 = c->adj_ptr(f);
-    ChunkArgs ca = c->cargs();
-    { PhaseTimer t(c, MPM_PH_MARK, st);
-      hipLaunchKernelGGL(k_mark, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, c->blk_flag, c->counters, 0, c->d_prims, c->n_prims,
-                         c->pstate, c->pvel, f); }
-    { PhaseTimer t(c, MPM_PH_COMPACT, st);
-      hipLaunchKernelGGL(k_compact, dim3(std::min(4096, (c->n_blocks + 255) / 256)), dim3(256), 0, st, c->n_blocks, c->blk_flag,
-                         c->active, c->counters); }
+    if (run_bin(c, f, 0, st)) return -1;
+    TileArgs ta = c->targs();
     { PhaseTimer t(c, MPM_PH_P2G_GRID, st);
-      hipLaunchKernelGGL(k_p2g, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, (float*)nullptr, c->g_pm); }
+      hipLaunchKernelGGL(k_p2g_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (float*)nullptr, c->tiles_pm); }
     { PhaseTimer t(c, MPM_PH_GRID_KEEP, st);
-      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->g_v, 0); }
+      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->tiles_pm, c->g_pm, c->g_v); }
     { PhaseTimer t(c, MPM_PH_G2P_BWD, st);
-      hipLaunchKernelGGL(k_g2p_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, in1, a1, a0, c->g_v, c->g_adjv); }
+      hipLaunchKernelGGL(k_g2p_bwd_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, in1, a1, a0, c->g_v, c->tiles_v); }
     { PhaseTimer t(c, MPM_PH_GRID_BWD, st);
-      hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->g_adjv, c->g_adjpm, c->pstate_g); }
+      hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->tiles_v, c->g_adjpm, c->pstate_g); }
     { PhaseTimer t(c, MPM_PH_P2G_BWD, st);
-      hipLaunchKernelGGL(k_p2g_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, ca, in, a1, a0, c->g_adjpm, c->d_prims, c->n_prims, c->pstate,
-                         c->pvel, c->pstate_g, c->pvel_g, f); }
+      hipLaunchKernelGGL(k_p2g_bwd_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, a1, a0, c->g_adjpm, c->d_prims, c->n_prims,
+                         c->pstate, c->pvel, c->pstate_g, c->pvel_g, f); }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1065,10 +1383,24 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     float4* g = (float4*)(w + L.off_grid);
     c->g_pm = g;
     c->g_v = g + L.n_nodes;
-    c->g_adjv = g + 2 * L.n_nodes;
-    c->g_adjpm = g + 3 * L.n_nodes;
-    c->blk_flag = (int*)(w + L.off_flags);
-    c->active = (int*)(w + L.off_active);
+    c->g_adjpm = g + 2 * L.n_nodes;
+    int* bi = (int*)(w + L.off_blkints);
+    c->blk_flag = bi;
+    c->node_flag = bi + (size_t)L.n_blocks;
+    c->pblocks = bi + (size_t)L.n_blocks * 2;
+    c->nblocks = bi + (size_t)L.n_blocks * 3;
+    c->blk_stamp = bi + (size_t)L.n_blocks * 4;
+    c->blk_item0 = bi + (size_t)L.n_blocks * 5;
+    c->blk_nitems = bi + (size_t)L.n_blocks * 6;
+    c->cell_count = (int*)(w + L.off_cells);
+    c->cell_start = c->cell_count + (size_t)L.n_blocks * 64;
+    c->bin_q = (int*)(w + L.off_pints);
+    c->bin_rank = c->bin_q + L.stride;
+    c->order = c->bin_q + 2 * (size_t)L.stride;
+    c->items = (int4*)(w + L.off_items);
+    c->max_items = L.max_items;
+    c->tiles_pm = (float4*)(w + L.off_tiles);
+    c->tiles_v = c->tiles_pm + (size_t)L.max_items * TILE;
     c->counters = (int*)(w + L.off_counters);
     size_t pb = (size_t)8 * max_frames * std::max(1, L.n_prims);
     c->pstate = (float*)(w + L.off_prim_bufs);
@@ -1146,6 +1478,7 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
     }
     c->grid_ctas = std::max(64, ncu * 8);
+    c->tile_ctas = std::max(64, ncu * 4);
     *out = c;
     return 0;
 }

@@ -7,6 +7,7 @@ torch's current stream.
 """
 import ctypes
 
+import numpy as np
 import torch
 
 from .. import _lib
@@ -14,6 +15,20 @@ from .. import _lib
 
 def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def cell_sort_perm(x, n_grid):
+    """Permutation that orders particles by (4x4x4 cell block, cell), the order
+    the engine bins them in each substep (cell index base = int(x / dx - 0.5),
+    truncating, like the reference's stencil base).  Kernels read particles
+    through per-substep index lists; storing them in this order keeps those
+    reads local."""
+    x = np.asarray(x, np.float64)
+    b = np.clip(np.trunc(x * n_grid - 0.5).astype(np.int64), 0, n_grid - 1)
+    nb = (n_grid + 3) // 4
+    blk = ((b[:, 0] >> 2) * nb + (b[:, 1] >> 2)) * nb + (b[:, 2] >> 2)
+    cell = ((b[:, 0] & 3) << 4) | ((b[:, 1] & 3) << 2) | (b[:, 2] & 3)
+    return np.argsort(blk * 64 + cell, kind="stable")
 
 
 def scene_desc(c, n_particles, n_prims):

@@ -11,7 +11,7 @@ import numpy as np
 import torch
 
 from ..config import derive
-from .core import Engine, scene_desc
+from .core import Engine, cell_sort_perm, scene_desc
 
 
 class _Scalar:
@@ -93,6 +93,9 @@ class MPMSimulator:
             ckpt_k = int(cfg.get("ckpt_k", 0) or 0) if hasattr(cfg, "get") else 0
         self.ckpt_k = ckpt_k
         self.grid_m = _GridField(self)
+        # particles are stored in cell order: slot i holds the caller's particle perm[i]
+        self._perm = None
+        self._inv = None
         self.engine = None
         self._ctx = None
         if hasattr(primitives, "_sim"):
@@ -136,6 +139,21 @@ class MPMSimulator:
     def _cols(self):
         return self.engine.scene_cols(0)
 
+    def _set_perm(self, x):
+        perm = cell_sort_perm(x, self.n_grid)
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(len(perm))
+        self._perm = torch.as_tensor(perm, device="cuda")
+        self._inv = torch.as_tensor(inv, device="cuda")
+        return perm
+
+    def _to_caller(self, t):
+        """rows in slot order -> rows in the caller's particle order"""
+        return t if self._inv is None else t.index_select(0, self._inv)
+
+    def _to_slots(self, t):
+        return t if self._perm is None else t.index_select(0, self._perm)
+
     # ---------------------------------------------------------------- setup
     def initialize(self, n_particles):
         """mpm_simulator.py:82-97 (material constants are per scene)."""
@@ -144,12 +162,13 @@ class MPMSimulator:
 
     def reset(self, x):
         """reset_kernel: frame 0 = positions, v = 0, F = I, C = 0 (mpm_simulator.py:402-413)."""
-        x = torch.as_tensor(np.asarray(x) if not isinstance(x, torch.Tensor) else x, dtype=torch.float32)
+        x = np.asarray(x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else x, np.float64)
         n = x.shape[0]
         self.n_particles[None] = n
         self._ensure_ctx(n)
+        perm = self._set_perm(x)
         soa = torch.zeros((3, self.engine.stride), dtype=torch.float32, device="cuda")
-        soa[:, self._cols()] = x.to("cuda").t()
+        soa[:, self._cols()] = torch.as_tensor(x[perm], dtype=torch.float32, device="cuda").t()
         self.engine.reset_frame0(soa)
         self.cur = 0
 
@@ -173,12 +192,12 @@ class MPMSimulator:
 
     # ---------------------------------------------------------------- state io
     def _frame_tensors(self, f):
-        fr = self.engine.frame(f)[:, self._cols()]
-        n = fr.shape[1]
-        x = fr[0:3].t()
-        v = fr[3:6].t()
-        C = fr[6:15].t().reshape(n, 3, 3)
-        F = fr[15:24].t().reshape(n, 3, 3)
+        fr = self._to_caller(self.engine.frame(f)[:, self._cols()].t())
+        n = fr.shape[0]
+        x = fr[:, 0:3]
+        v = fr[:, 3:6]
+        C = fr[:, 6:15].reshape(n, 3, 3)
+        F = fr[:, 15:24].reshape(n, 3, 3)
         return x, v, F, C
 
     def get_state(self, f):
@@ -195,7 +214,8 @@ class MPMSimulator:
         self._ensure_ctx(n)
         soa = torch.zeros((24, self.engine.stride), dtype=torch.float32, device="cuda")
         cols = self._cols()
-        x, v, F, C = [torch.as_tensor(np.asarray(a, np.float32)).to("cuda") for a in state[:4]]
+        perm = self._set_perm(np.asarray(state[0], np.float64).reshape(n, 3))
+        x, v, F, C = [torch.as_tensor(np.asarray(a, np.float32)[perm]).to("cuda") for a in state[:4]]
         soa[0:3, cols] = x.reshape(n, 3).t()
         soa[3:6, cols] = v.reshape(n, 3).t()
         soa[6:15, cols] = C.reshape(n, 9).t()
@@ -220,7 +240,7 @@ class MPMSimulator:
     def get_torch_xv(self, f):
         """(N, 6) float32 device tensor [x | v] of frame f (GradModel._get_obs layout)."""
         fr = self.engine.frame(f)[:, self._cols()]
-        return fr[0:6].t().contiguous()
+        return self._to_caller(fr[0:6].t()).contiguous()
 
     # ---------------------------------------------------------------- mass grid
     def compute_grid_m_kernel(self, f):
@@ -251,7 +271,7 @@ class MPMSimulator:
         xs = None
         if xv_grad is not None:
             xs = torch.zeros((self.engine.stride, 6), dtype=torch.float32, device="cuda")
-            xs[self._cols()] = xv_grad.detach().to("cuda", torch.float32).reshape(-1, 6)
+            xs[self._cols()] = self._to_slots(xv_grad.detach().to("cuda", torch.float32).reshape(-1, 6))
         pg = None
         if prim_grad is not None and self.n_primitive > 0:
             pg = prim_grad.detach().to("cuda", torch.float32).reshape(self.n_primitive, 8).contiguous()
@@ -259,9 +279,9 @@ class MPMSimulator:
 
     def get_adjoint(self, f):
         """(x, v, F, C) adjoints of frame f as (N,3), (N,3), (N,3,3), (N,3,3) tensors."""
-        a = self.engine.adjoint(f)[:, self._cols()]
-        n = a.shape[1]
-        return a[0:3].t(), a[3:6].t(), a[15:24].t().reshape(n, 3, 3), a[6:15].t().reshape(n, 3, 3)
+        a = self._to_caller(self.engine.adjoint(f)[:, self._cols()].t())
+        n = a.shape[0]
+        return a[:, 0:3], a[:, 3:6], a[:, 15:24].reshape(n, 3, 3), a[:, 6:15].reshape(n, 3, 3)
 
     def scale_adjoint(self, f, alpha):
         """decay_kernel (function.py:66-77): scale the adjoints of frame f."""

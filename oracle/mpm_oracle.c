@@ -1455,3 +1455,7 @@ void orc_particle_stress(const orc_sim *S, int np, const real *st, real *Fn, rea
         memcpy(stress + 9 * p, t.stress, sizeof(t.stress));
     }
 }
+/* backward_svd of one matrix (unit tests against autograd of a library SVD) */
+void orc_backward_svd(const real *gu, const real *gs, const real *gv, const real *u, const real *sig, const real *v, real *out) {
+    backward_svd(gu, gs, gv, u, sig, v, out);
+}

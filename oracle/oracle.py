@@ -339,6 +339,12 @@ class Oracle:
         self.L.orc_svd3(_ptr(A), _ptr(U), _ptr(S), _ptr(V))
         return U, S, V
 
+    def backward_svd(self, gu, gs, gv, u, s, v):
+        args = [self.arr(a, (3, 3)) for a in (gu, gs, gv, u, s, v)]
+        out = np.zeros((3, 3), self.dtype)
+        self.L.orc_backward_svd(*[_ptr(a) for a in args], _ptr(out))
+        return out
+
     def particle_stress(self, st):
         st = self.arr(st)
         Fn = np.zeros((st.shape[0], 9), self.dtype)
