@@ -35,6 +35,24 @@ ALGO_BYTES = {
 }
 
 
+# engine phase -> kernel name in the rocprof summaries
+PHASE_KERNEL = {"p2g": "k_p2g_tile", "p2g_grid": "k_p2g_tile", "g2p": "k_g2p_tile", "g2p_bwd": "k_g2p_bwd_tile",
+                "p2g_bwd": "k_p2g_bwd_tile", "bin": "k_bin", "grid_op": "k_grid_op", "grid_op_keep": "k_grid_op",
+                "grid_op_bwd": "k_grid_op_bwd"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batched64_config4.json")
+
+
+def measured_traffic(phase):
+    """HBM bytes per launch of the phase's kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)[PHASE_KERNEL[phase]]
+        return int(t["read_bytes"] + t["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -110,6 +128,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--check-scenes", type=int, default=2, help="scenes (smallest first) checked against the fp64 oracle")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
@@ -122,6 +141,13 @@ def main():
     from diffskill_amd.batch import BatchedTrajOpt, PlainTrajOpt, SceneBatch
 
     torch.cuda.init()
+    # calibration kernel of known traffic for the HBM byte counters: reads and
+    # writes 256 MiB (elementwise multiply), three launches
+    cal = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    for _ in range(3):
+        cal = cal.mul(1.0)
+    torch.cuda.synchronize()
+    del cal
     w = workloads.config4(n_scenes=args.scenes)
     H, S = w["H"], w["substeps"]
     T = H * S
@@ -133,21 +159,26 @@ def main():
     n_total = batch.total_particles
 
     # ---- gradient accuracy of the batched path vs the fp64 oracle (smallest scene)
-    grad_err = None
-    if not args.no_oracle:
+    grad_err, grad_err_l2, grad_checks = None, None, []
+    if not args.no_oracle and args.check_scenes > 0:
         opt._device_pass()
         torch.cuda.synchronize()
-        k = int(np.argmin(w["sizes"]))
-        o0 = int(batch.prim_offsets[k])
-        npk = len(w["scenes"][k]["prims"])
-        table = torch.clamp(init, -1, 1).numpy()[:, o0:o0 + npk]
-        g_gpu = opt.action_grads().detach().cpu().double().numpy()[:, o0:o0 + npk]
-        loss_gpu = float(opt.losses()[k])
-        loss_ref, ga, dims = oracle_scene_grads(w, k, table, targets[k].cpu().numpy(), threads=min(16, os.cpu_count() or 1))
-        g_flat = np.concatenate([g_gpu[:, j, :d] for j, d in enumerate(dims)], axis=1)
-        grad_err = float(np.abs(g_flat - ga).max() / max(np.abs(ga).max(), 1e-30))
-        log("grad check scene %d (%d particles): loss %.6g vs oracle %.6g, grad rel err %.3g"
-            % (k, w["sizes"][k], loss_gpu, loss_ref, grad_err))
+        g_all = opt.action_grads().detach().cpu().double().numpy()
+        losses = opt.losses().detach().cpu().numpy()
+        for k in [int(i) for i in np.argsort(w["sizes"])[:args.check_scenes]]:
+            o0 = int(batch.prim_offsets[k])
+            npk = len(w["scenes"][k]["prims"])
+            table = torch.clamp(init, -1, 1).numpy()[:, o0:o0 + npk]
+            loss_ref, ga, dims = oracle_scene_grads(w, k, table, targets[k].cpu().numpy(), threads=min(16, os.cpu_count() or 1))
+            g_flat = np.concatenate([g_all[:, o0 + j, :d] for j, d in enumerate(dims)], axis=1)
+            e_max = float(np.abs(g_flat - ga).max() / max(np.abs(ga).max(), 1e-30))
+            e_l2 = float(np.linalg.norm(g_flat - ga) / max(np.linalg.norm(ga), 1e-30))
+            grad_checks.append(dict(scene=k, particles=int(w["sizes"][k]), tools=[p["shape"] for p in w["scenes"][k]["prims"]],
+                                    loss=float(losses[k]), loss_oracle=float(loss_ref), rel_err_max=e_max, rel_err_l2=e_l2))
+            log("grad check scene %d (%d particles): loss %.6g vs oracle %.6g, grad rel err max-element %.3g, l2 %.3g"
+                % (k, w["sizes"][k], losses[k], loss_ref, e_max, e_l2))
+        grad_err = max(c["rel_err_max"] for c in grad_checks)
+        grad_err_l2 = max(c["rel_err_l2"] for c in grad_checks)
 
     # ---- batched path: warmup (includes graph capture), then K timed iterations
     for _ in range(args.warmup):
@@ -178,7 +209,7 @@ def main():
     algo = ALGO_BYTES[dom] * n_total
     achieved = algo / avg_s / 1e9
     roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom),
                     algo_bytes_per_launch=int(algo), avg_launch_us=kernels[dom]["avg_us"])
     log("kernels:", json.dumps(kernels))
 
@@ -218,7 +249,8 @@ def main():
                                % (len(w["scenes"]), n_total, T),
                    "scenes": len(w["scenes"]), "particles": n_total, "substeps": T, "grid": 64,
                    "ckpt_k": args.ckpt_k or "store-all", "hip_graph": not args.no_graph},
-        "plain": plain, "grad_rel_err": grad_err, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
+        "plain": plain, "grad_rel_err": grad_err, "grad_rel_err_l2": grad_err_l2, "grad_checks": grad_checks,
+        "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
