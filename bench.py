@@ -26,19 +26,21 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per particle for each particle kernel: the particle-state
 # (and particle-adjoint) floats it must read and write once, fp32
 ALGO_BYTES = {
-    "p2g": 96 + 36,            # read x v C F, write F'
-    "g2p": 12 + 60,            # read x, write x' v' C'
-    "p2g_grid": 96,            # read x v C F
+    "stress": 72 + 72,               # read C F, write F' and the affine matrix
+    "p2g": 60,                       # read x v and the affine matrix
+    "g2p": 12 + 60,                  # read x, write x' v' C'
+    "p2g_grid": 60,                  # read x v and the affine matrix
     "g2p_bwd": 12 + 12 + 60 + 12,    # read x, x', adj x' v' C', write adj x
-    "p2g_bwd": 96 + 36 + 12 + 96,    # read x v C F, adj F', adj x; write adj x v C F
+    "p2g_bwd": 60 + 12 + 24 + 36,    # read x v affine, adj x; write adj x v and the affine adjoint
+    "stress_bwd": 72 + 36 + 36 + 72,  # read C F, affine adjoint, adj F'; write adj C F
     "bin": 12,
 }
 
 
 # engine phase -> kernel name in the rocprof summaries
-PHASE_KERNEL = {"p2g": "k_p2g_tile", "p2g_grid": "k_p2g_tile", "g2p": "k_g2p_tile", "g2p_bwd": "k_g2p_bwd_tile",
-                "p2g_bwd": "k_p2g_bwd_tile", "bin": "k_bin", "grid_op": "k_grid_op", "grid_op_keep": "k_grid_op",
-                "grid_op_bwd": "k_grid_op_bwd"}
+PHASE_KERNEL = {"stress": "k_stress", "p2g": "k_p2g_scatter", "p2g_grid": "k_p2g_scatter", "g2p": "k_g2p_tile",
+                "g2p_bwd": "k_g2p_bwd_tile", "p2g_bwd": "k_p2g_bwd_gather", "stress_bwd": "k_stress_bwd", "bin": "k_bin",
+                "grid_op": "k_grid_op", "grid_op_keep": "k_grid_op", "grid_op_bwd": "k_grid_op_bwd"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batched64_config4.json")
 
 

@@ -49,8 +49,8 @@ EXPORTS = [
     "mpm_update_prims", "mpm_copy_frame", "mpm_set_profiling", "mpm_profile_read", "mpm_last_error",
 ]
 
-PHASES = ["bin", "compact", "blocks", "scatter", "p2g", "grid_op", "g2p", "p2g_grid", "grid_op_keep", "g2p_bwd",
-          "grid_op_bwd", "p2g_bwd"]
+PHASES = ["bin", "compact", "blocks", "scatter", "stress", "p2g", "grid_op", "g2p", "p2g_grid", "grid_op_keep",
+          "g2p_bwd", "grid_op_bwd", "p2g_bwd", "stress_bwd"]
 
 _LIB = None
 

@@ -353,27 +353,55 @@ __global__ void __launch_bounds__(PPB) k_scatter(BinArgs a) {
 }
 
 // ------------------------------------------------------------------ wave transpose reduction
-// Sums 32 per-lane values over the 64 lanes in 6 butterfly steps (31 + 1
-// shuffles instead of 32 x 6): after it, lanes 2j and 2j+1 hold the total of
-// value j.
-template <int H, int MASK>
-DEV void tstep(float* r, int lane) {
+// Sums 32 per-lane values over the 64 lanes of a wave in 6 butterfly steps,
+// entirely in VALU (no LDS): bit 5 and bit 4 with gfx950's v_permlane32_swap /
+// v_permlane16_swap (one swap + one add per value), bits 3..0 with DPP
+// row_mirror, row_half_mirror and quad permutations.  At each step a lane keeps
+// the half of its values selected by its lane bit and adds the partner's copy
+// of that half; afterwards lanes 2j and 2j+1 hold the total of value j.
+DEV float f_of(unsigned u) { return __uint_as_float(u); }
+DEV unsigned u_of(float f) { return __float_as_uint(f); }
+template <int H>
+DEV void tstep_swap32(float* r) {
+#pragma unroll
+    for (int j = 0; j < H; j++) {
+        auto sw = __builtin_amdgcn_permlane32_swap(u_of(r[j]), u_of(r[j + H]), false, false);
+        r[j] = f_of(sw[0]) + f_of(sw[1]);
+    }
+}
+template <int H>
+DEV void tstep_swap16(float* r) {
+#pragma unroll
+    for (int j = 0; j < H; j++) {
+        auto sw = __builtin_amdgcn_permlane16_swap(u_of(r[j]), u_of(r[j + H]), false, false);
+        r[j] = f_of(sw[0]) + f_of(sw[1]);
+    }
+}
+template <int H, int MASK, int DPP>
+DEV void tstep_dpp(float* r, int lane) {
     bool up = (lane & MASK) != 0;
 #pragma unroll
     for (int j = 0; j < H; j++) {
         float lo = r[j], hi = r[j + H];
         float send = up ? lo : hi;
         float keep = up ? hi : lo;
-        r[j] = keep + __shfl_xor(send, MASK);
+        r[j] = keep + f_of(__builtin_amdgcn_mov_dpp(u_of(send), DPP, 0xf, 0xf, false));
     }
 }
+// The mirrors pair lane i with i ^ 15 (resp. i ^ 7): the partner has the opposite
+// split bit and the same higher bits, so it holds the same value set and the
+// lane sets summed at each step stay disjoint (their xor span doubles).
+constexpr int DPP_ROW_MIRROR = 0x140;       // lane i <- lane 15 - i within a row of 16
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;  // lane i <- lane 7 - i within a half row
+constexpr int DPP_QUAD_XOR2 = 0x4E;         // quad_perm [2,3,0,1]
+constexpr int DPP_QUAD_XOR1 = 0xB1;         // quad_perm [1,0,3,2]
 DEV float tsum32(float* r, int lane) {
-    tstep<16, 32>(r, lane);
-    tstep<8, 16>(r, lane);
-    tstep<4, 8>(r, lane);
-    tstep<2, 4>(r, lane);
-    tstep<1, 2>(r, lane);
-    return r[0] + __shfl_xor(r[0], 1);
+    tstep_swap32<16>(r);
+    tstep_swap16<8>(r);
+    tstep_dpp<4, 8, DPP_ROW_MIRROR>(r, lane);
+    tstep_dpp<2, 4, DPP_ROW_HALF_MIRROR>(r, lane);
+    tstep_dpp<1, 2, DPP_QUAD_XOR2>(r, lane);
+    return r[0] + f_of(__builtin_amdgcn_mov_dpp(u_of(r[0]), DPP_QUAD_XOR1, 0xf, 0xf, false));
 }
 DEV int tile_index(int l0, int l1, int l2, int n) { return ((l0 + n / 9) * 6 + (l1 + (n / 3) % 3)) * 6 + (l2 + n % 3); }
 
@@ -426,57 +454,6 @@ struct TileArgs {
     const int* order;
     int stride;
 };
-
-// p2g over work items: constitutive update + scatter into the block's LDS tile,
-// flushed to the item's tile buffer (mpm_simulator.py:198-225).
-// st_out != null also stores F of frame f+1.
-__global__ void __launch_bounds__(WG) k_p2g_tile(TileArgs ta, const float* __restrict__ st, float* __restrict__ st_out,
-                                                 float4* __restrict__ tiles) {
-    __shared__ float tile[4 * TILE];
-    int n_items = ta.counters[CNT_ITEMS];
-    int lane = threadIdx.x & 63;
-    int stride = ta.stride;
-    for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
-        int4 it = ta.items[e];
-        const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
-        int bi, bj, bk;
-        block_origin(S, it.x, bi, bj, bk);
-        for (int k = threadIdx.x; k < 4 * TILE; k += WG) tile[k] = 0.f;
-        __syncthreads();
-        for (int k0 = 0; k0 < it.z; k0 += WG) {
-            int k = k0 + threadIdx.x;
-            bool valid = k < it.z;
-            int p = valid ? ta.order[it.y + k] : ta.order[it.y];
-            v3 x = load3(st, stride, 0, p);
-            v3 v = load3(st, stride, 3, p);
-            m3 C = load9(st, stride, 6, p);
-            m3 F = load9(st, stride, 15, p);
-            PTrace t;
-            particle_stress(S, C, F, t);
-            if (valid && st_out) store9(st_out, stride, 15, p, t.Fn);
-            Stencil sw;
-            stencil(S, x, sw);
-            int l0 = sw.base[0] - 4 * bi, l1 = sw.base[1] - 4 * bj, l2 = sw.base[2] - 4 * bk;
-            bool in_tile = (unsigned)l0 < 4u && (unsigned)l1 < 4u && (unsigned)l2 < 4u;
-            int key = (l0 << 4) | (l1 << 2) | l2;
-            v3 mv = S.p_mass * v;
-            const float* A = t.affine.a;
-            float dx = S.dx, pm = S.p_mass;
-            auto val = [&](int n, int c) -> float {
-                int a = n / 9, b = (n / 3) % 3, cc = n % 3;
-                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[cc][2];
-                if (c == 3) return wt * pm;
-                float d0 = ((float)a - sw.fx[0]) * dx, d1 = ((float)b - sw.fx[1]) * dx, d2 = ((float)cc - sw.fx[2]) * dx;
-                return wt * (comp(mv, c) + A[c * 3] * d0 + A[c * 3 + 1] * d1 + A[c * 3 + 2] * d2);
-            };
-            scatter_tile<4>(tile, valid && in_tile, key, l0, l1, l2, lane, val);
-        }
-        __syncthreads();
-        for (int k = threadIdx.x; k < TILE; k += WG)
-            tiles[(size_t)e * TILE + k] = make_float4(tile[k], tile[TILE + k], tile[2 * TILE + k], tile[3 * TILE + k]);
-        __syncthreads();
-    }
-}
 
 struct GridArgs {
     const SceneC* scenes;
@@ -829,20 +806,79 @@ DEV m3 backward_svd(const m3& gu, const float gs[3], const m3& gv, const m3& u, 
     return out;
 }
 
-// reverse of p2g + svd_grad + compute_F_tmp.grad over work items
-// (mpm_simulator.py:121-156,198-225), gathering the node adjoints from the
-// block's LDS tile; workgroup 0 also runs the primitive FK adjoint of substep f.
-__global__ void __launch_bounds__(WG) k_p2g_bwd_tile(TileArgs ta, const float* __restrict__ st, const float* __restrict__ a1,
-                                                     float* __restrict__ a0, const float4* __restrict__ g_adjpm,
-                                                     const PrimC* __restrict__ prims, int n_prims_total,
-                                                     const float* __restrict__ pstate, const float* __restrict__ pvel,
-                                                     float* __restrict__ gpstate, float* __restrict__ gpvel, int f) {
-    __shared__ float4 tg[TILE];
-    if (blockIdx.x == 0 && (int)threadIdx.x < n_prims_total) {
-        int q = threadIdx.x;
-        size_t o0 = ((size_t)f * n_prims_total + q) * 8, o1 = ((size_t)(f + 1) * n_prims_total + q) * 8;
-        prim_fk_rev(prims[q], pstate + o0, pvel + o0, gpstate + o1, gpstate + o0, gpvel + o0);
+// constitutive update in particle-slot order (coalesced): F_tmp, register SVD,
+// return map, stress (mpm_simulator.py:121-129,165-182,205-216).  Writes the
+// affine matrix stress_k * stress + p_mass * C of frame f and, when st_out is
+// given, F of frame f+1.
+__global__ void __launch_bounds__(PPB) k_stress(ChunkArgs ca, const float* __restrict__ st, float* __restrict__ st_out,
+                                                float* __restrict__ aff) {
+    int4 ch = ca.chunks[blockIdx.x];
+    int i = threadIdx.x;
+    if (i >= ch.z) return;
+    const SceneC& S = ca.scenes[ch.x];
+    int slot = ch.y + i;
+    m3 C = load9(st, ca.stride, 6, slot);
+    m3 F = load9(st, ca.stride, 15, slot);
+    PTrace t;
+    particle_stress(S, C, F, t);
+    if (st_out) store9(st_out, ca.stride, 15, slot, t.Fn);
+    store9(aff, ca.stride, 0, slot, t.affine);
+}
+
+// p2g scatter over work items (mpm_simulator.py:217-225): momentum and mass of
+// every particle of the item into the block's LDS tile, flushed to the item's
+// tile buffer.  Reads x, v and the stored affine matrix.
+__global__ void __launch_bounds__(WG) k_p2g_scatter(TileArgs ta, const float* __restrict__ st, const float* __restrict__ aff,
+                                                    float4* __restrict__ tiles) {
+    __shared__ float tile[4 * TILE];
+    int n_items = ta.counters[CNT_ITEMS];
+    int lane = threadIdx.x & 63;
+    int stride = ta.stride;
+    for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
+        int4 it = ta.items[e];
+        const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
+        int bi, bj, bk;
+        block_origin(S, it.x, bi, bj, bk);
+        for (int k = threadIdx.x; k < 4 * TILE; k += WG) tile[k] = 0.f;
+        __syncthreads();
+        for (int k0 = 0; k0 < it.z; k0 += WG) {
+            int k = k0 + threadIdx.x;
+            bool valid = k < it.z;
+            int p = valid ? ta.order[it.y + k] : ta.order[it.y];
+            v3 x = load3(st, stride, 0, p);
+            v3 v = load3(st, stride, 3, p);
+            m3 Am = load9(aff, stride, 0, p);
+            Stencil sw;
+            stencil(S, x, sw);
+            int l0 = sw.base[0] - 4 * bi, l1 = sw.base[1] - 4 * bj, l2 = sw.base[2] - 4 * bk;
+            bool in_tile = (unsigned)l0 < 4u && (unsigned)l1 < 4u && (unsigned)l2 < 4u;
+            int key = (l0 << 4) | (l1 << 2) | l2;
+            v3 mv = S.p_mass * v;
+            const float* A = Am.a;
+            float dx = S.dx, pm = S.p_mass;
+            auto val = [&](int n, int c) -> float {
+                int a = n / 9, b = (n / 3) % 3, cc = n % 3;
+                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[cc][2];
+                if (c == 3) return wt * pm;
+                float d0 = ((float)a - sw.fx[0]) * dx, d1 = ((float)b - sw.fx[1]) * dx, d2 = ((float)cc - sw.fx[2]) * dx;
+                return wt * (comp(mv, c) + A[c * 3] * d0 + A[c * 3 + 1] * d1 + A[c * 3 + 2] * d2);
+            };
+            scatter_tile<4>(tile, valid && in_tile, key, l0, l1, l2, lane, val);
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < TILE; k += WG)
+            tiles[(size_t)e * TILE + k] = make_float4(tile[k], tile[TILE + k], tile[2 * TILE + k], tile[3 * TILE + k]);
+        __syncthreads();
     }
+}
+
+// first half of the p2g adjoint over work items: gathers the node adjoints of
+// (momentum, mass) from the block's LDS tile and produces the adjoints of v,
+// of the affine matrix (to gaff) and the weight/position part of x.
+__global__ void __launch_bounds__(WG) k_p2g_bwd_gather(TileArgs ta, const float* __restrict__ st, const float* __restrict__ aff,
+                                                       float* __restrict__ a0, float* __restrict__ gaff_out,
+                                                       const float4* __restrict__ g_adjpm) {
+    __shared__ float4 tg[TILE];
     int n_items = ta.counters[CNT_ITEMS];
     int stride = ta.stride;
     for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
@@ -856,10 +892,8 @@ __global__ void __launch_bounds__(WG) k_p2g_bwd_tile(TileArgs ta, const float* _
             int p = ta.order[it.y + k];
             v3 x = load3(st, stride, 0, p);
             v3 v = load3(st, stride, 3, p);
-            m3 C = load9(st, stride, 6, p);
-            m3 F = load9(st, stride, 15, p);
-            PTrace t;
-            particle_stress(S, C, F, t);
+            m3 Am = load9(aff, stride, 0, p);
+            const float* A = Am.a;
             Stencil sw;
             stencil(S, x, sw);
             int l0 = sw.base[0] - 4 * bi, l1 = sw.base[1] - 4 * bj, l2 = sw.base[2] - 4 * bk;
@@ -868,7 +902,6 @@ __global__ void __launch_bounds__(WG) k_p2g_bwd_tile(TileArgs ta, const float* _
             m3 gaff = mzero();
             float gfx[3] = {0.f, 0.f, 0.f};
             v3 mv = S.p_mass * v;
-            const float* A = t.affine.a;
             if (in_tile) {
 #pragma unroll
                 for (int a = 0; a < 3; a++)
@@ -904,65 +937,93 @@ __global__ void __launch_bounds__(WG) k_p2g_bwd_tile(TileArgs ta, const float* _
                             }
                         }
             }
-            m3 gC, gst;
-#pragma unroll
-            for (int kk = 0; kk < 9; kk++) { gC.a[kk] = S.p_mass * gaff.a[kk]; gst.a[kk] = S.stress_k * gaff.a[kk]; }
-            m3 FmR;
-#pragma unroll
-            for (int kk = 0; kk < 9; kk++) FmR.a[kk] = t.Fn.a[kk] - t.R.a[kk];
-            m3 sF = mmul(gst, t.Fn), sT = mmul_tn(gst, FmR);
-            m3 gF1 = load9(a1, stride, 15, p);
-            m3 gF, gR;
-#pragma unroll
-            for (int kk = 0; kk < 9; kk++) { gF.a[kk] = 2.f * S.mu * (sF.a[kk] + sT.a[kk]) + gF1.a[kk]; gR.a[kk] = -2.f * S.mu * sF.a[kk]; }
-            float gJ = S.lam * (2.f * t.J - 1.f) * (gst.a[0] + gst.a[4] + gst.a[8]);
-            m3 cf = mcof(t.Fn);
-#pragma unroll
-            for (int kk = 0; kk < 9; kk++) gF.a[kk] += gJ * cf.a[kk];
-            m3 gU = mmul(gR, t.V), gV = mmul_tn(gR, t.U), gFt = mzero();
-            float gS[3] = {0.f, 0.f, 0.f};
-            if (t.yield) {
-                m3 D = mzero();
-#pragma unroll
-                for (int kk = 0; kk < 3; kk++) D.a[kk * 4] = t.e[kk];
-                m3 aa = mmul(mmul(gF, t.V), D), bb = mmul(mmul_tn(gF, t.U), D);
-#pragma unroll
-                for (int kk = 0; kk < 9; kk++) { gU.a[kk] += aa.a[kk]; gV.a[kk] += bb.a[kk]; }
-                m3 M = mmul(mmul_tn(t.U, gF), t.V);
-                float gep[3], geps[3], geh[3];
-                float r = t.dg / t.ehn, gr = 0.f;
-#pragma unroll
-                for (int kk = 0; kk < 3; kk++) { gep[kk] = M.a[kk * 4] * t.e[kk]; geps[kk] = gep[kk]; geh[kk] = -r * gep[kk]; gr -= gep[kk] * t.eh[kk]; }
-                float gdg = gr / t.ehn;
-                float gehn = -gr * t.dg / (t.ehn * t.ehn) + gdg;
-#pragma unroll
-                for (int kk = 0; kk < 3; kk++) geh[kk] += gehn * t.eh[kk] / t.ehn;
-                float mg = (geh[0] + geh[1] + geh[2]) / 3.f;
-#pragma unroll
-                for (int kk = 0; kk < 3; kk++) {
-                    geps[kk] += geh[kk] - mg;
-                    if (0.05f < t.sig[kk]) gS[kk] += geps[kk] / t.sc[kk];
-                }
-            } else {
-                gFt = gF;
-            }
-            m3 bs = backward_svd(gU, gS, gV, t.U, t.sig, t.V);
-#pragma unroll
-            for (int kk = 0; kk < 9; kk++) gFt.a[kk] += bs.a[kk];
-            m3 Am;
-#pragma unroll
-            for (int kk = 0; kk < 9; kk++) Am.a[kk] = ((kk % 4 == 0) ? 1.f : 0.f) + S.dt * C.a[kk];
-            m3 gCa = mmul_nt(gFt, F), gF0 = mmul_tn(Am, gFt);
-#pragma unroll
-            for (int kk = 0; kk < 9; kk++) gC.a[kk] += S.dt * gCa.a[kk];
             v3 gx0 = load3(a0, stride, 0, p) + S.inv_dx * mk(gfx[0], gfx[1], gfx[2]);
             store3(a0, stride, 0, p, gx0);
             store3(a0, stride, 3, p, gv);
-            store9(a0, stride, 6, p, gC);
-            store9(a0, stride, 15, p, gF0);
+            store9(gaff_out, stride, 0, p, gaff);
         }
         __syncthreads();
     }
+}
+
+// second half of the p2g adjoint in particle-slot order: stress, return map,
+// backward_svd, F_tmp (mpm_simulator.py:121-156,205-216); writes the adjoints
+// of C and F of frame f.  Workgroup 0 also runs the primitive FK adjoint of
+// substep f.
+__global__ void __launch_bounds__(PPB) k_stress_bwd(ChunkArgs ca, const float* __restrict__ st, const float* __restrict__ gaff_in,
+                                                    const float* __restrict__ a1, float* __restrict__ a0,
+                                                    const PrimC* __restrict__ prims, int n_prims_total,
+                                                    const float* __restrict__ pstate, const float* __restrict__ pvel,
+                                                    float* __restrict__ gpstate, float* __restrict__ gpvel, int f) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < n_prims_total) {
+        int q = threadIdx.x;
+        size_t o0 = ((size_t)f * n_prims_total + q) * 8, o1 = ((size_t)(f + 1) * n_prims_total + q) * 8;
+        prim_fk_rev(prims[q], pstate + o0, pvel + o0, gpstate + o1, gpstate + o0, gpvel + o0);
+    }
+    int4 ch = ca.chunks[blockIdx.x];
+    int i = threadIdx.x;
+    if (i >= ch.z) return;
+    const SceneC& S = ca.scenes[ch.x];
+    int p = ch.y + i;
+    int stride = ca.stride;
+    m3 C = load9(st, stride, 6, p);
+    m3 F = load9(st, stride, 15, p);
+    PTrace t;
+    particle_stress(S, C, F, t);
+    m3 gaff = load9(gaff_in, stride, 0, p);
+    m3 gC, gst;
+#pragma unroll
+    for (int kk = 0; kk < 9; kk++) { gC.a[kk] = S.p_mass * gaff.a[kk]; gst.a[kk] = S.stress_k * gaff.a[kk]; }
+    m3 FmR;
+#pragma unroll
+    for (int kk = 0; kk < 9; kk++) FmR.a[kk] = t.Fn.a[kk] - t.R.a[kk];
+    m3 sF = mmul(gst, t.Fn), sT = mmul_tn(gst, FmR);
+    m3 gF1 = load9(a1, stride, 15, p);
+    m3 gF, gR;
+#pragma unroll
+    for (int kk = 0; kk < 9; kk++) { gF.a[kk] = 2.f * S.mu * (sF.a[kk] + sT.a[kk]) + gF1.a[kk]; gR.a[kk] = -2.f * S.mu * sF.a[kk]; }
+    float gJ = S.lam * (2.f * t.J - 1.f) * (gst.a[0] + gst.a[4] + gst.a[8]);
+    m3 cf = mcof(t.Fn);
+#pragma unroll
+    for (int kk = 0; kk < 9; kk++) gF.a[kk] += gJ * cf.a[kk];
+    m3 gU = mmul(gR, t.V), gV = mmul_tn(gR, t.U), gFt = mzero();
+    float gS[3] = {0.f, 0.f, 0.f};
+    if (t.yield) {
+        m3 D = mzero();
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) D.a[kk * 4] = t.e[kk];
+        m3 aa = mmul(mmul(gF, t.V), D), bb = mmul(mmul_tn(gF, t.U), D);
+#pragma unroll
+        for (int kk = 0; kk < 9; kk++) { gU.a[kk] += aa.a[kk]; gV.a[kk] += bb.a[kk]; }
+        m3 M = mmul(mmul_tn(t.U, gF), t.V);
+        float gep[3], geps[3], geh[3];
+        float r = t.dg / t.ehn, gr = 0.f;
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) { gep[kk] = M.a[kk * 4] * t.e[kk]; geps[kk] = gep[kk]; geh[kk] = -r * gep[kk]; gr -= gep[kk] * t.eh[kk]; }
+        float gdg = gr / t.ehn;
+        float gehn = -gr * t.dg / (t.ehn * t.ehn) + gdg;
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) geh[kk] += gehn * t.eh[kk] / t.ehn;
+        float mg = (geh[0] + geh[1] + geh[2]) / 3.f;
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) {
+            geps[kk] += geh[kk] - mg;
+            if (0.05f < t.sig[kk]) gS[kk] += geps[kk] / t.sc[kk];
+        }
+    } else {
+        gFt = gF;
+    }
+    m3 bs = backward_svd(gU, gS, gV, t.U, t.sig, t.V);
+#pragma unroll
+    for (int kk = 0; kk < 9; kk++) gFt.a[kk] += bs.a[kk];
+    m3 Am;
+#pragma unroll
+    for (int kk = 0; kk < 9; kk++) Am.a[kk] = ((kk % 4 == 0) ? 1.f : 0.f) + S.dt * C.a[kk];
+    m3 gCa = mmul_nt(gFt, F), gF0 = mmul_tn(Am, gFt);
+#pragma unroll
+    for (int kk = 0; kk < 9; kk++) gC.a[kk] += S.dt * gCa.a[kk];
+    store9(a0, stride, 6, p, gC);
+    store9(a0, stride, 15, p, gF0);
 }
 
 // compute_grid_m_kernel (mpm_simulator.py:456-466): dense per-scene output
@@ -1092,6 +1153,9 @@ struct mpm_ctx {
     int* d_blk_scene = nullptr;
     float* ckpt = nullptr;
     float* ringbuf = nullptr;
+    float* aff = nullptr;   // affine matrix (9 rows) per frame slot: checkpoints then ring
+    float* gaff = nullptr;  // adjoint of the affine matrix (9 rows), one frame
+    std::vector<int> aff_of; // frame whose affine matrix each slot holds (-1: none)
     float* adj[2] = {nullptr, nullptr};
     float4 *g_pm = nullptr, *g_v = nullptr, *g_adjpm = nullptr;
     int *blk_flag = nullptr, *node_flag = nullptr, *pblocks = nullptr, *nblocks = nullptr, *counters = nullptr;
@@ -1122,6 +1186,8 @@ struct mpm_ctx {
         return ringbuf + (size_t)((f % K) - 1) * frame_floats();
     }
     float* adj_ptr(int f) const { return adj[f & 1]; }
+    int slot_of(int f) const { return (f % K == 0) ? f / K : n_ckpt + (f % K) - 1; }
+    float* aff_ptr(int f) const { return aff + (size_t)slot_of(f) * 9 * stride; }
     ChunkArgs cargs() const { ChunkArgs a; a.scenes = d_scenes; a.chunks = d_chunks; a.n_chunks = n_chunks; a.stride = stride; return a; }
     BinArgs bargs() const {
         BinArgs a;
@@ -1165,7 +1231,7 @@ struct PhaseTimer {
 };
 
 struct Layout {
-    size_t off_scenes, off_prims, off_chunks, off_blk_scene, off_ckpt, off_ring, off_adj, off_grid, off_blkints, off_cells,
+    size_t off_scenes, off_prims, off_chunks, off_blk_scene, off_ckpt, off_ring, off_aff, off_adj, off_grid, off_blkints, off_cells,
         off_pints, off_items, off_tiles, off_counters, off_prim_bufs, total;
     int stride, n_chunks, n_blocks, n_ckpt, ring, K, max_items;
     long long n_nodes;
@@ -1206,6 +1272,7 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k) 
     size_t ff = (size_t)NST * stride * sizeof(float);
     L.off_ckpt = o; o = align256(o + ff * L.n_ckpt);
     L.off_ring = o; o = align256(o + ff * L.ring);
+    L.off_aff = o; o = align256(o + sizeof(float) * 9 * (size_t)stride * (L.n_ckpt + L.ring + 1));
     L.off_adj = o; o = align256(o + ff * 2);
     L.off_grid = o; o = align256(o + sizeof(float4) * (size_t)L.n_nodes * 3);
     L.off_blkints = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 7);   // flags, node flags, 2 lists, stamp, item0, nitems
@@ -1256,8 +1323,12 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
     float* out = c->frame_ptr(f + 1);
     if (run_bin(c, f, 1, st)) return -1;
     TileArgs ta = c->targs();
+    float* aff = c->aff_ptr(f);
+    { PhaseTimer t(c, MPM_PH_STRESS, st);
+      hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, out, aff); }
+    c->aff_of[c->slot_of(f)] = f;
     { PhaseTimer t(c, MPM_PH_P2G, st);
-      hipLaunchKernelGGL(k_p2g_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, out, c->tiles_pm); }
+      hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm); }
     { PhaseTimer t(c, MPM_PH_GRID, st);
       hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->tiles_pm, c->g_pm, c->g_v); }
     { PhaseTimer t(c, MPM_PH_G2P, st);
@@ -1291,8 +1362,15 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     float* a0 = c->adj_ptr(f);
     if (run_bin(c, f, 0, st)) return -1;
     TileArgs ta = c->targs();
+    float* aff = c->aff_ptr(f);
+    if (c->aff_of[c->slot_of(f)] != f) {
+        // the affine matrix of frame f is not stored (recomputed segment ended at f)
+        PhaseTimer t(c, MPM_PH_STRESS, st);
+        hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, (float*)nullptr, aff);
+        c->aff_of[c->slot_of(f)] = f;
+    }
     { PhaseTimer t(c, MPM_PH_P2G_GRID, st);
-      hipLaunchKernelGGL(k_p2g_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (float*)nullptr, c->tiles_pm); }
+      hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm); }
     { PhaseTimer t(c, MPM_PH_GRID_KEEP, st);
       hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->tiles_pm, c->g_pm, c->g_v); }
     { PhaseTimer t(c, MPM_PH_G2P_BWD, st);
@@ -1300,8 +1378,10 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     { PhaseTimer t(c, MPM_PH_GRID_BWD, st);
       hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->tiles_v, c->g_adjpm, c->pstate_g); }
     { PhaseTimer t(c, MPM_PH_P2G_BWD, st);
-      hipLaunchKernelGGL(k_p2g_bwd_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, a1, a0, c->g_adjpm, c->d_prims, c->n_prims,
-                         c->pstate, c->pvel, c->pstate_g, c->pvel_g, f); }
+      hipLaunchKernelGGL(k_p2g_bwd_gather, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, a0, c->gaff, c->g_adjpm); }
+    { PhaseTimer t(c, MPM_PH_STRESS_BWD, st);
+      hipLaunchKernelGGL(k_stress_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, (const float*)c->gaff, a1, a0, c->d_prims,
+                         c->n_prims, c->pstate, c->pvel, c->pstate_g, c->pvel_g, f); }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1378,6 +1458,9 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->d_blk_scene = (int*)(w + L.off_blk_scene);
     c->ckpt = (float*)(w + L.off_ckpt);
     c->ringbuf = (float*)(w + L.off_ring);
+    c->aff = (float*)(w + L.off_aff);
+    c->gaff = c->aff + (size_t)9 * L.stride * (L.n_ckpt + L.ring);
+    c->aff_of.assign(L.n_ckpt + L.ring, -1);
     c->adj[0] = (float*)(w + L.off_adj);
     c->adj[1] = c->adj[0] + c->frame_floats();
     float4* g = (float4*)(w + L.off_grid);
@@ -1556,6 +1639,7 @@ static void mark_simulated(mpm_ctx* c, int f) {
 // frame f was written directly: the trajectory restarts there
 static void mark_set(mpm_ctx* c, int f) {
     c->valid_hi = f;
+    for (auto& a : c->aff_of) a = -1;
     if (f % c->K == 0) { c->seg_valid = -1; c->seg_lo = c->seg_hi = -1; }
     else { c->seg_valid = f / c->K; c->seg_lo = f; c->seg_hi = f; }
 }

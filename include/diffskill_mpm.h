@@ -153,9 +153,10 @@ int mpm_copy_frame(mpm_ctx *ctx, int src, int dst, void *stream);
  * every engine kernel (no counterpart in the reference; measurement only).
  * Phases: */
 enum {
-    MPM_PH_BIN = 0, MPM_PH_COMPACT = 1, MPM_PH_BLOCKS = 2, MPM_PH_SCATTER = 3, MPM_PH_P2G = 4,
-    MPM_PH_GRID = 5, MPM_PH_G2P = 6, MPM_PH_P2G_GRID = 7, MPM_PH_GRID_KEEP = 8,
-    MPM_PH_G2P_BWD = 9, MPM_PH_GRID_BWD = 10, MPM_PH_P2G_BWD = 11, MPM_N_PHASES = 12
+    MPM_PH_BIN = 0, MPM_PH_COMPACT = 1, MPM_PH_BLOCKS = 2, MPM_PH_SCATTER = 3, MPM_PH_STRESS = 4,
+    MPM_PH_P2G = 5, MPM_PH_GRID = 6, MPM_PH_G2P = 7, MPM_PH_P2G_GRID = 8, MPM_PH_GRID_KEEP = 9,
+    MPM_PH_G2P_BWD = 10, MPM_PH_GRID_BWD = 11, MPM_PH_P2G_BWD = 12, MPM_PH_STRESS_BWD = 13,
+    MPM_N_PHASES = 14
 };
 int mpm_set_profiling(mpm_ctx *ctx, int on);
 /* Accumulated milliseconds and launch counts per phase (synchronizes the
