@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--ckpt-k", type=int, default=0, help="checkpoint every K frames (0 = store all)")
+    ap.add_argument("--keep-blocks", type=int, default=-1,
+                    help="node blocks of each substep's grid kept for the adjoint (-1 default, 0 rebuild every grid)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -154,7 +156,7 @@ def main():
     H, S = w["H"], w["substeps"]
     T = H * S
     log("workload: %d scenes, %d particles, %d substeps" % (len(w["scenes"]), int(np.sum(w["sizes"])), T))
-    batch = SceneBatch(w["scenes"], T + 1, ckpt_k=args.ckpt_k)
+    batch = SceneBatch(w["scenes"], T + 1, ckpt_k=args.ckpt_k, keep_blocks=args.keep_blocks)
     targets = build_targets(w, batch)
     init = torch.as_tensor(workloads.action_tensor(w["actions"], H))
     opt = BatchedTrajOpt(batch, H, S, targets, lr=1e-2, init_actions=init, use_graph=not args.no_graph)
@@ -219,7 +221,7 @@ def main():
     plain = None
     if not args.no_plain:
         del opt
-        p = PlainTrajOpt(w["scenes"], H, S, targets, lr=1e-2, init_actions=init, ckpt_k=args.ckpt_k)
+        p = PlainTrajOpt(w["scenes"], H, S, targets, lr=1e-2, init_actions=init, ckpt_k=args.ckpt_k, keep_blocks=args.keep_blocks)
         p.iteration()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -250,7 +252,8 @@ def main():
                                "64^3 grid each, 1-2 capsule/box tools, %d substeps fwd+adjoint + Adam step"
                                % (len(w["scenes"]), n_total, T),
                    "scenes": len(w["scenes"]), "particles": n_total, "substeps": T, "grid": 64,
-                   "ckpt_k": args.ckpt_k or "store-all", "hip_graph": not args.no_graph},
+                   "ckpt_k": args.ckpt_k or "store-all", "keep_blocks": batch.engine.keep_blocks,
+                   "hip_graph": not args.no_graph},
         "plain": plain, "grad_rel_err": grad_err, "grad_rel_err_l2": grad_err_l2, "grad_checks": grad_checks,
         "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
     }

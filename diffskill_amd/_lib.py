@@ -67,8 +67,8 @@ def load():
     vp, ip, i32, i64 = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int64
     fpp = ctypes.POINTER(ctypes.c_void_p)
     sig = {
-        "mpm_workspace_bytes": (i64, [vp, i32, i32, i32]),
-        "mpm_create": (i32, [vp, i32, vp, i32, i32, vp, i64, vp, ctypes.POINTER(vp)]),
+        "mpm_workspace_bytes": (i64, [vp, i32, i32, i32, i32]),
+        "mpm_create": (i32, [vp, i32, vp, i32, i32, i32, vp, i64, vp, ctypes.POINTER(vp)]),
         "mpm_destroy": (i32, [vp]),
         "mpm_frame": (i32, [vp, i32, vp, fpp]),
         "mpm_stride": (i32, [vp]),

@@ -31,16 +31,17 @@ class SceneBatch:
     scenes: list of dicts with keys ``sim`` (SIMULATOR cfg), ``prims``
     (PRIMITIVES cfgs), ``x0`` ((n, 3) positions; v = 0, F = I, C = 0 like
     reset_kernel) and optionally ``prim_states`` (one 8-vector per primitive;
-    default: each primitive's init state).
+    default: each primitive's init state).  ``keep_blocks``: node blocks of
+    each substep's grid kept for the adjoint (-1 default capacity, 0 none).
     """
 
-    def __init__(self, scenes, max_frames, ckpt_k=0, softness=666.0):
+    def __init__(self, scenes, max_frames, ckpt_k=0, softness=666.0, keep_blocks=-1):
         self.specs = scenes
         self.consts = [derive(s["sim"]) for s in scenes]
         self.prims = [_prim_objects(s["prims"], softness) for s in scenes]
         descs = [scene_desc(c, len(s["x0"]), len(p)) for c, s, p in zip(self.consts, scenes, self.prims)]
         pdescs = [p.desc() for ps in self.prims for p in ps]
-        self.engine = Engine(descs, pdescs, max_frames, ckpt_k)
+        self.engine = Engine(descs, pdescs, max_frames, ckpt_k, keep_blocks)
         e = self.engine
         self.n_scenes = len(scenes)
         self.prim_offsets = np.concatenate([[0], np.cumsum([len(p) for p in self.prims])]).astype(int)
@@ -171,11 +172,11 @@ class PlainTrajOpt:
     driven with its own launches (no packing, no graph) -- the way the
     reference drives one Taichi scene per process."""
 
-    def __init__(self, scenes, H, substeps, targets, lr=1e-2, init_actions=None, ckpt_k=0, softness=666.0):
+    def __init__(self, scenes, H, substeps, targets, lr=1e-2, init_actions=None, ckpt_k=0, softness=666.0, keep_blocks=-1):
         self.parts = []
         P0 = 0
         for k, sc in enumerate(scenes):
-            b = SceneBatch([sc], H * substeps + 1, ckpt_k=ckpt_k, softness=softness)
+            b = SceneBatch([sc], H * substeps + 1, ckpt_k=ckpt_k, softness=softness, keep_blocks=keep_blocks)
             ia = None
             if init_actions is not None:
                 ia = init_actions[:, P0:P0 + max(1, b.n_prims)]

@@ -196,37 +196,44 @@ DEV m3 mcof(const m3& F) {
 // norm, descending; Givens QR of B gives U and R ~ diag(sigma).  U and V are
 // rotations, sigma_1 >= sigma_2 >= |sigma_3|, and sigma_3 carries the sign of
 // det(A) -- the convention of the reference's ti.svd (mpm_simulator.py:129).
-DEV void jacobi_rot(float& s_pp, float& s_qq, float& s_pq, float& s_rp, float& s_rq,
-                    float& vp0, float& vp1, float& vp2, float& vq0, float& vq1, float& vq2) {
-    if (fabsf(s_pq) < 1e-30f) return;
-    float th = (s_qq - s_pp) / (2.f * s_pq);
-    float t = (th >= 0.f ? 1.f : -1.f) / (fabsf(th) + sqrtf(fmaf(th, th, 1.f)));
-    float c = 1.f / sqrtf(fmaf(t, t, 1.f));
-    float s = t * c;
+template <typename T>
+DEV void jacobi_rot(T& s_pp, T& s_qq, T& s_pq, T& s_rp, T& s_rq, T& vp0, T& vp1, T& vp2, T& vq0, T& vq1, T& vq2) {
+    if (fabs(s_pq) < T(1e-30)) return;
+    T th = (s_qq - s_pp) / (T(2) * s_pq);
+    T t = (th >= T(0) ? T(1) : T(-1)) / (fabs(th) + sqrt(fma(th, th, T(1))));
+    T c = T(1) / sqrt(fma(t, t, T(1)));
+    T s = t * c;
     s_pp -= t * s_pq;
     s_qq += t * s_pq;
-    s_pq = 0.f;
-    float rp = s_rp, rq = s_rq;
+    s_pq = T(0);
+    T rp = s_rp, rq = s_rq;
     s_rp = c * rp - s * rq;
     s_rq = s * rp + c * rq;
-    float a0 = vp0, a1 = vp1, a2 = vp2;
+    T a0 = vp0, a1 = vp1, a2 = vp2;
     vp0 = c * a0 - s * vq0; vp1 = c * a1 - s * vq1; vp2 = c * a2 - s * vq2;
     vq0 = s * a0 + c * vq0; vq1 = s * a1 + c * vq1; vq2 = s * a2 + c * vq2;
 }
-DEV void givens_qr(float& a, float& b, float& c, float& s) {
-    float r = sqrtf(a * a + b * b);
-    if (r > 1e-30f) { c = a / r; s = b / r; } else { c = 1.f; s = 0.f; }
+template <typename T>
+DEV void givens_qr(T& a, T& b, T& c, T& s) {
+    T r = sqrt(a * a + b * b);
+    if (r > T(1e-30)) { c = a / r; s = b / r; } else { c = T(1); s = T(0); }
 }
-DEV void svd3(const m3& A, m3& U, float sig[3], m3& V) {
+// Jacobi on A^T A (5 cyclic sweeps) for V, column sort, Givens QR of A V for
+// U and the singular values; computed in T, returned in fp32.
+template <typename T>
+DEV void svd3_t(const m3& Af, m3& U, float sig[3], m3& V) {
+    T A[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) A[i] = (T)Af.a[i];
     // S = A^T A
-    float s00 = A.a[0] * A.a[0] + A.a[3] * A.a[3] + A.a[6] * A.a[6];
-    float s11 = A.a[1] * A.a[1] + A.a[4] * A.a[4] + A.a[7] * A.a[7];
-    float s22 = A.a[2] * A.a[2] + A.a[5] * A.a[5] + A.a[8] * A.a[8];
-    float s01 = A.a[0] * A.a[1] + A.a[3] * A.a[4] + A.a[6] * A.a[7];
-    float s02 = A.a[0] * A.a[2] + A.a[3] * A.a[5] + A.a[6] * A.a[8];
-    float s12 = A.a[1] * A.a[2] + A.a[4] * A.a[5] + A.a[7] * A.a[8];
+    T s00 = A[0] * A[0] + A[3] * A[3] + A[6] * A[6];
+    T s11 = A[1] * A[1] + A[4] * A[4] + A[7] * A[7];
+    T s22 = A[2] * A[2] + A[5] * A[5] + A[8] * A[8];
+    T s01 = A[0] * A[1] + A[3] * A[4] + A[6] * A[7];
+    T s02 = A[0] * A[2] + A[3] * A[5] + A[6] * A[8];
+    T s12 = A[1] * A[2] + A[4] * A[5] + A[7] * A[8];
     // V columns c0, c1, c2 stored as v[row][col] -> keep per column
-    float v00 = 1.f, v10 = 0.f, v20 = 0.f, v01 = 0.f, v11 = 1.f, v21 = 0.f, v02 = 0.f, v12 = 0.f, v22 = 1.f;
+    T v00 = 1, v10 = 0, v20 = 0, v01 = 0, v11 = 1, v21 = 0, v02 = 0, v12 = 0, v22 = 1;
 #pragma unroll
     for (int sweep = 0; sweep < 5; sweep++) {
         // (p,q) = (0,1), r = 2
@@ -237,24 +244,24 @@ DEV void svd3(const m3& A, m3& U, float sig[3], m3& V) {
         jacobi_rot(s11, s22, s12, s01, s02, v01, v11, v21, v02, v12, v22);
     }
     // B = A V
-    float b[9];
+    T b[9];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        b[i * 3 + 0] = A.a[i * 3] * v00 + A.a[i * 3 + 1] * v10 + A.a[i * 3 + 2] * v20;
-        b[i * 3 + 1] = A.a[i * 3] * v01 + A.a[i * 3 + 1] * v11 + A.a[i * 3 + 2] * v21;
-        b[i * 3 + 2] = A.a[i * 3] * v02 + A.a[i * 3 + 1] * v12 + A.a[i * 3 + 2] * v22;
+        b[i * 3 + 0] = A[i * 3] * v00 + A[i * 3 + 1] * v10 + A[i * 3 + 2] * v20;
+        b[i * 3 + 1] = A[i * 3] * v01 + A[i * 3 + 1] * v11 + A[i * 3 + 2] * v21;
+        b[i * 3 + 2] = A[i * 3] * v02 + A[i * 3 + 1] * v12 + A[i * 3 + 2] * v22;
     }
-    float vv[9] = {v00, v01, v02, v10, v11, v12, v20, v21, v22};
-    float rho[3];
+    T vv[9] = {v00, v01, v02, v10, v11, v12, v20, v21, v22};
+    T rho[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) rho[j] = b[j] * b[j] + b[3 + j] * b[3 + j] + b[6 + j] * b[6 + j];
     // sort columns descending; each swap negates one column to keep det V = +1
 #define DSK_SWAP(i, j)                                                                        \
     if (rho[i] < rho[j]) {                                                                    \
-        float t_ = rho[i]; rho[i] = rho[j]; rho[j] = t_;                                      \
+        T t_ = rho[i]; rho[i] = rho[j]; rho[j] = t_;                                      \
         _Pragma("unroll") for (int r_ = 0; r_ < 3; r_++) {                                    \
-            float tb = b[r_ * 3 + i]; b[r_ * 3 + i] = b[r_ * 3 + j]; b[r_ * 3 + j] = -tb;     \
-            float tv = vv[r_ * 3 + i]; vv[r_ * 3 + i] = vv[r_ * 3 + j]; vv[r_ * 3 + j] = -tv; \
+            T tb = b[r_ * 3 + i]; b[r_ * 3 + i] = b[r_ * 3 + j]; b[r_ * 3 + j] = -tb;     \
+            T tv = vv[r_ * 3 + i]; vv[r_ * 3 + i] = vv[r_ * 3 + j]; vv[r_ * 3 + j] = -tv; \
         }                                                                                     \
     }
     DSK_SWAP(0, 1)
@@ -262,48 +269,57 @@ DEV void svd3(const m3& A, m3& U, float sig[3], m3& V) {
     DSK_SWAP(1, 2)
 #undef DSK_SWAP
     // Givens QR: zero b10, b20, b21 ; U accumulates the rotations
-    float u[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
-    float c, s;
+    T u[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    T c, s;
     // rows 0,1 -> zero b[1][0]
     givens_qr(b[0], b[3], c, s);
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        float r0 = b[j], r1 = b[3 + j];
+        T r0 = b[j], r1 = b[3 + j];
         b[j] = c * r0 + s * r1; b[3 + j] = -s * r0 + c * r1;
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) {  // U = U * G^T, G rows (c s; -s c)
-        float u0 = u[i * 3], u1 = u[i * 3 + 1];
+        T u0 = u[i * 3], u1 = u[i * 3 + 1];
         u[i * 3] = c * u0 + s * u1; u[i * 3 + 1] = -s * u0 + c * u1;
     }
     // rows 0,2 -> zero b[2][0]
     givens_qr(b[0], b[6], c, s);
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        float r0 = b[j], r2 = b[6 + j];
+        T r0 = b[j], r2 = b[6 + j];
         b[j] = c * r0 + s * r2; b[6 + j] = -s * r0 + c * r2;
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        float u0 = u[i * 3], u2 = u[i * 3 + 2];
+        T u0 = u[i * 3], u2 = u[i * 3 + 2];
         u[i * 3] = c * u0 + s * u2; u[i * 3 + 2] = -s * u0 + c * u2;
     }
     // rows 1,2 -> zero b[2][1]
     givens_qr(b[4], b[7], c, s);
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        float r1 = b[3 + j], r2 = b[6 + j];
+        T r1 = b[3 + j], r2 = b[6 + j];
         b[3 + j] = c * r1 + s * r2; b[6 + j] = -s * r1 + c * r2;
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        float u1 = u[i * 3 + 1], u2 = u[i * 3 + 2];
+        T u1 = u[i * 3 + 1], u2 = u[i * 3 + 2];
         u[i * 3 + 1] = c * u1 + s * u2; u[i * 3 + 2] = -s * u1 + c * u2;
     }
-    sig[0] = b[0]; sig[1] = b[4]; sig[2] = b[8];
+    sig[0] = (float)b[0]; sig[1] = (float)b[4]; sig[2] = (float)b[8];
 #pragma unroll
-    for (int i = 0; i < 9; i++) { U.a[i] = u[i]; V.a[i] = vv[i]; }
+    for (int i = 0; i < 9; i++) { U.a[i] = (float)u[i]; V.a[i] = (float)vv[i]; }
 }
+// The SVD runs in fp64 (fp32 in and out): the singular-value gaps of nearly
+// isotropic deformations (all sigma ~ 1, the elastic regime) are far below
+// fp32 resolution, and backward_svd divides by them; an fp32 Jacobi doubles
+// the gradient error of long horizons against an fp64 trajectory.
+#ifndef DSK_SVD_REAL
+#define DSK_SVD_REAL double
+#endif
+DEV void svd3(const m3& A, m3& U, float sig[3], m3& V) { svd3_t<DSK_SVD_REAL>(A, U, sig, V); }
+
 
 // ------------------------------------------------------------------ shapes (local frame)
 // Box._sdf (primitives.py:369-374); g receives d sdf / d p

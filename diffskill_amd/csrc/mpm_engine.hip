@@ -209,7 +209,8 @@ struct BinArgs {
     int* bin_rank;
     int* blk_flag;
     int* node_flag;
-    int* counters;
+    int* counters;   // this bin slot's counters
+    int* gstamp;     // pass stamp shared by every bin slot
     int* pblocks;
     int* blk_stamp;
     int* blk_item0;
@@ -237,7 +238,9 @@ __global__ void __launch_bounds__(PPB) k_bin(BinArgs a, const float* __restrict_
         a.counters[CNT_NBLK] = 0;
         a.counters[CNT_PART] = 0;
         a.counters[CNT_ITEMS] = 0;
-        a.counters[CNT_STAMP] += 1;
+        int stamp = a.gstamp[0] + 1;
+        a.gstamp[0] = stamp;
+        a.counters[CNT_STAMP] = stamp;
     }
     if (do_fk && blockIdx.x == 0 && (int)threadIdx.x < n_prims_total) {
         int q = threadIdx.x;
@@ -258,29 +261,35 @@ __global__ void __launch_bounds__(PPB) k_bin(BinArgs a, const float* __restrict_
         int b2 = min(max((int)(x.z * S.inv_dx - 0.5f), 0), S.n_grid - 1);
         int gb = S.blk_off + ((b0 >> 2) * S.nb + (b1 >> 2)) * S.nb + (b2 >> 2);
         q = gb * 64 + (((b0 & 3) << 4) | ((b1 & 3) << 2) | (b2 & 3));
-        a.blk_flag[gb] = 1;
     }
+    // group the wave's lanes by cell (ballots only), then every group leader
+    // claims its group's ranks with one atomic, all in a single instruction
     unsigned long long act = __ballot(valid);
-    int rank = 0;
+    int rank = 0, cnt = 0, leader = lane;
     while (act) {
         int lead = __ffsll((long long)act) - 1;
         int q0 = __shfl(q, lead);
         unsigned long long m = __ballot(valid && q == q0);
-        int base = 0;
-        if (lane == lead) base = atomicAdd(&a.cell_count[q0], __popcll(m));
-        base = __shfl(base, lead);
-        if (valid && q == q0) rank = base + __popcll(m & lanemask_lt(lane));
+        if (valid && q == q0) { rank = __popcll(m & lanemask_lt(lane)); leader = lead; }
+        if (lane == lead) cnt = __popcll(m);
         act &= ~m;
     }
+    int base = 0;
+    if (valid && lane == leader) {
+        base = atomicAdd(&a.cell_count[q], cnt);
+        a.blk_flag[q >> 6] = 1;
+    }
+    rank += __shfl(base, leader);
     if (valid) {
         a.bin_q[slot] = q;
         a.bin_rank[slot] = rank;
     }
 }
 
-// compact flagged blocks into a list (wave-aggregated atomics); clears the flags
+// compact flagged blocks into a list (wave-aggregated atomics); clears the
+// flags; map (optional) receives each listed block's position
 __global__ void __launch_bounds__(256) k_compact(int n_blocks, int* __restrict__ flag, int* __restrict__ list,
-                                                 int* __restrict__ counter) {
+                                                 int* __restrict__ counter, int* __restrict__ map) {
     for (int b0 = blockIdx.x * blockDim.x; b0 < n_blocks; b0 += gridDim.x * blockDim.x) {
         int b = b0 + threadIdx.x;
         int fl = (b < n_blocks) ? flag[b] : 0;
@@ -292,7 +301,9 @@ __global__ void __launch_bounds__(256) k_compact(int n_blocks, int* __restrict__
         if (lane == leader) base = atomicAdd(counter, __popcll(mask));
         base = __shfl(base, leader);
         if (fl) {
-            list[base + __popcll(mask & lanemask_lt(lane))] = b;
+            int pos = base + __popcll(mask & lanemask_lt(lane));
+            list[pos] = b;
+            if (map) map[b] = pos;
             flag[b] = 0;
         }
     }
@@ -300,17 +311,22 @@ __global__ void __launch_bounds__(256) k_compact(int n_blocks, int* __restrict__
 
 // per occupied cell block (one wave, lane = cell): scan the cell counts, claim
 // a particle range and work items, stamp the block, mark the node blocks its
-// 6x6x6 footprint covers
+// 6x6x6 footprint covers.  A workgroup takes 4 blocks at a time and claims
+// their ranges with one 64-bit atomic on the (particles, items) counter pair.
 __global__ void __launch_bounds__(256) k_blocks(BinArgs a) {
+    __shared__ unsigned long long s_tot[4];
+    __shared__ unsigned long long s_base;
     int lane = threadIdx.x & 63;
-    int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    int nw = (gridDim.x * blockDim.x) >> 6;
+    int wave = threadIdx.x >> 6;
     int n = a.counters[CNT_PBLK];
     int stamp = a.counters[CNT_STAMP];
-    for (int e = wave; e < n; e += nw) {
-        int gb = a.pblocks[e];
-        int cnt = a.cell_count[gb * 64 + lane];
-        a.cell_count[gb * 64 + lane] = 0;
+    unsigned long long* pair = reinterpret_cast<unsigned long long*>(a.counters + CNT_PART);  // low: particles, high: items
+    for (int e0 = blockIdx.x * 4; e0 < n; e0 += gridDim.x * 4) {
+        int e = e0 + wave;
+        bool has = e < n;
+        int gb = has ? a.pblocks[e] : 0;
+        int cnt = has ? a.cell_count[gb * 64 + lane] : 0;
+        if (has) a.cell_count[gb * 64 + lane] = 0;
         int incl = cnt;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -319,13 +335,15 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a) {
         }
         int total = __shfl(incl, 63);
         int nit = (total + WI - 1) / WI;
-        int start = 0, ib = 0;
-        if (lane == 0) {
-            start = atomicAdd(&a.counters[CNT_PART], total);
-            ib = atomicAdd(&a.counters[CNT_ITEMS], nit);
-        }
-        start = __shfl(start, 0);
-        ib = __shfl(ib, 0);
+        if (lane == 0) s_tot[wave] = ((unsigned long long)nit << 32) | (unsigned)total;
+        __syncthreads();
+        if (threadIdx.x == 0) s_base = atomicAdd(pair, s_tot[0] + s_tot[1] + s_tot[2] + s_tot[3]);
+        __syncthreads();
+        unsigned long long off = s_base;
+        for (int w = 0; w < wave; w++) off += s_tot[w];
+        int start = (int)(off & 0xffffffffull), ib = (int)(off >> 32);
+        __syncthreads();
+        if (!has) continue;
         a.cell_start[gb * 64 + lane] = start + incl - cnt;
         for (int it = lane; it < nit; it += 64) a.items[ib + it] = make_int4(gb, start + it * WI, min(WI, total - it * WI), 0);
         if (lane == 0) {
@@ -453,6 +471,11 @@ struct TileArgs {
     const int* counters;
     const int* order;
     int stride;
+    // grid kept from the forward pass: node block e of the active list at
+    // keep[e * 64 + node], nmap[block] = e; valid while the list fits keep_cap
+    const float4* keep_v;
+    const int* nmap;
+    int keep_cap;
 };
 
 struct GridArgs {
@@ -467,7 +490,14 @@ struct GridArgs {
     const float* pstate;   // [frame][n_prims_total][8]
     int n_prims_total;
     int f;
+    float4* keep_pm;       // kept grid (see TileArgs)
+    float4* keep_v;
+    int keep_cap;
+    int keep_write;        // grid_op stores its result into the kept grid
 };
+
+// true when the active node-block list of this pass fits the kept grid
+DEV bool kept_fits(const int* counters, int keep_cap) { return counters[CNT_NBLK] <= keep_cap; }
 
 DEV void node_coords(const SceneC& S, int gb, int lane, int& B0, int& B1, int& B2, int& I0, int& I1, int& I2, long long& node) {
     block_origin(S, gb, B0, B1, B2);
@@ -477,27 +507,50 @@ DEV void node_coords(const SceneC& S, int gb, int lane, int& B0, int& B1, int& B
     node = S.g_off + (long long)(gb - S.blk_off) * NODES_PER_BLK + lane;
 }
 
+DEV int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // owner-computes gather: the node's value is the sum, in a fixed order, of the
-// tiles of the (up to 8) cell blocks whose footprint covers it
+// tiles of the (up to 8) cell blocks whose footprint covers it.  The block
+// metadata is wave-uniform (scalar loads, issued together); the first tile of
+// every covering block is loaded before any is summed, so the dependent-load
+// chain is stamp -> item range -> tile, not eight of them in a row.
 DEV float4 gather_tiles(const GridArgs& ga, const SceneC& S, int B0, int B1, int B2, int lane, const float4* __restrict__ tiles,
                         int stamp) {
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     int l0 = lane >> 4, l1 = (lane >> 2) & 3, l2 = lane & 3;
+    int pgb[8], i0[8], ni[8];
 #pragma unroll
     for (int d = 0; d < 8; d++) {
-        int d0 = (d >> 2) & 1, d1 = (d >> 1) & 1, d2 = d & 1;
-        int P0 = B0 - d0, P1 = B1 - d1, P2 = B2 - d2;
-        if (P0 < 0 || P1 < 0 || P2 < 0) continue;
-        int pgb = S.blk_off + (P0 * S.nb + P1) * S.nb + P2;
-        if (ga.blk_stamp[pgb] != stamp) continue;
-        int t0 = l0 + 4 * d0, t1 = l1 + 4 * d1, t2 = l2 + 4 * d2;
-        if (t0 >= 6 || t1 >= 6 || t2 >= 6) continue;
-        int tidx = (t0 * 6 + t1) * 6 + t2;
-        int i0 = ga.blk_item0[pgb], ni = ga.blk_nitems[pgb];
-        for (int j = 0; j < ni; j++) {
-            float4 v = tiles[(size_t)(i0 + j) * TILE + tidx];
-            acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
-        }
+        int P0 = B0 - ((d >> 2) & 1), P1 = B1 - ((d >> 1) & 1), P2 = B2 - (d & 1);
+        pgb[d] = (P0 < 0 || P1 < 0 || P2 < 0) ? -1 : S.blk_off + (P0 * S.nb + P1) * S.nb + P2;
+    }
+    int st[8];
+#pragma unroll
+    for (int d = 0; d < 8; d++) st[d] = pgb[d] >= 0 ? ga.blk_stamp[pgb[d]] : stamp - 1;
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+        bool hit = st[d] == stamp;
+        i0[d] = hit ? ga.blk_item0[pgb[d]] : 0;
+        ni[d] = hit ? ga.blk_nitems[pgb[d]] : 0;
+    }
+    float4 v[8];
+    int tidx[8];
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+        int t0 = l0 + 4 * ((d >> 2) & 1), t1 = l1 + 4 * ((d >> 1) & 1), t2 = l2 + 4 * (d & 1);
+        bool in = t0 < 6 && t1 < 6 && t2 < 6;
+        tidx[d] = in ? (t0 * 6 + t1) * 6 + t2 : -1;
+        v[d] = (ni[d] > 0 && in) ? tiles[(size_t)i0[d] * TILE + tidx[d]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+        acc.x += v[d].x; acc.y += v[d].y; acc.z += v[d].z; acc.w += v[d].w;
+        // further work items of a block with more than WI particles
+        if (tidx[d] >= 0)
+            for (int j = 1; j < ni[d]; j++) {
+                float4 u = tiles[(size_t)(i0[d] + j) * TILE + tidx[d]];
+                acc.x += u.x; acc.y += u.y; acc.z += u.z; acc.w += u.w;
+            }
     }
     return acc;
 }
@@ -505,15 +558,18 @@ DEV float4 gather_tiles(const GridArgs& ga, const SceneC& S, int B0, int B1, int
 // grid_op over the active node blocks (mpm_simulator.py:230-262); one wave per
 // 4x4x4 node block.  Writes the gathered momentum/mass (kept for the adjoint)
 // and the velocity.
+// With only_overflow set it runs only when the kept grid overflowed (the
+// adjoint then recomputes the grid instead of reading the kept one).
 __global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
-                                                 float4* __restrict__ g_v) {
+                                                 float4* __restrict__ g_v, int only_overflow) {
+    if (only_overflow && kept_fits(ga.counters, ga.keep_cap)) return;
     int n_active = ga.counters[CNT_NBLK];
     int stamp = ga.counters[CNT_STAMP];
     int lane = threadIdx.x & 63;
     int wave = threadIdx.x >> 6;
-    for (int e = blockIdx.x * 4 + wave; e < n_active; e += gridDim.x * 4) {
-        int gb = ga.nblocks[e];
-        const SceneC& S = ga.scenes[ga.blk_scene[gb]];
+    for (int e = uniform(blockIdx.x * 4 + wave); e < n_active; e += gridDim.x * 4) {
+        int gb = uniform(ga.nblocks[e]);
+        const SceneC& S = ga.scenes[uniform(ga.blk_scene[gb])];
         int B0, B1, B2, I0, I1, I2;
         long long node;
         node_coords(S, gb, lane, B0, B1, B2, I0, I1, I2, node);
@@ -524,8 +580,13 @@ __global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, const float4* __re
             const float* ps1 = ga.pstate + ((size_t)(ga.f + 1) * ga.n_prims_total + S.prim_off) * 8;
             v = grid_node(S, ga.prims + S.prim_off, ps0, ps1, I0, I1, I2, pm, nullptr);
         }
+        float4 v4 = make_float4(v.x, v.y, v.z, 0.f);
         g_pm[node] = pm;
-        g_v[node] = make_float4(v.x, v.y, v.z, 0.f);
+        g_v[node] = v4;
+        if (ga.keep_write && e < ga.keep_cap) {
+            ga.keep_pm[(size_t)e * NODES_PER_BLK + lane] = pm;
+            ga.keep_v[(size_t)e * NODES_PER_BLK + lane] = v4;
+        }
     }
 }
 
@@ -535,6 +596,23 @@ DEV void load_tile(float4* tv, const SceneC& S, int bi, int bj, int bk, const fl
         int t0 = k / 36, t1 = (k / 6) % 6, t2 = k % 6;
         int I0 = bi * 4 + t0, I1 = bj * 4 + t1, I2 = bk * 4 + t2;
         tv[k] = in_grid(S, I0, I1, I2) ? g[node_id(S, I0, I1, I2)] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// the same tile from the kept (compact) grid
+DEV void load_tile_kept(float4* tv, const SceneC& S, int bi, int bj, int bk, const float4* __restrict__ keep,
+                        const int* __restrict__ nmap, int keep_cap) {
+    for (int k = threadIdx.x; k < TILE; k += WG) {
+        int t0 = k / 36, t1 = (k / 6) % 6, t2 = k % 6;
+        int I0 = bi * 4 + t0, I1 = bj * 4 + t1, I2 = bk * 4 + t2;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (in_grid(S, I0, I1, I2)) {
+            int gb = S.blk_off + ((I0 >> 2) * S.nb + (I1 >> 2)) * S.nb + (I2 >> 2);
+            int e = nmap[gb];
+            if ((unsigned)e < (unsigned)keep_cap)
+                v = keep[(size_t)e * NODES_PER_BLK + (((I0 & 3) << 4) | ((I1 & 3) << 2) | (I2 & 3))];
+        }
+        tv[k] = v;
     }
 }
 
@@ -596,10 +674,11 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __res
 // a1 = adjoint of frame f+1, st1 = frame f+1.
 __global__ void __launch_bounds__(WG) k_g2p_bwd_tile(TileArgs ta, const float* __restrict__ st, const float* __restrict__ st1,
                                                      const float* __restrict__ a1, float* __restrict__ a0,
-                                                     const float4* __restrict__ g_v, float4* __restrict__ tiles_v) {
+                                                     const float4* __restrict__ g_v, float4* __restrict__ tiles_v, int use_kept) {
     __shared__ float4 tv[TILE];
     __shared__ float acc[4 * TILE];
     int n_items = ta.counters[CNT_ITEMS];
+    bool kept = use_kept && kept_fits(ta.counters, ta.keep_cap);
     int lane = threadIdx.x & 63;
     int stride = ta.stride;
     for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
@@ -607,7 +686,8 @@ __global__ void __launch_bounds__(WG) k_g2p_bwd_tile(TileArgs ta, const float* _
         const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
         int bi, bj, bk;
         block_origin(S, it.x, bi, bj, bk);
-        load_tile(tv, S, bi, bj, bk, g_v);
+        if (kept) load_tile_kept(tv, S, bi, bj, bk, ta.keep_v, ta.nmap, ta.keep_cap);
+        else load_tile(tv, S, bi, bj, bk, g_v);
         for (int k = threadIdx.x; k < 4 * TILE; k += WG) acc[k] = 0.f;
         __syncthreads();
         for (int k0 = 0; k0 < it.z; k0 += WG) {
@@ -683,18 +763,19 @@ DEV float wave_sum(float v) {
 // reverse of grid_op: adjoint of (momentum, mass) per node, plus the adjoint
 // of the primitive states of frames f and f+1 (wave-reduced, then atomics).
 __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* __restrict__ g_pm, const float4* __restrict__ tiles_v,
-                                                     float4* __restrict__ g_adjpm, float* __restrict__ gpstate) {
+                                                     float4* __restrict__ g_adjpm, float* __restrict__ gpstate, int use_kept) {
     int n_active = ga.counters[CNT_NBLK];
+    bool kept = use_kept && kept_fits(ga.counters, ga.keep_cap);
     int stamp = ga.counters[CNT_STAMP];
     int lane = threadIdx.x & 63;
     int wave = threadIdx.x >> 6;
-    for (int e = blockIdx.x * 4 + wave; e < n_active; e += gridDim.x * 4) {
-        int gb = ga.nblocks[e];
-        const SceneC& S = ga.scenes[ga.blk_scene[gb]];
+    for (int e = uniform(blockIdx.x * 4 + wave); e < n_active; e += gridDim.x * 4) {
+        int gb = uniform(ga.nblocks[e]);
+        const SceneC& S = ga.scenes[uniform(ga.blk_scene[gb])];
         int B0, B1, B2, I0, I1, I2;
         long long node;
         node_coords(S, gb, lane, B0, B1, B2, I0, I1, I2, node);
-        float4 pm = g_pm[node];
+        float4 pm = kept ? ga.keep_pm[(size_t)e * NODES_PER_BLK + lane] : g_pm[node];
         float4 gv4 = gather_tiles(ga, S, B0, B1, B2, lane, tiles_v, stamp);
         bool act = pm.w > 1e-12f && in_grid(S, I0, I1, I2);
         const float* ps0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
@@ -829,8 +910,9 @@ __global__ void __launch_bounds__(PPB) k_stress(ChunkArgs ca, const float* __res
 // every particle of the item into the block's LDS tile, flushed to the item's
 // tile buffer.  Reads x, v and the stored affine matrix.
 __global__ void __launch_bounds__(WG) k_p2g_scatter(TileArgs ta, const float* __restrict__ st, const float* __restrict__ aff,
-                                                    float4* __restrict__ tiles) {
+                                                    float4* __restrict__ tiles, int only_overflow) {
     __shared__ float tile[4 * TILE];
+    if (only_overflow && kept_fits(ta.counters, ta.keep_cap)) return;
     int n_items = ta.counters[CNT_ITEMS];
     int lane = threadIdx.x & 63;
     int stride = ta.stride;
@@ -1158,10 +1240,22 @@ struct mpm_ctx {
     std::vector<int> aff_of; // frame whose affine matrix each slot holds (-1: none)
     float* adj[2] = {nullptr, nullptr};
     float4 *g_pm = nullptr, *g_v = nullptr, *g_adjpm = nullptr;
-    int *blk_flag = nullptr, *node_flag = nullptr, *pblocks = nullptr, *nblocks = nullptr, *counters = nullptr;
-    int *cell_count = nullptr, *cell_start = nullptr, *bin_q = nullptr, *bin_rank = nullptr, *order = nullptr;
-    int *blk_stamp = nullptr, *blk_item0 = nullptr, *blk_nitems = nullptr;
-    int4* items = nullptr;
+    int *blk_flag = nullptr, *node_flag = nullptr, *pblocks = nullptr, *gstamp = nullptr;
+    int *cell_count = nullptr, *cell_start = nullptr, *bin_q = nullptr, *bin_rank = nullptr;
+    // Bin slots: the binning of a substep (index list, work items, per-block
+    // item ranges and stamps, active node-block list and map, counters) and,
+    // when the grid is kept, that substep's grid (momentum/mass and velocity of
+    // its active node blocks).  With keeping on there is one bin slot per frame
+    // slot, so the adjoint of a substep the forward ran reuses both instead of
+    // binning and rebuilding the grid again; otherwise one shared slot.
+    struct BinSlot {
+        int* order; int4* items; int* blk_stamp; int* blk_item0; int* blk_nitems; int* nblocks; int* nmap; int* counters;
+        float4* keep_pm; float4* keep_v;
+    };
+    std::vector<BinSlot> bslots;
+    std::vector<int> kept_of;  // substep whose bins + grid each bin slot holds (-1: none)
+    bool keep = false;
+    int keep_cap = 0;          // node blocks a kept grid can hold
     float4 *tiles_pm = nullptr, *tiles_v = nullptr;
     int max_items = 0;
     int tile_ctas = 1024; // persistent work-item workgroups
@@ -1188,25 +1282,36 @@ struct mpm_ctx {
     float* adj_ptr(int f) const { return adj[f & 1]; }
     int slot_of(int f) const { return (f % K == 0) ? f / K : n_ckpt + (f % K) - 1; }
     float* aff_ptr(int f) const { return aff + (size_t)slot_of(f) * 9 * stride; }
+    int bslot_of(int f) const { return keep ? slot_of(f) : 0; }
+    void drop_kept() { for (auto& k : kept_of) k = -1; }
     ChunkArgs cargs() const { ChunkArgs a; a.scenes = d_scenes; a.chunks = d_chunks; a.n_chunks = n_chunks; a.stride = stride; return a; }
-    BinArgs bargs() const {
+    BinArgs bargs(int bs) const {
+        const BinSlot& b = bslots[bs];
         BinArgs a;
         a.scenes = d_scenes; a.chunks = d_chunks; a.blk_scene = d_blk_scene; a.stride = stride;
         a.cell_count = cell_count; a.cell_start = cell_start; a.bin_q = bin_q; a.bin_rank = bin_rank;
-        a.blk_flag = blk_flag; a.node_flag = node_flag; a.counters = counters; a.pblocks = pblocks;
-        a.blk_stamp = blk_stamp; a.blk_item0 = blk_item0; a.blk_nitems = blk_nitems; a.items = items; a.order = order;
+        a.blk_flag = blk_flag; a.node_flag = node_flag; a.counters = b.counters; a.gstamp = gstamp; a.pblocks = pblocks;
+        a.blk_stamp = b.blk_stamp; a.blk_item0 = b.blk_item0; a.blk_nitems = b.blk_nitems; a.items = b.items; a.order = b.order;
         return a;
     }
-    TileArgs targs() const {
+    TileArgs targs(int bs) const {
+        const BinSlot& b = bslots[bs];
         TileArgs t;
-        t.scenes = d_scenes; t.blk_scene = d_blk_scene; t.items = items; t.counters = counters; t.order = order; t.stride = stride;
+        t.scenes = d_scenes; t.blk_scene = d_blk_scene; t.items = b.items; t.counters = b.counters; t.order = b.order; t.stride = stride;
+        t.keep_v = b.keep_v; t.nmap = b.nmap; t.keep_cap = keep_cap;
         return t;
     }
-    GridArgs gargs(int f) const {
+    // write_keep: the forward grid_op also stores the grid into the bin slot
+    GridArgs gargs(int f, int bs, bool write_keep) const {
+        const BinSlot& b = bslots[bs];
         GridArgs g;
-        g.scenes = d_scenes; g.prims = d_prims; g.blk_scene = d_blk_scene; g.nblocks = nblocks; g.counters = counters;
-        g.blk_stamp = blk_stamp; g.blk_item0 = blk_item0; g.blk_nitems = blk_nitems;
+        g.scenes = d_scenes; g.prims = d_prims; g.blk_scene = d_blk_scene; g.nblocks = b.nblocks; g.counters = b.counters;
+        g.blk_stamp = b.blk_stamp; g.blk_item0 = b.blk_item0; g.blk_nitems = b.blk_nitems;
         g.pstate = pstate; g.n_prims_total = n_prims; g.f = f;
+        g.keep_pm = b.keep_pm;
+        g.keep_v = b.keep_v;
+        g.keep_cap = b.keep_pm ? keep_cap : 0;
+        g.keep_write = (write_keep && b.keep_pm) ? 1 : 0;
         return g;
     }
 };
@@ -1232,15 +1337,18 @@ struct PhaseTimer {
 
 struct Layout {
     size_t off_scenes, off_prims, off_chunks, off_blk_scene, off_ckpt, off_ring, off_aff, off_adj, off_grid, off_blkints, off_cells,
-        off_pints, off_items, off_tiles, off_counters, off_prim_bufs, total;
-    int stride, n_chunks, n_blocks, n_ckpt, ring, K, max_items;
+        off_pints, off_bslots, off_keep, off_tiles, off_counters, off_prim_bufs, total;
+    // one bin slot: order, items, blk_stamp, blk_item0, blk_nitems, nblocks, nmap, counters
+    size_t bs_order, bs_items, bs_stamp, bs_item0, bs_nitems, bs_nblocks, bs_nmap, bs_counters, bs_bytes, keep_bytes;
+    int stride, n_chunks, n_blocks, n_ckpt, ring, K, max_items, n_bslots, keep_cap;
     long long n_nodes;
     int n_prims;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k) {
+// keep_blocks: < 0 default capacity, 0 no kept grids, > 0 capacity in node blocks
+Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, int keep_blocks) {
     Layout L;
     memset(&L, 0, sizeof(L));
     int stride = 0, n_chunks = 0, n_blocks = 0, n_prims = 0;
@@ -1275,12 +1383,30 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k) 
     L.off_aff = o; o = align256(o + sizeof(float) * 9 * (size_t)stride * (L.n_ckpt + L.ring + 1));
     L.off_adj = o; o = align256(o + ff * 2);
     L.off_grid = o; o = align256(o + sizeof(float4) * (size_t)L.n_nodes * 3);
-    L.off_blkints = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 7);   // flags, node flags, 2 lists, stamp, item0, nitems
+    L.off_blkints = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 3);   // block flags, node flags, occupied list
     L.off_cells = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 64 * 2); // cell counts, cell starts
-    L.off_pints = o; o = align256(o + sizeof(int) * (size_t)stride * 3);         // bin key, bin rank, order
+    L.off_pints = o; o = align256(o + sizeof(int) * (size_t)stride * 2);         // bin key, bin rank
     // work items: at most one partial item per occupied cell block plus stride / WI full ones
     L.max_items = stride / WI + std::min(L.n_blocks, stride) + 1;
-    L.off_items = o; o = align256(o + sizeof(int4) * (size_t)L.max_items);
+    // default kept-grid capacity: generous for particle clouds of at least a few
+    // hundred particles per occupied cell block; a pass whose active node-block
+    // list does not fit falls back to recomputing its grid in the adjoint
+    int cap = keep_blocks < 0 ? stride / 64 + 64 * n : keep_blocks;
+    L.keep_cap = std::min(cap, L.n_blocks);
+    L.n_bslots = L.keep_cap > 0 ? L.n_ckpt + L.ring : 1;
+    size_t b = 0;
+    L.bs_order = b; b = align256(b + sizeof(int) * (size_t)stride);
+    L.bs_items = b; b = align256(b + sizeof(int4) * (size_t)L.max_items);
+    L.bs_stamp = b; b = align256(b + sizeof(int) * (size_t)L.n_blocks);
+    L.bs_item0 = b; b = align256(b + sizeof(int) * (size_t)L.n_blocks);
+    L.bs_nitems = b; b = align256(b + sizeof(int) * (size_t)L.n_blocks);
+    L.bs_nblocks = b; b = align256(b + sizeof(int) * (size_t)L.n_blocks);
+    L.bs_nmap = b; b = align256(b + sizeof(int) * (size_t)L.n_blocks);
+    L.bs_counters = b; b = align256(b + sizeof(int) * 64);
+    L.bs_bytes = b;
+    L.keep_bytes = (size_t)L.keep_cap * NODES_PER_BLK * sizeof(float4) * 2;
+    L.off_bslots = o; o = align256(o + L.bs_bytes * L.n_bslots);
+    L.off_keep = o; o = align256(o + L.keep_bytes * L.n_bslots);
     L.off_tiles = o; o = align256(o + sizeof(float4) * (size_t)L.max_items * TILE * 2);
     L.off_counters = o; o = align256(o + sizeof(int) * 64);
     L.off_prim_bufs = o; o = align256(o + sizeof(float) * 8 * (size_t)max_frames * std::max(1, n_prims) * 4);
@@ -1301,16 +1427,20 @@ int launch_fill(float* p, size_t n, hipStream_t st) {
 // counting-sort frame f's particles into cell blocks and work items
 int run_bin(mpm_ctx* c, int f, int do_fk, hipStream_t st) {
     const float* in = c->frame_ptr(f);
-    BinArgs ba = c->bargs();
+    int bs = c->bslot_of(f);
+    const mpm_ctx::BinSlot& B = c->bslots[bs];
+    BinArgs ba = c->bargs(bs);
     int cblocks = std::min(4096, (c->n_blocks + 255) / 256);
     { PhaseTimer t(c, MPM_PH_BIN, st);
       hipLaunchKernelGGL(k_bin, dim3(c->n_chunks), dim3(PPB), 0, st, ba, in, do_fk, c->d_prims, c->n_prims, c->pstate, c->pvel, f); }
     { PhaseTimer t(c, MPM_PH_COMPACT, st);
-      hipLaunchKernelGGL(k_compact, dim3(cblocks), dim3(256), 0, st, c->n_blocks, c->blk_flag, c->pblocks, c->counters + CNT_PBLK); }
+      hipLaunchKernelGGL(k_compact, dim3(cblocks), dim3(256), 0, st, c->n_blocks, c->blk_flag, c->pblocks, B.counters + CNT_PBLK,
+                         (int*)nullptr); }
     { PhaseTimer t(c, MPM_PH_BLOCKS, st);
       hipLaunchKernelGGL(k_blocks, dim3(c->grid_ctas), dim3(256), 0, st, ba); }
     { PhaseTimer t(c, MPM_PH_COMPACT, st);
-      hipLaunchKernelGGL(k_compact, dim3(cblocks), dim3(256), 0, st, c->n_blocks, c->node_flag, c->nblocks, c->counters + CNT_NBLK); }
+      hipLaunchKernelGGL(k_compact, dim3(cblocks), dim3(256), 0, st, c->n_blocks, c->node_flag, B.nblocks, B.counters + CNT_NBLK,
+                         B.nmap); }
     { PhaseTimer t(c, MPM_PH_SCATTER, st);
       hipLaunchKernelGGL(k_scatter, dim3(c->n_chunks), dim3(PPB), 0, st, ba); }
     HIPCHK(hipGetLastError());
@@ -1322,18 +1452,20 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
     const float* in = c->frame_ptr(f);
     float* out = c->frame_ptr(f + 1);
     if (run_bin(c, f, 1, st)) return -1;
-    TileArgs ta = c->targs();
+    int bs = c->bslot_of(f);
+    TileArgs ta = c->targs(bs);
     float* aff = c->aff_ptr(f);
     { PhaseTimer t(c, MPM_PH_STRESS, st);
       hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, out, aff); }
     c->aff_of[c->slot_of(f)] = f;
     { PhaseTimer t(c, MPM_PH_P2G, st);
-      hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm); }
+      hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm, 0); }
     { PhaseTimer t(c, MPM_PH_GRID, st);
-      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->tiles_pm, c->g_pm, c->g_v); }
+      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f, bs, c->keep), c->tiles_pm, c->g_pm, c->g_v, 0); }
     { PhaseTimer t(c, MPM_PH_G2P, st);
       hipLaunchKernelGGL(k_g2p_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, out, c->g_v); }
     HIPCHK(hipGetLastError());
+    c->kept_of[bs] = c->keep ? f : -1;
     return 0;
 }
 
@@ -1360,8 +1492,15 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     const float* in1 = c->frame_ptr(f + 1);
     const float* a1 = c->adj_ptr(f + 1);
     float* a0 = c->adj_ptr(f);
-    if (run_bin(c, f, 0, st)) return -1;
-    TileArgs ta = c->targs();
+    int bs = c->bslot_of(f);
+    // bins and grid of substep f kept from its forward run, or rebuilt here
+    bool kept = c->keep && c->kept_of[bs] == f;
+    if (!kept) {
+        c->kept_of[bs] = -1;
+        if (run_bin(c, f, 0, st)) return -1;
+    }
+    TileArgs ta = c->targs(bs);
+    GridArgs ga = c->gargs(f, bs, false);
     float* aff = c->aff_ptr(f);
     if (c->aff_of[c->slot_of(f)] != f) {
         // the affine matrix of frame f is not stored (recomputed segment ended at f)
@@ -1369,14 +1508,21 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
         hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, (float*)nullptr, aff);
         c->aff_of[c->slot_of(f)] = f;
     }
-    { PhaseTimer t(c, MPM_PH_P2G_GRID, st);
-      hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm); }
-    { PhaseTimer t(c, MPM_PH_GRID_KEEP, st);
-      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->tiles_pm, c->g_pm, c->g_v); }
+    // rebuild the grid when it was not kept; a kept grid whose active list
+    // overflowed the capacity is rebuilt on the device (only_overflow launches)
+    if (!kept || c->keep_cap < c->n_blocks) {
+        int only = kept ? 1 : 0;
+        { PhaseTimer t(c, MPM_PH_P2G_GRID, st);
+          hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm, only); }
+        { PhaseTimer t(c, MPM_PH_GRID_KEEP, st);
+          hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->tiles_pm, c->g_pm, c->g_v, only); }
+    }
+    int use_kept = kept ? 1 : 0;
     { PhaseTimer t(c, MPM_PH_G2P_BWD, st);
-      hipLaunchKernelGGL(k_g2p_bwd_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, in1, a1, a0, c->g_v, c->tiles_v); }
+      hipLaunchKernelGGL(k_g2p_bwd_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, in1, a1, a0, c->g_v, c->tiles_v, use_kept); }
     { PhaseTimer t(c, MPM_PH_GRID_BWD, st);
-      hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f), c->g_pm, c->tiles_v, c->g_adjpm, c->pstate_g); }
+      hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->g_pm, c->tiles_v, c->g_adjpm, c->pstate_g,
+                         use_kept); }
     { PhaseTimer t(c, MPM_PH_P2G_BWD, st);
       hipLaunchKernelGGL(k_p2g_bwd_gather, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, a0, c->gaff, c->g_adjpm); }
     { PhaseTimer t(c, MPM_PH_STRESS_BWD, st);
@@ -1420,13 +1566,13 @@ extern "C" {
 
 const char* mpm_last_error(void) { return g_err.c_str(); }
 
-int64_t mpm_workspace_bytes(const mpm_scene_desc* scenes, int n_scenes, int max_frames, int ckpt_k) {
+int64_t mpm_workspace_bytes(const mpm_scene_desc* scenes, int n_scenes, int max_frames, int ckpt_k, int keep_blocks) {
     if (!scenes || n_scenes <= 0 || max_frames < 2) return -1;
-    return (int64_t)make_layout(scenes, n_scenes, max_frames, ckpt_k).total;
+    return (int64_t)make_layout(scenes, n_scenes, max_frames, ckpt_k, keep_blocks).total;
 }
 
-int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* prims, int max_frames, int ckpt_k, void* workspace,
-               int64_t workspace_bytes, void* stream, mpm_ctx** out) {
+int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* prims, int max_frames, int ckpt_k, int keep_blocks,
+               void* workspace, int64_t workspace_bytes, void* stream, mpm_ctx** out) {
     if (!out) return fail("mpm_create: out is null");
     *out = nullptr;
     if (!scenes || n_scenes <= 0) return fail("mpm_create: no scenes");
@@ -1436,7 +1582,7 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
         if (scenes[i].n_particles < 0) return fail("mpm_create: negative particle count");
         if (scenes[i].n_prims < 0 || scenes[i].n_prims > 4) return fail("mpm_create: at most 4 primitives per scene");
     }
-    Layout L = make_layout(scenes, n_scenes, max_frames, ckpt_k);
+    Layout L = make_layout(scenes, n_scenes, max_frames, ckpt_k, keep_blocks);
     if (!workspace || workspace_bytes < (int64_t)L.total)
         return fail("mpm_create: workspace smaller than mpm_workspace_bytes()");
     hipStream_t st = S_(stream);
@@ -1471,20 +1617,35 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->blk_flag = bi;
     c->node_flag = bi + (size_t)L.n_blocks;
     c->pblocks = bi + (size_t)L.n_blocks * 2;
-    c->nblocks = bi + (size_t)L.n_blocks * 3;
-    c->blk_stamp = bi + (size_t)L.n_blocks * 4;
-    c->blk_item0 = bi + (size_t)L.n_blocks * 5;
-    c->blk_nitems = bi + (size_t)L.n_blocks * 6;
     c->cell_count = (int*)(w + L.off_cells);
     c->cell_start = c->cell_count + (size_t)L.n_blocks * 64;
     c->bin_q = (int*)(w + L.off_pints);
     c->bin_rank = c->bin_q + L.stride;
-    c->order = c->bin_q + 2 * (size_t)L.stride;
-    c->items = (int4*)(w + L.off_items);
+    c->keep = L.keep_cap > 0;
+    c->keep_cap = L.keep_cap;
+    for (int k = 0; k < L.n_bslots; k++) {
+        char* b = w + L.off_bslots + (size_t)k * L.bs_bytes;
+        mpm_ctx::BinSlot S;
+        S.order = (int*)(b + L.bs_order);
+        S.items = (int4*)(b + L.bs_items);
+        S.blk_stamp = (int*)(b + L.bs_stamp);
+        S.blk_item0 = (int*)(b + L.bs_item0);
+        S.blk_nitems = (int*)(b + L.bs_nitems);
+        S.nblocks = (int*)(b + L.bs_nblocks);
+        S.nmap = (int*)(b + L.bs_nmap);
+        S.counters = (int*)(b + L.bs_counters);
+        S.keep_pm = S.keep_v = nullptr;
+        if (c->keep) {
+            S.keep_pm = (float4*)(w + L.off_keep + (size_t)k * L.keep_bytes);
+            S.keep_v = S.keep_pm + (size_t)L.keep_cap * NODES_PER_BLK;
+        }
+        c->bslots.push_back(S);
+    }
+    c->kept_of.assign(L.n_bslots, -1);
     c->max_items = L.max_items;
     c->tiles_pm = (float4*)(w + L.off_tiles);
     c->tiles_v = c->tiles_pm + (size_t)L.max_items * TILE;
-    c->counters = (int*)(w + L.off_counters);
+    c->gstamp = (int*)(w + L.off_counters);
     size_t pb = (size_t)8 * max_frames * std::max(1, L.n_prims);
     c->pstate = (float*)(w + L.off_prim_bufs);
     c->pstate_g = c->pstate + pb;
@@ -1640,6 +1801,7 @@ static void mark_simulated(mpm_ctx* c, int f) {
 static void mark_set(mpm_ctx* c, int f) {
     c->valid_hi = f;
     for (auto& a : c->aff_of) a = -1;
+    c->drop_kept();
     if (f % c->K == 0) { c->seg_valid = -1; c->seg_lo = c->seg_hi = -1; }
     else { c->seg_valid = f / c->K; c->seg_lo = f; c->seg_hi = f; }
 }
@@ -1673,6 +1835,7 @@ int mpm_set_prim_frame(mpm_ctx* ctx, int f, const float* src, void* stream) {
     if (ctx->n_prims == 0) return 0;
     HIPCHK(hipMemcpyAsync(ctx->pstate + (size_t)f * ctx->n_prims * 8, src, sizeof(float) * 8 * ctx->n_prims, hipMemcpyDeviceToDevice,
                           S_(stream)));
+    ctx->drop_kept();  // kept grids were built with the old primitive states
     return 0;
 }
 
@@ -1774,6 +1937,7 @@ int mpm_update_prims(mpm_ctx* ctx, const mpm_prim_desc* prims, void* stream) {
         if (prims[q].type != ctx->prims[q].type) return fail("mpm_update_prims: primitive types may not change");
         ctx->prims[q] = to_primc(prims[q]);
     }
+    ctx->drop_kept();  // kept grids were built with the old primitive parameters
     if (ctx->n_prims == 0) return 0;
     HIPCHK(hipMemcpyAsync(ctx->d_prims, ctx->prims.data(), sizeof(PrimC) * ctx->n_prims, hipMemcpyHostToDevice, S_(stream)));
     HIPCHK(hipStreamSynchronize(S_(stream)));

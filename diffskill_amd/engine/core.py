@@ -47,23 +47,25 @@ def scene_desc(c, n_particles, n_prims):
 class Engine:
     """A packed multi-scene context (scene k: constants, particle count, primitives)."""
 
-    def __init__(self, scene_descs, prim_descs, max_frames, ckpt_k=0):
+    def __init__(self, scene_descs, prim_descs, max_frames, ckpt_k=0, keep_blocks=-1):
         if not torch.cuda.is_available():
             raise RuntimeError("the MI355X engine needs a HIP device (torch.cuda.is_available() is False)")
         self.L = _lib.load()
         self.n_scenes = len(scene_descs)
         self.max_frames = int(max_frames)
         self.ckpt_k = int(ckpt_k or 0)
+        # node blocks of each substep's grid kept for the adjoint (-1 default, 0 none)
+        self.keep_blocks = int(keep_blocks)
         self._scenes = (_lib.SceneDesc * self.n_scenes)(*scene_descs)
         self.n_prims = len(prim_descs)
         self._prims = (_lib.PrimDesc * max(1, self.n_prims))(*prim_descs)
-        nbytes = self.L.mpm_workspace_bytes(self._scenes, self.n_scenes, self.max_frames, self.ckpt_k)
+        nbytes = self.L.mpm_workspace_bytes(self._scenes, self.n_scenes, self.max_frames, self.ckpt_k, self.keep_blocks)
         if nbytes <= 0:
             raise RuntimeError("mpm_workspace_bytes failed")
         self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device="cuda")
         ctx = ctypes.c_void_p()
         _lib.check(self.L.mpm_create(self._scenes, self.n_scenes, self._prims, self.max_frames, self.ckpt_k,
-                                     ctypes.c_void_p(self.ws.data_ptr()), int(nbytes), _stream(), ctypes.byref(ctx)),
+                                     self.keep_blocks, ctypes.c_void_p(self.ws.data_ptr()), int(nbytes), _stream(), ctypes.byref(ctx)),
                    "mpm_create")
         self.ctx = ctx
         self.stride = self.L.mpm_stride(ctx)

@@ -66,7 +66,7 @@ class _GridGrad:
 
 
 class MPMSimulator:
-    def __init__(self, cfg, primitives=(), ckpt_k=None):
+    def __init__(self, cfg, primitives=(), ckpt_k=None, keep_blocks=-1):
         c = derive(cfg)
         self.cfg = cfg
         self.consts = c
@@ -92,6 +92,7 @@ class MPMSimulator:
         if ckpt_k is None:
             ckpt_k = int(cfg.get("ckpt_k", 0) or 0) if hasattr(cfg, "get") else 0
         self.ckpt_k = ckpt_k
+        self.keep_blocks = keep_blocks
         self.grid_m = _GridField(self)
         # particles are stored in cell order: slot i holds the caller's particle perm[i]
         self._perm = None
@@ -115,7 +116,8 @@ class MPMSimulator:
         keep = None
         if self.engine is not None and self.n_primitive > 0:
             keep = self.engine.prim_state[0, :self.n_primitive].clone()
-        self.engine = Engine([scene_desc(self.consts, n, self.n_primitive)], self._prim_descs(), self.max_steps, self.ckpt_k)
+        self.engine = Engine([scene_desc(self.consts, n, self.n_primitive)], self._prim_descs(), self.max_steps, self.ckpt_k,
+                             self.keep_blocks)
         self._ctx = self.engine.ctx
         if keep is not None:
             self.engine.prim_state[0, :self.n_primitive] = keep

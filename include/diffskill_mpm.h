@@ -82,15 +82,21 @@ typedef struct mpm_ctx mpm_ctx;
 
 /* Bytes of device workspace a context needs: particle frames (checkpoint
  * every ckpt_k frames plus one live segment of ckpt_k - 1 frames; ckpt_k >=
- * max_frames means store-all), two adjoint frames, grids, primitive frames. */
-int64_t mpm_workspace_bytes(const mpm_scene_desc *scenes, int n_scenes, int max_frames, int ckpt_k);
+ * max_frames means store-all), two adjoint frames, grids, primitive frames.
+ * keep_blocks sets how many active 4x4x4 node blocks of each substep's grid
+ * (and that substep's binning) are kept per frame slot for the adjoint, which
+ * then skips re-binning and rebuilding the grid: < 0 picks a default from the
+ * particle count, 0 keeps nothing (the adjoint rebuilds every grid), > 0 is
+ * the capacity.  A substep whose active list exceeds it is rebuilt. */
+int64_t mpm_workspace_bytes(const mpm_scene_desc *scenes, int n_scenes, int max_frames, int ckpt_k,
+                            int keep_blocks);
 
 /* Create a context over a caller-owned workspace (device pointer, >=
  * mpm_workspace_bytes).  prims: concatenated primitives of all scenes, in
  * scene order.  The workspace must stay alive until mpm_destroy. */
 int mpm_create(const mpm_scene_desc *scenes, int n_scenes, const mpm_prim_desc *prims,
-               int max_frames, int ckpt_k, void *workspace, int64_t workspace_bytes,
-               void *stream, mpm_ctx **out);
+               int max_frames, int ckpt_k, int keep_blocks, void *workspace,
+               int64_t workspace_bytes, void *stream, mpm_ctx **out);
 int mpm_destroy(mpm_ctx *ctx);
 
 /* Device pointer of frame f's particle state (SoA, 24 rows of `stride`

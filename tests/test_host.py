@@ -32,14 +32,17 @@ def test_binding_signatures_and_error_string():
     L = _lib.load()
     assert L.mpm_last_error() is not None
     # host-side validation only: no device work
-    assert L.mpm_workspace_bytes(None, 0, 10, 0) == -1
+    assert L.mpm_workspace_bytes(None, 0, 10, 0, -1) == -1
     d = _lib.SceneDesc()
     d.n_grid, d.n_particles, d.n_prims = 64, 1000, 1
-    assert L.mpm_workspace_bytes(ctypes.byref(d), 1, 101, 0) > 0
-    store_all = L.mpm_workspace_bytes(ctypes.byref(d), 1, 1001, 0)
-    ckpt = L.mpm_workspace_bytes(ctypes.byref(d), 1, 1001, 50)
+    assert L.mpm_workspace_bytes(ctypes.byref(d), 1, 101, 0, -1) > 0
+    store_all = L.mpm_workspace_bytes(ctypes.byref(d), 1, 1001, 0, -1)
+    ckpt = L.mpm_workspace_bytes(ctypes.byref(d), 1, 1001, 50, -1)
     assert ckpt < store_all
-    rc = L.mpm_create(ctypes.byref(d), 1, None, 1, 0, None, 0, None, ctypes.byref(ctypes.c_void_p()))
+    # kept grids and bins cost memory per frame slot; 0 keeps nothing
+    assert L.mpm_workspace_bytes(ctypes.byref(d), 1, 1001, 0, 0) < store_all
+    assert L.mpm_workspace_bytes(ctypes.byref(d), 1, 1001, 0, 64) < L.mpm_workspace_bytes(ctypes.byref(d), 1, 1001, 0, 128)
+    rc = L.mpm_create(ctypes.byref(d), 1, None, 1, 0, -1, None, 0, None, ctypes.byref(ctypes.c_void_p()))
     assert rc != 0 and b"max_frames" in L.mpm_last_error()
 
 
