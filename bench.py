@@ -128,6 +128,7 @@ def main():
     ap.add_argument("--keep-blocks", type=int, default=-1,
                     help="node blocks of each substep's grid kept for the adjoint (-1 default, 0 rebuild every grid)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
@@ -161,6 +162,11 @@ def main():
     init = torch.as_tensor(workloads.action_tensor(w["actions"], H))
     opt = BatchedTrajOpt(batch, H, S, targets, lr=1e-2, init_actions=init, use_graph=not args.no_graph)
     n_total = batch.total_particles
+    launch = None
+    if not args.no_tune:
+        launch, tune_ms = opt.autotune()
+        log("autotuned launch shape %s (%d candidates, %.1f-%.1f ms per eager pass)"
+            % (launch, len(tune_ms), min(tune_ms.values()), max(tune_ms.values())))
 
     # ---- gradient accuracy of the batched path vs the fp64 oracle (smallest scene)
     grad_err, grad_err_l2, grad_checks = None, None, []
@@ -253,6 +259,7 @@ def main():
                                % (len(w["scenes"]), n_total, T),
                    "scenes": len(w["scenes"]), "particles": n_total, "substeps": T, "grid": 64,
                    "ckpt_k": args.ckpt_k or "store-all", "keep_blocks": batch.engine.keep_blocks,
+                   "launch": launch or batch.engine.get_launch(), "autotuned": launch is not None,
                    "hip_graph": not args.no_graph},
         "plain": plain, "grad_rel_err": grad_err, "grad_rel_err_l2": grad_err_l2, "grad_checks": grad_checks,
         "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,

@@ -138,6 +138,14 @@ class BatchedTrajOpt:
         for s in range(self.H):
             e.action_grad_into(s, self.S, self.grads[s])
 
+    def autotune(self, candidates=None, reps=2):
+        """Pick the engine's launch shape (particles per work item, workgroup
+        counts) by timing this optimiser's own pass; before graph capture."""
+        from .tune import autotune
+        if self.graph is not None:
+            raise RuntimeError("autotune must run before the pass is captured into a graph")
+        return autotune(self.b.engine, self._device_pass, candidates, reps)
+
     def _capture(self):
         # warm up on a side stream, then capture
         s = torch.cuda.Stream()

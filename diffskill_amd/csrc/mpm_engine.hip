@@ -189,14 +189,22 @@ struct ChunkArgs {
 // Each substep the particles of a frame are counting-sorted by (4x4x4 cell
 // block, cell): per-cell counts with wave-aggregated atomics, per-block scans,
 // then an index scatter.  A block's particles form one or more work items of
-// at most WI particles; the particle phases run one workgroup per work item
+// at most `wi` particles (a runtime launch parameter, a multiple of WG
+// between WI_MIN and WI_MAX, default WI); the particle phases run one
+// workgroup per work item
 // over a 6x6x6-node LDS tile (the stencil footprint of a cell block).
-constexpr int WI = 1024;     // particles per work item
+constexpr int WI = 1024;     // default particles per work item
+constexpr int WI_MIN = 256;  // the work-item capacity of a context is sized for this
+constexpr int WI_MAX = 8192;
 constexpr int TILE = 216;    // 6x6x6 nodes
 constexpr int WG = 256;      // threads per work-item workgroup
 constexpr int MAXSEG = 4;    // distinct cells per wave reduced through the transpose tree
 
-enum { CNT_PBLK = 0, CNT_NBLK = 1, CNT_PART = 2, CNT_ITEMS = 3, CNT_STAMP = 4 };
+// per bin slot counters
+enum { CNT_NBLK = 1, CNT_PART = 2, CNT_ITEMS = 3, CNT_STAMP = 4 };
+// global binning state: the pass stamp base and the occupied-block count of
+// the pass in flight (both advanced/reset by the pass's last kernel)
+enum { G_STAMP = 0, G_PBLK = 1 };
 
 struct BinArgs {
     const SceneC* scenes;
@@ -210,13 +218,14 @@ struct BinArgs {
     int* blk_flag;
     int* node_flag;
     int* counters;   // this bin slot's counters
-    int* gstamp;     // pass stamp shared by every bin slot
+    int* gstamp;     // global binning state (G_STAMP, G_PBLK)
     int* pblocks;
     int* blk_stamp;
     int* blk_item0;
     int* blk_nitems;
     int4* items;
     int* order;
+    int wi;          // particles per work item
 };
 
 DEV unsigned long long lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
@@ -230,16 +239,16 @@ DEV void block_origin(const SceneC& S, int gb, int& bi, int& bj, int& bk) {
 }
 
 // bin frame f's particles by cell; optionally run primitive forward kinematics
-// f -> f+1 (forward_kinematics, primive_base.py:152-156)
+// f -> f+1 (forward_kinematics, primive_base.py:152-156).  A pass is stamped
+// gstamp[G_STAMP] + 1; the first group to reach a cell block (exchange of the
+// block's flag with the stamp) appends it to the occupied-block list.
 __global__ void __launch_bounds__(PPB) k_bin(BinArgs a, const float* __restrict__ st, int do_fk, const PrimC* __restrict__ prims,
                                              int n_prims_total, float* __restrict__ pstate, const float* __restrict__ pvel, int f) {
+    int stamp = a.gstamp[G_STAMP] + 1;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.counters[CNT_PBLK] = 0;
         a.counters[CNT_NBLK] = 0;
         a.counters[CNT_PART] = 0;
         a.counters[CNT_ITEMS] = 0;
-        int stamp = a.gstamp[0] + 1;
-        a.gstamp[0] = stamp;
         a.counters[CNT_STAMP] = stamp;
     }
     if (do_fk && blockIdx.x == 0 && (int)threadIdx.x < n_prims_total) {
@@ -275,50 +284,41 @@ __global__ void __launch_bounds__(PPB) k_bin(BinArgs a, const float* __restrict_
         act &= ~m;
     }
     int base = 0;
+    bool fresh = false;
     if (valid && lane == leader) {
         base = atomicAdd(&a.cell_count[q], cnt);
-        a.blk_flag[q >> 6] = 1;
+        int gb = q >> 6;
+        fresh = a.blk_flag[gb] != stamp && atomicExch(&a.blk_flag[gb], stamp) != stamp;
     }
     rank += __shfl(base, leader);
+    unsigned long long nm = __ballot(fresh);
+    if (nm) {
+        int lead = __ffsll((long long)nm) - 1;
+        int b0 = 0;
+        if (lane == lead) b0 = atomicAdd(&a.gstamp[G_PBLK], __popcll(nm));
+        b0 = __shfl(b0, lead);
+        if (fresh) a.pblocks[b0 + __popcll(nm & lanemask_lt(lane))] = q >> 6;
+    }
     if (valid) {
         a.bin_q[slot] = q;
         a.bin_rank[slot] = rank;
     }
 }
 
-// compact flagged blocks into a list (wave-aggregated atomics); clears the
-// flags; map (optional) receives each listed block's position
-__global__ void __launch_bounds__(256) k_compact(int n_blocks, int* __restrict__ flag, int* __restrict__ list,
-                                                 int* __restrict__ counter, int* __restrict__ map) {
-    for (int b0 = blockIdx.x * blockDim.x; b0 < n_blocks; b0 += gridDim.x * blockDim.x) {
-        int b = b0 + threadIdx.x;
-        int fl = (b < n_blocks) ? flag[b] : 0;
-        unsigned long long mask = __ballot(fl != 0);
-        if (mask == 0ull) continue;
-        int lane = threadIdx.x & 63;
-        int leader = __ffsll((long long)mask) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(counter, __popcll(mask));
-        base = __shfl(base, leader);
-        if (fl) {
-            int pos = base + __popcll(mask & lanemask_lt(lane));
-            list[pos] = b;
-            if (map) map[b] = pos;
-            flag[b] = 0;
-        }
-    }
-}
-
 // per occupied cell block (one wave, lane = cell): scan the cell counts, claim
-// a particle range and work items, stamp the block, mark the node blocks its
-// 6x6x6 footprint covers.  A workgroup takes 4 blocks at a time and claims
-// their ranges with one 64-bit atomic on the (particles, items) counter pair.
-__global__ void __launch_bounds__(256) k_blocks(BinArgs a) {
+// a particle range and work items, stamp the block, and append the node blocks
+// its 6x6x6 footprint covers to the active node-block list (first setter of a
+// node block's flag, as in k_bin) with their list positions in nmap.  A
+// workgroup takes 4 blocks at a time and claims their ranges with one 64-bit
+// atomic on the (particles, items) counter pair and one on the node count.
+__global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nblocks, int* __restrict__ nmap) {
     __shared__ unsigned long long s_tot[4];
     __shared__ unsigned long long s_base;
+    __shared__ int s_nn[4];
+    __shared__ int s_nbase;
     int lane = threadIdx.x & 63;
     int wave = threadIdx.x >> 6;
-    int n = a.counters[CNT_PBLK];
+    int n = a.gstamp[G_PBLK];
     int stamp = a.counters[CNT_STAMP];
     unsigned long long* pair = reinterpret_cast<unsigned long long*>(a.counters + CNT_PART);  // low: particles, high: items
     for (int e0 = blockIdx.x * 4; e0 < n; e0 += gridDim.x * 4) {
@@ -334,34 +334,56 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a) {
             if (lane >= d) incl += t;
         }
         int total = __shfl(incl, 63);
-        int nit = (total + WI - 1) / WI;
-        if (lane == 0) s_tot[wave] = ((unsigned long long)nit << 32) | (unsigned)total;
+        int nit = (total + a.wi - 1) / a.wi;
+        // node blocks of the footprint reached first by this pass
+        int nb = -1;
+        if (has && lane < 8) {
+            const SceneC& S = a.scenes[a.blk_scene[gb]];
+            int bi, bj, bk;
+            block_origin(S, gb, bi, bj, bk);
+            int ni = bi + ((lane >> 2) & 1), nj = bj + ((lane >> 1) & 1), nk = bk + (lane & 1);
+            if (ni < S.nb && nj < S.nb && nk < S.nb) nb = S.blk_off + (ni * S.nb + nj) * S.nb + nk;
+        }
+        bool fresh = nb >= 0 && a.node_flag[nb] != stamp && atomicExch(&a.node_flag[nb], stamp) != stamp;
+        unsigned long long fm = __ballot(fresh);
+        if (lane == 0) {
+            s_tot[wave] = ((unsigned long long)nit << 32) | (unsigned)total;
+            s_nn[wave] = __popcll(fm);
+        }
         __syncthreads();
-        if (threadIdx.x == 0) s_base = atomicAdd(pair, s_tot[0] + s_tot[1] + s_tot[2] + s_tot[3]);
+        if (threadIdx.x == 0) {
+            s_base = atomicAdd(pair, s_tot[0] + s_tot[1] + s_tot[2] + s_tot[3]);
+            s_nbase = atomicAdd(&a.counters[CNT_NBLK], s_nn[0] + s_nn[1] + s_nn[2] + s_nn[3]);
+        }
         __syncthreads();
         unsigned long long off = s_base;
-        for (int w = 0; w < wave; w++) off += s_tot[w];
+        int noff = s_nbase;
+        for (int w = 0; w < wave; w++) { off += s_tot[w]; noff += s_nn[w]; }
         int start = (int)(off & 0xffffffffull), ib = (int)(off >> 32);
         __syncthreads();
+        if (fresh) {
+            int pos = noff + __popcll(fm & lanemask_lt(lane));
+            nblocks[pos] = nb;
+            nmap[nb] = pos;
+        }
         if (!has) continue;
         a.cell_start[gb * 64 + lane] = start + incl - cnt;
-        for (int it = lane; it < nit; it += 64) a.items[ib + it] = make_int4(gb, start + it * WI, min(WI, total - it * WI), 0);
+        for (int it = lane; it < nit; it += 64) a.items[ib + it] = make_int4(gb, start + it * a.wi, min(a.wi, total - it * a.wi), 0);
         if (lane == 0) {
             a.blk_stamp[gb] = stamp;
             a.blk_item0[gb] = ib;
             a.blk_nitems[gb] = nit;
         }
-        if (lane < 8) {
-            const SceneC& S = a.scenes[a.blk_scene[gb]];
-            int bi, bj, bk;
-            block_origin(S, gb, bi, bj, bk);
-            int ni = bi + ((lane >> 2) & 1), nj = bj + ((lane >> 1) & 1), nk = bk + (lane & 1);
-            if (ni < S.nb && nj < S.nb && nk < S.nb) a.node_flag[S.blk_off + (ni * S.nb + nj) * S.nb + nk] = 1;
-        }
     }
 }
 
+// index scatter; the pass's last kernel also advances the stamp base and
+// resets the occupied-block count for the next pass
 __global__ void __launch_bounds__(PPB) k_scatter(BinArgs a) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.gstamp[G_STAMP] += 1;
+        a.gstamp[G_PBLK] = 0;
+    }
     int4 ch = a.chunks[blockIdx.x];
     int i = threadIdx.x;
     if (i >= ch.z) return;
@@ -545,7 +567,7 @@ DEV float4 gather_tiles(const GridArgs& ga, const SceneC& S, int B0, int B1, int
 #pragma unroll
     for (int d = 0; d < 8; d++) {
         acc.x += v[d].x; acc.y += v[d].y; acc.z += v[d].z; acc.w += v[d].w;
-        // further work items of a block with more than WI particles
+        // further work items of a block with more than wi particles
         if (tidx[d] >= 0)
             for (int j = 1; j < ni[d]; j++) {
                 float4 u = tiles[(size_t)(i0[d] + j) * TILE + tidx[d]];
@@ -1258,6 +1280,7 @@ struct mpm_ctx {
     int keep_cap = 0;          // node blocks a kept grid can hold
     float4 *tiles_pm = nullptr, *tiles_v = nullptr;
     int max_items = 0;
+    int wi = WI;          // particles per work item
     int tile_ctas = 1024; // persistent work-item workgroups
     float *pstate = nullptr, *pstate_g = nullptr, *pvel = nullptr, *pvel_g = nullptr;
     int valid_hi = -1;    // frames > valid_hi are stale
@@ -1266,6 +1289,11 @@ struct mpm_ctx {
     int seg_hi = -1;      // highest resident ring frame of that segment
     int adj_top = -1;     // frame whose adjoint is current
     int grid_ctas = 1024; // persistent grid-phase workgroups
+    // the constitutive update of a forward substep does not depend on its
+    // binning: it runs on a second stream (default priority) between an
+    // event fork and join, so the two overlap (parallel branches in a graph)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // per-kernel timing with HIP events on the launch stream (mpm_set_profiling)
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -1292,6 +1320,7 @@ struct mpm_ctx {
         a.cell_count = cell_count; a.cell_start = cell_start; a.bin_q = bin_q; a.bin_rank = bin_rank;
         a.blk_flag = blk_flag; a.node_flag = node_flag; a.counters = b.counters; a.gstamp = gstamp; a.pblocks = pblocks;
         a.blk_stamp = b.blk_stamp; a.blk_item0 = b.blk_item0; a.blk_nitems = b.blk_nitems; a.items = b.items; a.order = b.order;
+        a.wi = wi;
         return a;
     }
     TileArgs targs(int bs) const {
@@ -1386,8 +1415,9 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     L.off_blkints = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 3);   // block flags, node flags, occupied list
     L.off_cells = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 64 * 2); // cell counts, cell starts
     L.off_pints = o; o = align256(o + sizeof(int) * (size_t)stride * 2);         // bin key, bin rank
-    // work items: at most one partial item per occupied cell block plus stride / WI full ones
-    L.max_items = stride / WI + std::min(L.n_blocks, stride) + 1;
+    // work items: at most one partial item per occupied cell block plus
+    // stride / wi full ones, for the smallest allowed wi
+    L.max_items = stride / WI_MIN + std::min(L.n_blocks, stride) + 1;
     // default kept-grid capacity: generous for particle clouds of at least a few
     // hundred particles per occupied cell block; a pass whose active node-block
     // list does not fit falls back to recomputing its grid in the adjoint
@@ -1430,17 +1460,10 @@ int run_bin(mpm_ctx* c, int f, int do_fk, hipStream_t st) {
     int bs = c->bslot_of(f);
     const mpm_ctx::BinSlot& B = c->bslots[bs];
     BinArgs ba = c->bargs(bs);
-    int cblocks = std::min(4096, (c->n_blocks + 255) / 256);
     { PhaseTimer t(c, MPM_PH_BIN, st);
       hipLaunchKernelGGL(k_bin, dim3(c->n_chunks), dim3(PPB), 0, st, ba, in, do_fk, c->d_prims, c->n_prims, c->pstate, c->pvel, f); }
-    { PhaseTimer t(c, MPM_PH_COMPACT, st);
-      hipLaunchKernelGGL(k_compact, dim3(cblocks), dim3(256), 0, st, c->n_blocks, c->blk_flag, c->pblocks, B.counters + CNT_PBLK,
-                         (int*)nullptr); }
     { PhaseTimer t(c, MPM_PH_BLOCKS, st);
-      hipLaunchKernelGGL(k_blocks, dim3(c->grid_ctas), dim3(256), 0, st, ba); }
-    { PhaseTimer t(c, MPM_PH_COMPACT, st);
-      hipLaunchKernelGGL(k_compact, dim3(cblocks), dim3(256), 0, st, c->n_blocks, c->node_flag, B.nblocks, B.counters + CNT_NBLK,
-                         B.nmap); }
+      hipLaunchKernelGGL(k_blocks, dim3(c->grid_ctas), dim3(256), 0, st, ba, B.nblocks, B.nmap); }
     { PhaseTimer t(c, MPM_PH_SCATTER, st);
       hipLaunchKernelGGL(k_scatter, dim3(c->n_chunks), dim3(PPB), 0, st, ba); }
     HIPCHK(hipGetLastError());
@@ -1451,13 +1474,23 @@ int run_bin(mpm_ctx* c, int f, int do_fk, hipStream_t st) {
 int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
     const float* in = c->frame_ptr(f);
     float* out = c->frame_ptr(f + 1);
-    if (run_bin(c, f, 1, st)) return -1;
     int bs = c->bslot_of(f);
     TileArgs ta = c->targs(bs);
     float* aff = c->aff_ptr(f);
-    { PhaseTimer t(c, MPM_PH_STRESS, st);
-      hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, out, aff); }
+    // profiling serialises the two branches so that every kernel's event
+    // interval is its own
+    bool fork = !c->prof;
+    hipStream_t sst = fork ? c->side : st;
+    if (fork) {
+        HIPCHK(hipEventRecord(c->ev_fork, st));
+        HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    }
+    { PhaseTimer t(c, MPM_PH_STRESS, sst);
+      hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, sst, c->cargs(), in, out, aff); }
+    if (fork) HIPCHK(hipEventRecord(c->ev_join, c->side));
     c->aff_of[c->slot_of(f)] = f;
+    if (run_bin(c, f, 1, st)) return -1;
+    if (fork) HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
     { PhaseTimer t(c, MPM_PH_P2G, st);
       hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm, 0); }
     { PhaseTimer t(c, MPM_PH_GRID, st);
@@ -1511,11 +1544,14 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     // rebuild the grid when it was not kept; a kept grid whose active list
     // overflowed the capacity is rebuilt on the device (only_overflow launches)
     if (!kept || c->keep_cap < c->n_blocks) {
+        // a kept grid overflows rarely: the device-side fallback launches
+        // small persistent grids so that their no-op case dispatches quickly
         int only = kept ? 1 : 0;
+        int tctas = only ? 64 : c->tile_ctas, gctas = only ? 64 : c->grid_ctas;
         { PhaseTimer t(c, MPM_PH_P2G_GRID, st);
-          hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm, only); }
+          hipLaunchKernelGGL(k_p2g_scatter, dim3(tctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm, only); }
         { PhaseTimer t(c, MPM_PH_GRID_KEEP, st);
-          hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->tiles_pm, c->g_pm, c->g_v, only); }
+          hipLaunchKernelGGL(k_grid_op, dim3(gctas), dim3(256), 0, st, ga, c->tiles_pm, c->g_pm, c->g_v, only); }
     }
     int use_kept = kept ? 1 : 0;
     { PhaseTimer t(c, MPM_PH_G2P_BWD, st);
@@ -1723,15 +1759,47 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     }
     c->grid_ctas = std::max(64, ncu * 8);
     c->tile_ctas = std::max(64, ncu * 4);
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        mpm_destroy(c);
+        return fail("mpm_create: stream/event creation failed");
+    }
     *out = c;
     return 0;
 }
 
 int mpm_destroy(mpm_ctx* ctx) {
     if (!ctx) return 0;
+    if (ctx->side) { hipStreamSynchronize(ctx->side); hipStreamDestroy(ctx->side); }
+    if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
     for (auto& r : ctx->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
     for (auto e : ctx->ev_pool) hipEventDestroy(e);
     delete ctx;
+    return 0;
+}
+
+int mpm_set_launch(mpm_ctx* ctx, int particles_per_item, int tile_ctas, int grid_ctas) {
+    if (!ctx) return fail("mpm_set_launch: null context");
+    if (particles_per_item != 0 &&
+        (particles_per_item < WI_MIN || particles_per_item > WI_MAX || particles_per_item % WG != 0))
+        return fail("mpm_set_launch: particles_per_item must be a multiple of " + std::to_string(WG) + " in [" +
+                    std::to_string(WI_MIN) + ", " + std::to_string(WI_MAX) + "]");
+    if (tile_ctas < 0 || tile_ctas > 65536 || grid_ctas < 0 || grid_ctas > 65536)
+        return fail("mpm_set_launch: workgroup counts must be in [1, 65536] (0 keeps the current value)");
+    if (particles_per_item) ctx->wi = particles_per_item;
+    if (tile_ctas) ctx->tile_ctas = tile_ctas;
+    if (grid_ctas) ctx->grid_ctas = grid_ctas;
+    ctx->drop_kept();  // kept binnings were made with the previous item size
+    return 0;
+}
+
+int mpm_get_launch(mpm_ctx* ctx, int* particles_per_item, int* tile_ctas, int* grid_ctas) {
+    if (!ctx) return fail("mpm_get_launch: null context");
+    if (particles_per_item) *particles_per_item = ctx->wi;
+    if (tile_ctas) *tile_ctas = ctx->tile_ctas;
+    if (grid_ctas) *grid_ctas = ctx->grid_ctas;
     return 0;
 }
 

