@@ -129,6 +129,8 @@ def main():
                     help="node blocks of each substep's grid kept for the adjoint (-1 default, 0 rebuild every grid)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--launch", default=None,
+                    help="PPI,TILE,GRID: particles per work item and workgroup counts (skips the autotuner)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
@@ -163,7 +165,9 @@ def main():
     opt = BatchedTrajOpt(batch, H, S, targets, lr=1e-2, init_actions=init, use_graph=not args.no_graph)
     n_total = batch.total_particles
     launch = None
-    if not args.no_tune:
+    if args.launch:
+        batch.engine.set_launch(*[int(v) for v in args.launch.split(",")])
+    elif not args.no_tune:
         launch, tune_ms = opt.autotune()
         log("autotuned launch shape %s (%d candidates, %.1f-%.1f ms per eager pass)"
             % (launch, len(tune_ms), min(tune_ms.values()), max(tune_ms.values())))
