@@ -47,7 +47,7 @@ EXPORTS = [
     "mpm_set_prim_frame", "mpm_get_prim_frame", "mpm_set_actions", "mpm_substep_fwd", "mpm_substep_bwd",
     "mpm_grid_mass", "mpm_grid_mass_bwd", "mpm_add_obs_grad", "mpm_action_grad", "mpm_clear_grads",
     "mpm_update_prims", "mpm_copy_frame", "mpm_set_profiling", "mpm_profile_read", "mpm_last_error",
-    "mpm_set_launch", "mpm_get_launch",
+    "mpm_set_launch", "mpm_get_launch", "mpm_min_dist_cols", "mpm_min_dist", "mpm_min_dist_bwd",
 ]
 
 PHASES = ["bin", "compact", "blocks", "scatter", "stress", "p2g", "grid_op", "g2p", "p2g_grid", "grid_op_keep",
@@ -93,6 +93,9 @@ def load():
         "mpm_copy_frame": (i32, [vp, i32, i32, vp]),
         "mpm_set_profiling": (i32, [vp, i32]),
         "mpm_set_launch": (i32, [vp, i32, i32, i32]),
+        "mpm_min_dist_cols": (i32, [vp, i32]),
+        "mpm_min_dist": (i32, [vp, i32, vp, i32, vp]),
+        "mpm_min_dist_bwd": (i32, [vp, i32, vp, i32, vp]),
         "mpm_get_launch": (i32, [vp, vp, vp, vp]),
         "mpm_profile_read": (i32, [vp, vp, vp, i32]),
         "mpm_last_error": (ctypes.c_char_p, []),

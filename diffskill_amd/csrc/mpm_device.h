@@ -606,6 +606,23 @@ DEV void unit_sdf_rev(const PrimC& P, const Unit& u, const PS& s, v3 x, float gd
     if (u.sh == SH_CHOP) g->gap += ggap * gd;
     to_local_rev(s, u, x, gd * gp, g);
 }
+// adjoint of unit_sdf w.r.t. both the primitive state (into g) and the world
+// point x (returned): x enters as x - centre, so its adjoint is the offset's
+DEV v3 unit_sdf_rev_x(const PrimC& P, const Unit& u, const PS& s, v3 x, float gd, PSg* g) {
+    v3 p = to_local(s, u, x), gp;
+    float ggap;
+    local_sdf(P, u.sh, s.gap, p, &gp, &ggap);
+    if (u.sh == SH_CHOP) g->gap += ggap * gd;
+    v3 gl = gd * gp;
+    v3 gx;
+    if (u.norot) gx = gl;
+    else {
+        q4 giq = mkq(0.f, 0.f, 0.f, 0.f);
+        gx = qrot_rev(conj_n(s.rot), x - unit_center(s, u.jaw), gl, &giq);
+    }
+    to_local_rev(s, u, x, gl, g);
+    return gx;
+}
 // world normal qrot(rot, _normal(local)) (primive_base.py:81-85)
 DEV v3 unit_normal(const PrimC& P, const Unit& u, const PS& s, v3 x) {
     v3 n = local_normal(P, u.sh, s.gap, to_local(s, u, x));

@@ -1190,6 +1190,68 @@ __global__ void __launch_bounds__(PPB) k_grid_mass_bwd(ChunkArgs ca, const float
 }
 
 // reset_kernel (mpm_simulator.py:402-410)
+// compute_min_dist (function.py:79-88): distance of every particle of frame f
+// to every collision unit of its scene's primitives (both jaws of a two-jaw
+// gripper), out[col][stride]; columns past a scene's own count are untouched
+__global__ void __launch_bounds__(PPB) k_min_dist(ChunkArgs ca, const float* __restrict__ st, const PrimC* __restrict__ prims,
+                                                  const float* __restrict__ ps, float* __restrict__ out) {
+    int4 ch = ca.chunks[blockIdx.x];
+    int i = threadIdx.x;
+    if (i >= ch.z) return;
+    const SceneC& S = ca.scenes[ch.x];
+    int slot = ch.y + i;
+    v3 x = load3(st, ca.stride, 0, slot);
+    int col = 0;
+    for (int q = 0; q < S.n_prims; q++) {
+        const PrimC& P = prims[S.prim_off + q];
+        PS s = load_ps(ps + (size_t)(S.prim_off + q) * 8);
+        Unit u[2];
+        int m = prim_units(P, u);
+        for (int j = 0; j < m; j++, col++) out[(size_t)col * ca.stride + slot] = unit_sdf(P, u[j], s, x);
+    }
+}
+
+// compute_min_dist.grad (function.py:154-160): gout[col][stride] adjoints of
+// the distances; adds d/dx into the particle adjoint (x rows) and the
+// primitive-state adjoint of frame f (wave-reduced, one atomic per wave)
+__global__ void __launch_bounds__(PPB) k_min_dist_bwd(ChunkArgs ca, const float* __restrict__ st, const PrimC* __restrict__ prims,
+                                                      const float* __restrict__ ps, const float* __restrict__ gout,
+                                                      float* __restrict__ adj, float* __restrict__ gps) {
+    int4 ch = ca.chunks[blockIdx.x];
+    int i = threadIdx.x;
+    int lane = i & 63;
+    bool valid = i < ch.z;
+    const SceneC& S = ca.scenes[ch.x];
+    int slot = ch.y + (valid ? i : 0);
+    v3 x = load3(st, ca.stride, 0, slot);
+    v3 gx = mk(0.f, 0.f, 0.f);
+    int col = 0;
+    for (int q = 0; q < S.n_prims; q++) {
+        const PrimC& P = prims[S.prim_off + q];
+        PS s = load_ps(ps + (size_t)(S.prim_off + q) * 8);
+        Unit u[2];
+        int m = prim_units(P, u);
+        PSg g = zero_psg();
+        for (int j = 0; j < m; j++, col++) {
+            float gd = valid ? gout[(size_t)col * ca.stride + slot] : 0.f;
+            if (gd != 0.f) gx += unit_sdf_rev_x(P, u[j], s, x, gd, &g);
+        }
+        float vals[8] = {g.pos.x, g.pos.y, g.pos.z, g.rot.w, g.rot.x, g.rot.y, g.rot.z, g.gap};
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < 8; k++) any |= (vals[k] != 0.f);
+        if (__ballot(any) == 0ull) continue;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float v = vals[k];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (lane == 0 && v != 0.f) unsafeAtomicAdd(gps + (size_t)(S.prim_off + q) * 8 + k, v);
+        }
+    }
+    if (valid) store3(adj, ca.stride, 0, slot, load3(adj, ca.stride, 0, slot) + gx);
+}
+
 __global__ void k_reset_frame0(int stride, const float* __restrict__ x, float* __restrict__ st) {
     int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= stride) return;
@@ -1984,6 +2046,53 @@ int mpm_add_obs_grad(mpm_ctx* ctx, int f, const float* xv_grad, const float* pri
     int n = std::max(ctx->stride, ctx->n_prims * 8);
     hipLaunchKernelGGL(k_obs_grad, dim3((n + 255) / 256), dim3(256), 0, S_(stream), ctx->stride, ctx->n_prims, xv_grad, prim_grad,
                        ctx->adj_ptr(f), ctx->pstate_g + (size_t)f * ctx->n_prims * 8);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mpm_min_dist_cols(mpm_ctx* ctx, int scene) {
+    if (!ctx) return fail("mpm_min_dist_cols: null context");
+    int lo = 0, hi = ctx->n_scenes;
+    if (scene >= 0) {
+        if (scene >= ctx->n_scenes) return fail("mpm_min_dist_cols: bad scene");
+        lo = scene; hi = scene + 1;
+    }
+    int best = 0;
+    for (int k = lo; k < hi; k++) {
+        const SceneC& S = ctx->scenes[k];
+        int c = 0;
+        for (int q = 0; q < S.n_prims; q++) {
+            int t = ctx->prims[S.prim_off + q].type;
+            c += (t == MPM_GRIPPER || t == MPM_GRIPPER2) ? 2 : 1;
+        }
+        best = std::max(best, c);
+    }
+    return best;
+}
+
+int mpm_min_dist(mpm_ctx* ctx, int f, float* out, int n_cols, void* stream) {
+    if (!ctx || !out) return fail("mpm_min_dist: null argument");
+    if (n_cols < mpm_min_dist_cols(ctx, -1)) return fail("mpm_min_dist: output has fewer columns than the scenes need");
+    hipStream_t st = S_(stream);
+    if (ensure_frame(ctx, f, st)) return -1;
+    if (ctx->n_prims == 0) return 0;
+    hipLaunchKernelGGL(k_min_dist, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), ctx->d_prims,
+                       ctx->pstate + (size_t)f * ctx->n_prims * 8, out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mpm_min_dist_bwd(mpm_ctx* ctx, int f, const float* gout, int n_cols, void* stream) {
+    if (!ctx || !gout) return fail("mpm_min_dist_bwd: null argument");
+    if (n_cols < mpm_min_dist_cols(ctx, -1)) return fail("mpm_min_dist_bwd: adjoint has fewer columns than the scenes need");
+    if (ctx->adj_top < 0) ctx->adj_top = f;
+    if (f != ctx->adj_top) return fail("mpm_min_dist_bwd: frame is not the current adjoint frame");
+    hipStream_t st = S_(stream);
+    if (ensure_frame(ctx, f, st)) return -1;
+    if (ctx->n_prims == 0) return 0;
+    hipLaunchKernelGGL(k_min_dist_bwd, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), ctx->d_prims,
+                       ctx->pstate + (size_t)f * ctx->n_prims * 8, gout, ctx->adj_ptr(f),
+                       ctx->pstate_g + (size_t)f * ctx->n_prims * 8);
     HIPCHK(hipGetLastError());
     return 0;
 }

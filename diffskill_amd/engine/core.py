@@ -154,6 +154,25 @@ class Engine:
         b = ctypes.c_void_p(prim_grad.data_ptr()) if prim_grad is not None else None
         _lib.check(self.L.mpm_add_obs_grad(self.ctx, int(f), a, b, _stream()), "mpm_add_obs_grad")
 
+    def min_dist_cols(self, scene=-1):
+        n = self.L.mpm_min_dist_cols(self.ctx, int(scene))
+        if n < 0:
+            _lib.check(n, "mpm_min_dist_cols")
+        return n
+
+    def min_dist(self, f):
+        """compute_min_dist of frame f: (n_cols, stride) float32, slot order."""
+        nc = max(1, self.min_dist_cols())
+        out = torch.zeros((nc, self.stride), dtype=torch.float32, device="cuda")
+        _lib.check(self.L.mpm_min_dist(self.ctx, int(f), ctypes.c_void_p(out.data_ptr()), nc, _stream()), "mpm_min_dist")
+        return out
+
+    def min_dist_bwd(self, f, g_slots):
+        """Adds the adjoint of the distances (n_cols, stride) into frame f's adjoints."""
+        g = g_slots.detach().to("cuda", torch.float32).contiguous()
+        _lib.check(self.L.mpm_min_dist_bwd(self.ctx, int(f), ctypes.c_void_p(g.data_ptr()), int(g.shape[0]), _stream()),
+                   "mpm_min_dist_bwd")
+
     def action_grad(self, s, n_substeps):
         out = torch.zeros((max(1, self.n_prims), 7), dtype=torch.float32, device="cuda")
         _lib.check(self.L.mpm_action_grad(self.ctx, int(s), int(n_substeps), ctypes.c_void_p(out.data_ptr()), _stream()),

@@ -23,9 +23,14 @@ class GradModel:
         self.softness = softness
         self._forward_func = None
         self.output_grid = output_grid
-        if return_dist:
-            raise NotImplementedError("return_dist (per-particle tool distances) is not implemented by the MI355X engine")
-        self.return_dist = False
+        self.return_dist = bool(return_dist)
+        if self.return_dist:
+            # one column per primitive, two (sdf_2 with flag -1, +1) for the
+            # 8-state ones (function.py:29-37,79-88); Chopsticks has no sdf_2
+            for p in self.primitives:
+                if p.state_dim == 8 and p.shape == "Chopsticks":
+                    raise NotImplementedError("return_dist with Chopsticks: the reference calls sdf_2, which Chopsticks "
+                                              "does not define")
         self.init_state = self.env.get_state()["state"]
         if init_sampler is None:
             init_sampler = lambda: self.init_state  # noqa: E731
@@ -52,6 +57,8 @@ class GradModel:
             for idx, p in enumerate(self.primitives):
                 if p.state_dim != 8:
                     c[idx, 7] = 0
+        if self.return_dist:
+            x = torch.cat((x, self.sim.compute_min_dist(f).to(device)), 1)
         outputs = x.clone(), c.clone()
         if len(self.output_grid) > 0:
             for _ in self.output_grid:
@@ -65,6 +72,10 @@ class GradModel:
         f = s * self.sim.substeps
         if len(self.output_grid) > 0 and len(args) > 0 and args[0] is not None:
             self.sim.compute_grid_m_grad(f, args[0].detach().to("cuda", torch.float32).contiguous())
+        if self.return_dist and obs_grad is not None:
+            n_d = obs_grad.shape[-1] - self.dim * 2
+            self.sim.compute_min_dist_grad(f, obs_grad[..., self.dim * 2:].reshape(-1, n_d))
+            obs_grad = obs_grad[..., :self.dim * 2]
         obs_grad = obs_grad.reshape(-1, self.dim * 2) if obs_grad is not None else None
         mg = None
         if manipulator_grad is not None and len(self.primitives) > 0:

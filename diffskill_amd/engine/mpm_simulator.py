@@ -279,6 +279,21 @@ class MPMSimulator:
             pg = prim_grad.detach().to("cuda", torch.float32).reshape(self.n_primitive, 8).contiguous()
         self.engine.add_obs_grad(f, xs, pg)
 
+    def compute_min_dist(self, f):
+        """compute_min_dist (function.py:79-88): (N, n_cols) distances of the
+        particles of frame f to every primitive (both jaws of a Gripper or
+        Gripper2), caller order."""
+        d = self.engine.min_dist(f)[:, self._cols()]
+        return self._to_caller(d.t()).contiguous()
+
+    def compute_min_dist_grad(self, f, grad):
+        """compute_min_dist.grad (function.py:154-160): grad (N, n_cols) adds into
+        the adjoints of frame f (particle x and primitive states)."""
+        g = self._to_slots(grad.detach().to("cuda", torch.float32).reshape(self.n_particles[None], -1))
+        gs = torch.zeros((g.shape[1], self.engine.stride), dtype=torch.float32, device="cuda")
+        gs[:, self._cols()] = g.t()
+        self.engine.min_dist_bwd(f, gs)
+
     def get_adjoint(self, f):
         """(x, v, F, C) adjoints of frame f as (N,3), (N,3), (N,3,3), (N,3,3) tensors."""
         a = self._to_caller(self.engine.adjoint(f)[:, self._cols()].t())

@@ -40,8 +40,7 @@ def sample_shapes(shapes, seed=None):
 
 class TaichiEnv:
     def __init__(self, cfg, nn=False, loss=True, return_dist=False, init_particles=None, ckpt_k=None):
-        if return_dist:
-            raise NotImplementedError("return_dist is not implemented by the MI355X engine")
+        self.return_dist = bool(return_dist)
         full = get_cfg_defaults()
         full.merge(cfg)
         self.full_cfg = full
@@ -92,11 +91,26 @@ class TaichiEnv:
         self.simulator.set_primitive_state(0, state)
 
     def get_obs(self, s, device):
-        """taichi_env.py:356-372 (positions only, plus tool states)."""
+        """taichi_env.py:356-372 (positions only, plus tool states); with
+        return_dist one distance column per primitive (taichi_env.py:85-98,
+        374-383): the sum of both jaw distances for a Gripper, the primitive's
+        SDF otherwise (for Gripper2 the nearer jaw, Gripper2.sdf)."""
         f = s * self.simulator.substeps
         x = self.simulator.get_torch_x(f, device)
         c = torch.zeros((len(self.primitives), 8), device=device)
         for idx, p in enumerate(self.primitives):
             st = p.get_state(f)
             c[idx, :len(st)] = torch.as_tensor(st, dtype=torch.float32)
+        if self.return_dist:
+            d = self.simulator.compute_min_dist(f).to(device)
+            cols, k = [], 0
+            for p in self.primitives:
+                if p.shape in ("Gripper", "Gripper2"):
+                    pair = d[:, k:k + 2]
+                    cols.append(pair.sum(1, keepdim=True) if p.shape == "Gripper" else pair.min(1, keepdim=True).values)
+                    k += 2
+                else:
+                    cols.append(d[:, k:k + 1])
+                    k += 1
+            x = torch.cat([x] + cols, 1)
         return x.clone(), c.clone()

@@ -155,6 +155,19 @@ int mpm_update_prims(mpm_ctx *ctx, const mpm_prim_desc *prims, void *stream);
  * MPMSimulator.step in copy mode (mpm_simulator.py:449-451). */
 int mpm_copy_frame(mpm_ctx *ctx, int src, int dst, void *stream);
 
+/* compute_min_dist (plb/engine/function.py:79-88): signed distance of every
+ * particle of frame f to every primitive of its scene -- Primitive.sdf, or
+ * for Gripper / Gripper2 both jaws (sdf_2 with flag -1, then +1) -- in
+ * primitive order, into out[col][stride] (device, n_cols rows; columns past a
+ * scene's own count are left untouched).  mpm_min_dist_cols gives the column
+ * count of one scene, or the largest over all scenes for scene < 0. */
+int mpm_min_dist_cols(mpm_ctx *ctx, int scene);
+int mpm_min_dist(mpm_ctx *ctx, int f, float *out, int n_cols, void *stream);
+/* compute_min_dist.grad (function.py:154-160): gout[col][stride] adjoints of
+ * the distances; adds their d/dx into the adjoint of frame f (which must be
+ * the current adjoint frame) and the primitive-state adjoint of frame f. */
+int mpm_min_dist_bwd(mpm_ctx *ctx, int f, const float *gout, int n_cols, void *stream);
+
 /* Per-kernel timing with HIP events recorded on the launch stream around
  * every engine kernel (no counterpart in the reference; measurement only).
  * Phases (MPM_PH_COMPACT is kept for numbering and no longer launched: the

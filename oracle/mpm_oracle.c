@@ -1426,6 +1426,61 @@ void orc_prim_normal(const orc_prim *P, const real *ps, int npts, const real *x,
         } else unit_normal(&u[0], ps, x + 3 * i, out + 3 * i);
     }
 }
+/* compute_min_dist (function.py:79-88): per particle, the SDF of every
+ * primitive (Primitive.sdf), or for the two-jaw grippers both jaw SDFs
+ * (sdf_2 with flag -1 then +1), columns in primitive order:
+ * out[col * np + p].  Returns the number of columns. */
+int orc_min_dist_cols(const orc_prim *prims, int n_prims) {
+    int c = 0;
+    unit_t u[2];
+    for (int q = 0; q < n_prims; q++) c += prim_units(&prims[q], u);
+    return c;
+}
+void orc_min_dist(const orc_prim *prims, int n_prims, int np, const real *st, const real *ps, real *out) {
+    int col = 0;
+    for (int q = 0; q < n_prims; q++) {
+        unit_t u[2];
+        int m = prim_units(&prims[q], u);
+        for (int j = 0; j < m; j++, col++)
+            for (int p = 0; p < np; p++) out[(size_t)col * np + p] = unit_sdf(&u[j], ps + q * NPS, st + (size_t)p * NST);
+    }
+}
+/* compute_min_dist.grad (function.py:154-160): gout[col * np + p] is the
+ * adjoint of each distance; adds d/dx into adj (n x 24, x rows) and the
+ * primitive-state adjoint into gps (n_prims x 8). */
+void orc_min_dist_grad(const orc_prim *prims, int n_prims, int np, const real *st, const real *ps, const real *gout, real *adj,
+                       real *gps) {
+    int col = 0;
+    for (int q = 0; q < n_prims; q++) {
+        unit_t u[2];
+        int m = prim_units(&prims[q], u);
+        for (int j = 0; j < m; j++, col++)
+            for (int p = 0; p < np; p++) {
+                real gd = gout[(size_t)col * np + p];
+                if (gd == 0) continue;
+                const real *x = st + (size_t)p * NST;
+                const real *s = ps + q * NPS;
+                real lp[3], g[3], ggap;
+                to_local(s, u[j].jaw, u[j].norot, x, lp);
+                local_sdf(u[j].P, u[j].sh, s[7], lp, g, &ggap);
+                for (int i = 0; i < 3; i++) g[i] *= gd;
+                if (u[j].sh == SH_CHOP) gps[q * NPS + 7] += ggap * gd;
+                /* x enters as x - centre: its adjoint is that of the offset */
+                real c[3], d[3], gx[3] = {0, 0, 0};
+                unit_center(s, u[j].jaw, c);
+                for (int i = 0; i < 3; i++) d[i] = x[i] - c[i];
+                if (u[j].norot) memcpy(gx, g, sizeof(gx));
+                else {
+                    real iq[4], giq[4] = {0, 0, 0, 0};
+                    conj_normalized(s + 3, iq);
+                    qrot_rev(iq, d, g, giq, gx);
+                }
+                for (int i = 0; i < 3; i++) adj[(size_t)p * NST + i] += gx[i];
+                to_local_rev(s, u[j].jaw, u[j].norot, x, g, gps + q * NPS);
+            }
+    }
+}
+
 /* one node of grid_op, for unit tests of the boundary / collision logic */
 void orc_grid_node(const orc_sim *S, const orc_prim *prims, const real *ps0, const real *ps1, const int *I,
                    const real *pm, real *vout) {

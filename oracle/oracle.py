@@ -293,6 +293,30 @@ class Oracle:
         self.L.orc_set_velocity(ctypes.byref(self.P[j]), _ptr(a), ctypes.c_int(n_substeps), _ptr(pv))
         return pv
 
+    def min_dist(self, st, ps):
+        """compute_min_dist: (n_cols, n) distances of every particle to every
+        primitive (both jaws of a two-jaw gripper)."""
+        st = self.arr(st)
+        k = max(1, self.np_)
+        ps = self.arr(ps, (k, 8))
+        nc = self.L.orc_min_dist_cols(self.P, ctypes.c_int(self.np_))
+        out = np.zeros((nc, st.shape[0]), self.dtype)
+        self.L.orc_min_dist(self.P, ctypes.c_int(self.np_), ctypes.c_int(st.shape[0]), _ptr(st), _ptr(ps), _ptr(out))
+        return out
+
+    def min_dist_grad(self, st, ps, gout):
+        """Adjoint of min_dist: (particle adjoint (n, 24) with the x rows set,
+        primitive-state adjoint (n_prims, 8))."""
+        st = self.arr(st)
+        k = max(1, self.np_)
+        ps = self.arr(ps, (k, 8))
+        gout = self.arr(gout)
+        adj = np.zeros_like(st)
+        gps = np.zeros((k, 8), self.dtype)
+        self.L.orc_min_dist_grad(self.P, ctypes.c_int(self.np_), ctypes.c_int(st.shape[0]), _ptr(st), _ptr(ps), _ptr(gout),
+                                 _ptr(adj), _ptr(gps))
+        return adj, gps
+
     def prim_sdf(self, j, ps, x):
         x = self.arr(x, (-1, 3))
         ps = self.arr(ps, (8,))

@@ -72,3 +72,84 @@ def test_primitive_forward_and_action_grads(shape):
     print("%s: loss %.8g vs %.8g, grad rel %.3g" % (shape, loss, loss_ref, err))
     assert abs(loss - loss_ref) < 1e-5 * abs(loss_ref)
     assert err < 1e-3, (g, g_ref)
+
+
+@pytest.mark.parametrize("shape", sorted(s for s in SHAPES if s != "Chopsticks"))
+def test_return_dist_observations_and_gradients(shape):
+    """GradModel(return_dist=True): the per-particle tool distances appended
+    to the observation (function.py:79-88,104-116) and their adjoint
+    (function.py:154-160), injected at every observed step, against the fp64
+    oracle."""
+    import torch
+    from diffskill_amd.engine import GradModel, TaichiEnv
+    from scenes import state_list
+    print("case return_dist %s" % shape, flush=True)
+    sim_cfg, prims, st = scene(shape)
+    A = prims[0]["action"]["dim"]
+    acts = np.random.default_rng(3).uniform(-0.5, 0.5, (2, A))
+    env = TaichiEnv(dict(SIMULATOR=sim_cfg, PRIMITIVES=prims, SHAPES=[]), init_particles=st[:, :3])
+    env.initialize()
+    env.simulator.set_state(0, state_list(st, [prim_state0(p) for p in prims]))
+    env.primitives.set_softness(666.0)
+    gm = GradModel(env, softness=666.0, output_grid=(1,), return_dist=True)
+    obs = gm.reset()
+    nd = obs[0].shape[1] - 6
+    assert nd == (2 if shape in ("Gripper", "Gripper2") else 1)
+    w = [torch.as_tensor(np.random.default_rng(10 + s).normal(size=(2000, nd)), dtype=torch.float32, device="cuda")
+         for s in range(2)]
+    a = torch.tensor(acts, dtype=torch.float32, device="cuda", requires_grad=True)
+    loss = 0
+    d_gpu = []
+    for s in range(2):
+        obs = gm.forward(s, a[s], *obs)
+        d_gpu.append(obs[0][:, 6:].detach().cpu().double().numpy())
+        loss = loss + (w[s] * obs[0][:, 6:]).sum()
+    loss = loss + 1e-3 * obs[2].sum()
+    loss.backward()
+    g = a.grad.detach().cpu().double().numpy()
+
+    osim, c = oracle_sim(sim_cfg)
+    o = Oracle(osim, [oracle_prim(p) for p in prims])
+    S = c["substeps"]
+    ro = OracleRollout(o, S)
+    frames, ps = ro.run(st, prim_state0(prims[0])[None], acts)
+    frame_adj, prim_adj = {}, {}
+    for s in range(2):
+        f = (s + 1) * S
+        d_ref = o.min_dist(frames[f], ps[f]).T
+        np.testing.assert_allclose(d_gpu[s], d_ref, rtol=0, atol=1e-6)
+        adj, gps = o.min_dist_grad(frames[f], ps[f], w[s].cpu().double().numpy().T)
+        frame_adj[f] = adj
+        prim_adj[f] = gps
+    T = 2 * S
+    frame_adj[T] = frame_adj[T] + o.grid_mass_grad(frames[T], np.full((32, 32, 32), 1e-3))
+    ga, _, _ = ro.grad(frame_adj, prim_adj)
+    err = np.abs(g - ga).max() / np.abs(ga).max()
+    print("%s: return_dist grad rel %.3g" % (shape, err))
+    assert err < 1e-3, (g, ga)
+
+
+def test_taichi_env_return_dist_columns():
+    """TaichiEnv(return_dist=True).get_obs: positions plus one distance column
+    per primitive (taichi_env.py:85-98,356-383): both jaws summed for a
+    Gripper, Gripper2.sdf (nearer jaw) and Sphere.sdf otherwise."""
+    from diffskill_amd.engine import TaichiEnv
+    from scenes import state_list
+    sim_cfg, _, st = scene("Sphere")
+    prims = []
+    for name, pos in (("Gripper", (0.5, 0.2, 0.5)), ("Gripper2", (0.45, 0.16, 0.55)), ("Sphere", (0.55, 0.19, 0.45))):
+        cfg = dict(SHAPES[name])
+        cfg.update(init_pos=pos, init_rot=(1.0, 0.0, 0.0, 0.0), friction=0.9)
+        prims.append(cfg)
+    env = TaichiEnv(dict(SIMULATOR=sim_cfg, PRIMITIVES=prims, SHAPES=[]), init_particles=st[:, :3], return_dist=True)
+    env.initialize()
+    ps0 = [prim_state0(p) for p in prims]
+    env.simulator.set_state(0, state_list(st, ps0))
+    x, c = env.get_obs(0, "cuda")
+    assert x.shape == (2000, 6)
+    osim, _ = oracle_sim(sim_cfg)
+    o = Oracle(osim, [oracle_prim(p) for p in prims])
+    d = o.min_dist(st, np.stack(ps0))          # Gripper jaws, Gripper2 jaws, Sphere
+    ref = np.stack([d[0] + d[1], np.minimum(d[2], d[3]), d[4]], 1)
+    np.testing.assert_allclose(x[:, 3:].cpu().double().numpy(), ref, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(x[:, :3].cpu().double().numpy(), st[:, :3], rtol=0, atol=1e-7)

@@ -382,3 +382,43 @@ def test_adjoint_state_gradient_matches_central_differences():
         s2[p, c] -= 1e-6
         d = (loss(s1)[0] - loss(s2)[0]) / 2e-6
         assert abs(adj0[p, c] - d) < 1e-5 * max(1.0, abs(d)), (p, c, adj0[p, c], d)
+
+
+@pytest.mark.parametrize("shape", sorted(FD_SHAPES))
+def test_min_dist_matches_sdf_and_its_adjoint_matches_central_differences(shape):
+    """compute_min_dist (function.py:79-88): one column per primitive (both
+    jaws for the two-jaw gripper, whose minimum is Gripper.sdf); its adjoint
+    (function.py:154-160) w.r.t. particle positions and the primitive state
+    equals central differences in fp64."""
+    sim, _ = sim_from_cfg(base_sim(n_grid=32))
+    cfg = dict(FD_SHAPES[shape])
+    q = np.array([0.9, 0.2, 0.3, 0.1])
+    q /= np.linalg.norm(q)
+    cfg.update(init_pos=(0.5, 0.185, 0.5), init_rot=tuple(q))
+    o = Oracle(sim, [prim_from_cfg(cfg)])
+    rng = np.random.default_rng(1)
+    st = reset_state(rng.uniform([0.42, 0.12, 0.42], [0.58, 0.26, 0.58], (40, 3)))
+    ps = prim_state_from_cfg(cfg)[None]
+    d = o.min_dist(st, ps)
+    assert d.shape == ((2 if shape == "Gripper" else 1), 40)
+    np.testing.assert_allclose(d.min(axis=0), o.prim_sdf(0, ps[0], st[:, :3]), rtol=0, atol=1e-15)
+    w = rng.normal(size=d.shape)
+    adj, gps = o.min_dist_grad(st, ps, w)
+    assert np.all(adj[:, 3:] == 0)
+    eps = 1e-7
+
+    def L(s, p):
+        return (w * o.min_dist(s, p)).sum()
+
+    for p, c in [(0, 0), (5, 1), (11, 2), (23, 0), (37, 2)]:
+        s1, s2 = st.copy(), st.copy()
+        s1[p, c] += eps
+        s2[p, c] -= eps
+        num = (L(s1, ps) - L(s2, ps)) / (2 * eps)
+        assert abs(adj[p, c] - num) <= 1e-6 * max(1.0, abs(num)), (p, c, adj[p, c], num)
+    for k in range(8):
+        p1, p2 = ps.copy(), ps.copy()
+        p1[0, k] += eps
+        p2[0, k] -= eps
+        num = (L(st, p1) - L(st, p2)) / (2 * eps)
+        assert abs(gps[0, k] - num) <= 1e-6 * max(1.0, abs(num)), (k, gps[0, k], num)
