@@ -1131,27 +1131,54 @@ __global__ void __launch_bounds__(PPB) k_stress_bwd(ChunkArgs ca, const float* _
 }
 
 // compute_grid_m_kernel (mpm_simulator.py:456-466): dense per-scene output
+// compute_grid_m_kernel (mpm_simulator.py:456-466) into a dense per-scene
+// grid.  Particles are stored in cell order, so the lanes of a wave mostly
+// share a stencil base: each group of equal bases is summed through the wave
+// transpose tree and adds its 27 node masses with one atomic each (lanes past
+// MAXSEG groups add their own).
 __global__ void __launch_bounds__(PPB) k_grid_mass(ChunkArgs ca, const float* __restrict__ st, float* __restrict__ out) {
     int4 ch = ca.chunks[blockIdx.x];
     int i = threadIdx.x;
-    if (i >= ch.z) return;
+    int lane = i & 63;
+    bool valid = i < ch.z;
     const SceneC& S = ca.scenes[ch.x];
-    int slot = ch.y + i;
+    int slot = ch.y + (valid ? i : 0);
     v3 x = load3(st, ca.stride, 0, slot);
     Stencil sw;
     stencil(S, x, sw);
     float* o = out + (size_t)S.m_off;
     int n = S.n_grid;
+    unsigned long long act = __ballot(valid);
+    int seg = 0;
+    while (act) {
+        if (seg == MAXSEG) {
+            if ((act >> lane) & 1ull) {
 #pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                int I0 = sw.base[0] + a, I1 = sw.base[1] + b, I2 = sw.base[2] + c;
-                if (!in_grid(S, I0, I1, I2)) continue;
-                unsafeAtomicAdd(o + ((size_t)I0 * n + I1) * n + I2, sw.w[a][0] * sw.w[b][1] * sw.w[c][2] * S.p_mass);
+                for (int j = 0; j < 27; j++) {
+                    int I0 = sw.base[0] + j / 9, I1 = sw.base[1] + (j / 3) % 3, I2 = sw.base[2] + j % 3;
+                    if (in_grid(S, I0, I1, I2))
+                        unsafeAtomicAdd(o + ((size_t)I0 * n + I1) * n + I2, sw.w[j / 9][0] * sw.w[(j / 3) % 3][1] * sw.w[j % 3][2] * S.p_mass);
+                }
             }
+            break;
+        }
+        int lead = __ffsll((long long)act) - 1;
+        int s0 = __shfl(sw.base[0], lead), s1 = __shfl(sw.base[1], lead), s2 = __shfl(sw.base[2], lead);
+        unsigned long long m = __ballot(valid && sw.base[0] == s0 && sw.base[1] == s1 && sw.base[2] == s2);
+        bool in = (m >> lane) & 1ull;
+        float r[32];
+#pragma unroll
+        for (int j = 0; j < 32; j++)
+            r[j] = (in && j < 27) ? sw.w[j / 9][0] * sw.w[(j / 3) % 3][1] * sw.w[j % 3][2] * S.p_mass : 0.f;
+        float sum = tsum32(r, lane);
+        int j = lane >> 1;
+        if (!(lane & 1) && j < 27) {
+            int I0 = s0 + j / 9, I1 = s1 + (j / 3) % 3, I2 = s2 + j % 3;
+            if (in_grid(S, I0, I1, I2)) unsafeAtomicAdd(o + ((size_t)I0 * n + I1) * n + I2, sum);
+        }
+        act &= ~m;
+        seg++;
+    }
 }
 __global__ void __launch_bounds__(PPB) k_grid_mass_bwd(ChunkArgs ca, const float* __restrict__ st, const float* __restrict__ gm,
                                                        float* __restrict__ adj) {
