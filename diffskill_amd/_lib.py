@@ -48,10 +48,11 @@ EXPORTS = [
     "mpm_grid_mass", "mpm_grid_mass_bwd", "mpm_add_obs_grad", "mpm_action_grad", "mpm_clear_grads",
     "mpm_update_prims", "mpm_copy_frame", "mpm_set_profiling", "mpm_profile_read", "mpm_last_error",
     "mpm_set_launch", "mpm_get_launch", "mpm_min_dist_cols", "mpm_min_dist", "mpm_min_dist_bwd",
+    "mpm_collision_bytes", "mpm_set_collisions", "mpm_get_collisions",
 ]
 
 PHASES = ["bin", "compact", "blocks", "scatter", "stress", "p2g", "grid_op", "g2p", "p2g_grid", "grid_op_keep",
-          "g2p_bwd", "grid_op_bwd", "p2g_bwd", "stress_bwd"]
+          "g2p_bwd", "grid_op_bwd", "p2g_bwd", "stress_bwd", "collide", "collide_bwd"]
 
 _LIB = None
 
@@ -97,6 +98,9 @@ def load():
         "mpm_min_dist": (i32, [vp, i32, vp, i32, vp]),
         "mpm_min_dist_bwd": (i32, [vp, i32, vp, i32, vp]),
         "mpm_get_launch": (i32, [vp, vp, vp, vp]),
+        "mpm_collision_bytes": (i64, [vp, i32, i32]),
+        "mpm_set_collisions": (i32, [vp, i32, vp, i32, vp, vp, i64, vp]),
+        "mpm_get_collisions": (i32, [vp, i32, vp, vp, vp, vp]),
         "mpm_profile_read": (i32, [vp, vp, vp, i32]),
         "mpm_last_error": (ctypes.c_char_p, []),
     }

@@ -13,6 +13,7 @@ import torch
 
 from .config import derive
 from .engine.core import Engine, cell_sort_perm, scene_desc
+from .engine.mpm_simulator import collision_pairs, collision_rand
 from .engine.primitive.primitives import CLASSES
 
 
@@ -46,6 +47,15 @@ class SceneBatch:
         self.n_scenes = len(scenes)
         self.prim_offsets = np.concatenate([[0], np.cumsum([len(p) for p in self.prims])]).astype(int)
         self.n_prims = int(self.prim_offsets[-1])
+        # tool-tool collision pairs of every scene (each scene draws its own rand_num)
+        pairs, rands = [], []
+        for k, s in enumerate(scenes):
+            pk = collision_pairs([p.cfg for p in self.prims[k]])
+            pairs += [(i + int(self.prim_offsets[k]), j + int(self.prim_offsets[k])) for i, j in pk]
+            if pk:
+                rands.append(collision_rand(len(pk)))
+        if pairs:
+            e.set_collisions(pairs, np.concatenate(rands, 0))
         frame0 = torch.zeros((24, e.stride), dtype=torch.float32, device="cuda")
         eye = torch.tensor([1., 0, 0, 0, 1, 0, 0, 0, 1], device="cuda")
         ps0 = torch.zeros((max(1, self.n_prims), 8), dtype=torch.float32, device="cuda")

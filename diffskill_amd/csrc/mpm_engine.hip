@@ -18,6 +18,7 @@
 //   primitives [frame][prim][8] (pos, quat wxyz, gap), velocities [frame][prim][8].
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -1279,6 +1280,150 @@ __global__ void __launch_bounds__(PPB) k_min_dist_bwd(ChunkArgs ca, const float*
     if (valid) store3(adj, ca.stride, 0, slot, load3(adj, ca.stride, 0, slot) + gx);
 }
 
+// ------------------------------------------------------------------ tool-tool collision
+// Pairs (i, j) of one scene: after forward kinematics primitive i is moved
+// out of box j (set_surface_points / set_collision_idx /
+// apply_collision_projection, mpm_simulator.py:286-305,316-320).
+struct CollArgs {
+    const int4* pairs;    // (i, j, 0, 0): absolute primitive indices
+    const int2* groups;   // per scene with pairs: (first pair, pair count)
+    const float4* rand;   // [pair][point] rand_num (xyz)
+    int* idx;             // [frame][pair] collision_idx, -1: no contact
+    float4* pt;           // [frame][pair] the chosen surface point
+    float4* pre;          // [frame][pair] position of i before the pair's move
+    int n_pairs, n_points;
+};
+
+// Primitive.sdf / normal of a whole primitive at world point x: a two-jaw
+// gripper takes the smaller jaw SDF and the normal of jaw -1 when a <= b
+// (primitives.py:486-496,618-628).  *js is the jaw receiving the SDF's
+// adjoint (Taichi's min gives it to the left operand only when strictly
+// smaller), *jn the jaw whose normal is used.
+DEV float prim_sdf_jaws(const PrimC& P, const PS& s, v3 x, int* js, int* jn) {
+    Unit u[2];
+    int m = prim_units(P, u);
+    float a = unit_sdf(P, u[0], s, x);
+    if (m == 1) { *js = *jn = 0; return a; }
+    float b = unit_sdf(P, u[1], s, x);
+    *js = a < b ? 0 : 1;
+    *jn = a <= b ? 0 : 1;
+    return tmin(a, b);
+}
+
+// point k of pair p on box j: rand * size clamped to the box (Box._project,
+// primitives.py:369-372) in j's frame (get_surface_pos, primive_base.py:284-287)
+DEV v3 coll_surface_point(const CollArgs& ca, int p, int k, const PrimC& Pj, const PS& sj) {
+    float4 r = ca.rand[(size_t)p * ca.n_points + k];
+    v3 loc = mk(clampv(r.x * Pj.size[0], -Pj.size[0], Pj.size[0]), clampv(r.y * Pj.size[1], -Pj.size[1], Pj.size[1]),
+                clampv(r.z * Pj.size[2], -Pj.size[2], Pj.size[2]));
+    return qrot(sj.rot, loc) + sj.pos;
+}
+
+// One workgroup per scene with pairs.  Each pair's points are scored in
+// parallel against the poses after kinematics; the collision index is the
+// first point of the strictly smallest negative SDF (what the reference's
+// loop computes run serially).  Lane 0 then applies the moves in pair order.
+__global__ void __launch_bounds__(256) k_tool_collide(CollArgs ca, const PrimC* __restrict__ prims, float* __restrict__ pstate,
+                                                      int n_prims_total, int f1) {
+    int2 gr = ca.groups[blockIdx.x];
+    float* ps1 = pstate + (size_t)f1 * n_prims_total * 8;
+    size_t rec0 = (size_t)f1 * ca.n_pairs;
+    __shared__ float s_d[4];
+    __shared__ int s_k[4];
+    int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int p = gr.x; p < gr.x + gr.y; p++) {
+        int4 pr = ca.pairs[p];
+        const PrimC& Pi = prims[pr.x];
+        const PrimC& Pj = prims[pr.y];
+        PS si = load_ps(ps1 + (size_t)pr.x * 8), sj = load_ps(ps1 + (size_t)pr.y * 8);
+        float best = 0.f;
+        int bk = INT_MAX;
+        for (int k = t; k < ca.n_points; k += 256) {
+            int js, jn;
+            float d = prim_sdf_jaws(Pi, si, coll_surface_point(ca, p, k, Pj, sj), &js, &jn);
+            if (d < best) { best = d; bk = k; }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            float od = __shfl_xor(best, o);
+            int ok = __shfl_xor(bk, o);
+            if (od < best || (od == best && ok < bk)) { best = od; bk = ok; }
+        }
+        if (lane == 0) { s_d[wv] = best; s_k[wv] = bk; }
+        __syncthreads();
+        if (t == 0) {
+            for (int w = 1; w < 4; w++)
+                if (s_d[w] < best || (s_d[w] == best && s_k[w] < bk)) { best = s_d[w]; bk = s_k[w]; }
+            int k = bk == INT_MAX ? -1 : bk;
+            v3 x = k >= 0 ? coll_surface_point(ca, p, k, Pj, sj) : mk(0.f, 0.f, 0.f);
+            ca.idx[rec0 + p] = k;
+            ca.pt[rec0 + p] = make_float4(x.x, x.y, x.z, 0.f);
+        }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    for (int p = gr.x; p < gr.x + gr.y; p++) {
+        int i = ca.pairs[p].x;
+        float* psi = ps1 + (size_t)i * 8;
+        ca.pre[rec0 + p] = make_float4(psi[0], psi[1], psi[2], 0.f);
+        if (ca.idx[rec0 + p] < 0) continue;
+        const PrimC& Pi = prims[i];
+        PS si = load_ps(psi);
+        float4 x4 = ca.pt[rec0 + p];
+        v3 x = mk(x4.x, x4.y, x4.z);
+        int js, jn;
+        float d = prim_sdf_jaws(Pi, si, x, &js, &jn);
+        Unit u[2];
+        prim_units(Pi, u);
+        v3 n = unit_normal(Pi, u[jn], si, x);
+        float L = sqrtf(dot(n, n));
+        psi[0] += n.x / L * d;
+        psi[1] += n.y / L * d;
+        psi[2] += n.z / L * d;
+    }
+}
+
+// Adjoint of the moves, pairs in reverse order, one lane per scene: with
+// position_i' = position_i + n^ d, g_pos += J_pos^T g_pos and the rotation /
+// gap of i get J^T g_pos.  J is taken at the pose each move started from
+// (the recorded position; rotation and gap are not moved).  The points are
+// constants (rand_points has no gradient, mpm_simulator.py:65).
+__global__ void __launch_bounds__(64) k_tool_collide_bwd(CollArgs ca, const PrimC* __restrict__ prims,
+                                                         const float* __restrict__ pstate, float* __restrict__ gpstate,
+                                                         int n_prims_total, int f1) {
+    if (threadIdx.x != 0) return;
+    int2 gr = ca.groups[blockIdx.x];
+    size_t rec0 = (size_t)f1 * ca.n_pairs;
+    for (int p = gr.x + gr.y - 1; p >= gr.x; p--) {
+        if (ca.idx[rec0 + p] < 0) continue;
+        int i = ca.pairs[p].x;
+        const PrimC& Pi = prims[i];
+        size_t o = ((size_t)f1 * n_prims_total + i) * 8;
+        PS s = load_ps(pstate + o);
+        float4 q4 = ca.pre[rec0 + p], x4 = ca.pt[rec0 + p];
+        s.pos = mk(q4.x, q4.y, q4.z);
+        v3 x = mk(x4.x, x4.y, x4.z);
+        int js, jn;
+        float d = prim_sdf_jaws(Pi, s, x, &js, &jn);
+        Unit u[2];
+        prim_units(Pi, u);
+        v3 n = unit_normal(Pi, u[jn], s, x);
+        float L = sqrtf(dot(n, n));
+        v3 nh = mk(n.x / L, n.y / L, n.z / L);
+        float* gi = gpstate + o;
+        v3 g = mk(gi[0], gi[1], gi[2]);
+        v3 gnh = d * g;
+        float tt = dot(nh, gnh);
+        v3 gn = mk((gnh.x - nh.x * tt) / L, (gnh.y - nh.y * tt) / L, (gnh.z - nh.z * tt) / L);
+        PSg acc = zero_psg();
+        unit_sdf_rev(Pi, u[js], s, x, dot(nh, g), &acc);
+        unit_normal_rev(Pi, u[jn], s, x, gn, &acc);
+        gi[0] += acc.pos.x; gi[1] += acc.pos.y; gi[2] += acc.pos.z;
+        gi[3] += acc.rot.w; gi[4] += acc.rot.x; gi[5] += acc.rot.y; gi[6] += acc.rot.z;
+        gi[7] += acc.gap;
+    }
+}
+
 __global__ void k_reset_frame0(int stride, const float* __restrict__ x, float* __restrict__ st) {
     int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= stride) return;
@@ -1383,6 +1528,10 @@ struct mpm_ctx {
     // event fork and join, so the two overlap (parallel branches in a graph)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // tool-tool collision pairs (mpm_set_collisions); tables and per-frame
+    // records live in a caller-owned device buffer
+    int n_pairs = 0, n_cgroups = 0;
+    CollArgs coll{};
     // per-kernel timing with HIP events on the launch stream (mpm_set_profiling)
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -1579,6 +1728,10 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
     if (fork) HIPCHK(hipEventRecord(c->ev_join, c->side));
     c->aff_of[c->slot_of(f)] = f;
     if (run_bin(c, f, 1, st)) return -1;
+    if (c->n_pairs > 0) {
+        PhaseTimer t(c, MPM_PH_COLLIDE, st);
+        hipLaunchKernelGGL(k_tool_collide, dim3(c->n_cgroups), dim3(256), 0, st, c->coll, c->d_prims, c->pstate, c->n_prims, f + 1);
+    }
     if (fork) HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
     { PhaseTimer t(c, MPM_PH_P2G, st);
       hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm, 0); }
@@ -1648,6 +1801,13 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     { PhaseTimer t(c, MPM_PH_GRID_BWD, st);
       hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->g_pm, c->tiles_v, c->g_adjpm, c->pstate_g,
                          use_kept); }
+    if (c->n_pairs > 0) {
+        // between the grid adjoint (which adds into frame f+1's primitive
+        // adjoint) and the kinematics adjoint (which consumes it)
+        PhaseTimer t(c, MPM_PH_COLLIDE_BWD, st);
+        hipLaunchKernelGGL(k_tool_collide_bwd, dim3(c->n_cgroups), dim3(64), 0, st, c->coll, c->d_prims, c->pstate, c->pstate_g,
+                           c->n_prims, f + 1);
+    }
     { PhaseTimer t(c, MPM_PH_P2G_BWD, st);
       hipLaunchKernelGGL(k_p2g_bwd_gather, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, a0, c->gaff, c->g_adjpm); }
     { PhaseTimer t(c, MPM_PH_STRESS_BWD, st);
@@ -2121,6 +2281,112 @@ int mpm_min_dist_bwd(mpm_ctx* ctx, int f, const float* gout, int n_cols, void* s
                        ctx->pstate + (size_t)f * ctx->n_prims * 8, gout, ctx->adj_ptr(f),
                        ctx->pstate_g + (size_t)f * ctx->n_prims * 8);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static size_t coll_layout(int n_pairs, int n_points, int n_scenes, int max_frames, size_t off[6]) {
+    size_t o = 0;
+    off[0] = o; o = align256(o + sizeof(int4) * (size_t)n_pairs);
+    off[1] = o; o = align256(o + sizeof(int2) * (size_t)n_scenes);
+    off[2] = o; o = align256(o + sizeof(float4) * (size_t)n_pairs * n_points);
+    off[3] = o; o = align256(o + sizeof(int) * (size_t)max_frames * n_pairs);
+    off[4] = o; o = align256(o + sizeof(float4) * (size_t)max_frames * n_pairs);
+    off[5] = o; o = align256(o + sizeof(float4) * (size_t)max_frames * n_pairs);
+    return o;
+}
+
+static const int COLL_MAX_POINTS = 1 << 16;
+static const int COLL_MAX_PAIRS = 1 << 12;
+
+int64_t mpm_collision_bytes(mpm_ctx* ctx, int n_pairs, int n_points) {
+    if (!ctx || n_pairs < 0 || n_pairs > COLL_MAX_PAIRS || n_points < 1 || n_points > COLL_MAX_POINTS) return -1;
+    size_t off[6];
+    return (int64_t)coll_layout(n_pairs, n_points, ctx->n_scenes, ctx->max_frames, off);
+}
+
+int mpm_set_collisions(mpm_ctx* ctx, int n_pairs, const int* pairs, int n_points, const float* rand_num, void* buffer,
+                       int64_t buffer_bytes, void* stream) {
+    if (!ctx) return fail("mpm_set_collisions: null context");
+    hipStream_t st = S_(stream);
+    if (n_pairs == 0) {
+        ctx->n_pairs = 0;
+        ctx->n_cgroups = 0;
+        ctx->coll = CollArgs{};
+        return 0;
+    }
+    if (n_pairs < 0 || n_pairs > COLL_MAX_PAIRS) return fail("mpm_set_collisions: pair count outside [0, 4096]");
+    if (n_points < 1 || n_points > COLL_MAX_POINTS) return fail("mpm_set_collisions: point count outside [1, 65536]");
+    if (!pairs || !rand_num || !buffer) return fail("mpm_set_collisions: null argument");
+    size_t off[6];
+    size_t need = coll_layout(n_pairs, n_points, ctx->n_scenes, ctx->max_frames, off);
+    if (buffer_bytes < (int64_t)need) return fail("mpm_set_collisions: buffer smaller than mpm_collision_bytes");
+    if (((uintptr_t)buffer & 255) != 0) return fail("mpm_set_collisions: buffer not 256-byte aligned");
+    // scene of each primitive
+    std::vector<int> prim_scene(ctx->n_prims, -1);
+    for (int k = 0; k < ctx->n_scenes; k++)
+        for (int q = 0; q < ctx->scenes[k].n_prims; q++) prim_scene[ctx->scenes[k].prim_off + q] = k;
+    std::vector<int4> hp(n_pairs);
+    std::vector<int2> hg;
+    int last_scene = -1;
+    for (int p = 0; p < n_pairs; p++) {
+        int i = pairs[2 * p], j = pairs[2 * p + 1];
+        if (i < 0 || i >= ctx->n_prims || j < 0 || j >= ctx->n_prims)
+            return fail("mpm_set_collisions: pair " + std::to_string(p) + " names a primitive outside the context");
+        if (prim_scene[i] != prim_scene[j]) return fail("mpm_set_collisions: pair " + std::to_string(p) + " spans two scenes");
+        int tj = ctx->prims[j].type;
+        if (tj != P_BOX && tj != P_GRIPPER)
+            return fail("mpm_set_collisions: pair " + std::to_string(p) + ": the sampled primitive must be a Box or Gripper");
+        int sc = prim_scene[i];
+        if (sc < last_scene) return fail("mpm_set_collisions: pairs are not grouped by ascending scene");
+        if (sc != last_scene) { hg.push_back(make_int2(p, 0)); last_scene = sc; }
+        hg.back().y++;
+        hp[p] = make_int4(i, j, 0, 0);
+    }
+    std::vector<float4> hr((size_t)n_pairs * n_points);
+    for (size_t k = 0; k < hr.size(); k++) hr[k] = make_float4(rand_num[3 * k], rand_num[3 * k + 1], rand_num[3 * k + 2], 0.f);
+    char* b = (char*)buffer;
+    CollArgs ca{};
+    ca.pairs = (const int4*)(b + off[0]);
+    ca.groups = (const int2*)(b + off[1]);
+    ca.rand = (const float4*)(b + off[2]);
+    ca.idx = (int*)(b + off[3]);
+    ca.pt = (float4*)(b + off[4]);
+    ca.pre = (float4*)(b + off[5]);
+    ca.n_pairs = n_pairs;
+    ca.n_points = n_points;
+    HIPCHK(hipMemcpyAsync(b + off[0], hp.data(), sizeof(int4) * hp.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b + off[1], hg.data(), sizeof(int2) * hg.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b + off[2], hr.data(), sizeof(float4) * hr.size(), hipMemcpyHostToDevice, st));
+    // no frame has records yet: every collision index reads -1 (all bytes 0xff)
+    HIPCHK(hipMemsetAsync(b + off[3], 0xff, sizeof(int) * (size_t)ctx->max_frames * n_pairs, st));
+    HIPCHK(hipMemsetAsync(b + off[4], 0, need - off[4], st));
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->coll = ca;
+    ctx->n_pairs = n_pairs;
+    ctx->n_cgroups = (int)hg.size();
+    return 0;
+}
+
+int mpm_get_collisions(mpm_ctx* ctx, int f, int* idx, float* points, float* pre, void* stream) {
+    if (!ctx) return fail("mpm_get_collisions: null context");
+    if (f < 1 || f >= ctx->max_frames) return fail("mpm_get_collisions: frame outside [1, max_frames)");
+    if (ctx->n_pairs == 0) return 0;
+    hipStream_t st = S_(stream);
+    int n = ctx->n_pairs;
+    size_t r0 = (size_t)f * n;
+    std::vector<float4> tmp(n);
+    if (idx) HIPCHK(hipMemcpyAsync(idx, ctx->coll.idx + r0, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    if (points) {
+        HIPCHK(hipMemcpyAsync(tmp.data(), ctx->coll.pt + r0, sizeof(float4) * n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int p = 0; p < n; p++) { points[3 * p] = tmp[p].x; points[3 * p + 1] = tmp[p].y; points[3 * p + 2] = tmp[p].z; }
+    }
+    if (pre) {
+        HIPCHK(hipMemcpyAsync(tmp.data(), ctx->coll.pre + r0, sizeof(float4) * n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int p = 0; p < n; p++) { pre[3 * p] = tmp[p].x; pre[3 * p + 1] = tmp[p].y; pre[3 * p + 2] = tmp[p].z; }
+    }
+    HIPCHK(hipStreamSynchronize(st));
     return 0;
 }
 

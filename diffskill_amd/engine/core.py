@@ -173,6 +173,43 @@ class Engine:
         _lib.check(self.L.mpm_min_dist_bwd(self.ctx, int(f), ctypes.c_void_p(g.data_ptr()), int(g.shape[0]), _stream()),
                    "mpm_min_dist_bwd")
 
+    def set_collisions(self, pairs, rand_num):
+        """Tool-tool collision pairs [(i, j), ...] (primitive indices of this
+        context, grouped by scene) and their rand_num (n_pairs, n_points, 3);
+        an empty list switches the projection off."""
+        pairs = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+        n = pairs.shape[0]
+        if n == 0:
+            _lib.check(self.L.mpm_set_collisions(self.ctx, 0, None, 1, None, None, 0, _stream()), "mpm_set_collisions")
+            self.coll_buf = None
+            self.n_pairs = 0
+            return
+        rand = np.ascontiguousarray(np.asarray(rand_num, np.float32))
+        if rand.ndim != 3 or rand.shape[0] != n or rand.shape[2] != 3:
+            raise ValueError("rand_num must be (n_pairs, n_points, 3), got %s" % (rand.shape,))
+        nbytes = self.L.mpm_collision_bytes(self.ctx, n, rand.shape[1])
+        if nbytes <= 0:
+            raise ValueError("mpm_collision_bytes rejected %d pairs x %d points" % (n, rand.shape[1]))
+        buf = torch.empty(int(nbytes), dtype=torch.uint8, device="cuda")
+        _lib.check(self.L.mpm_set_collisions(self.ctx, n, pairs.ctypes.data_as(ctypes.c_void_p), rand.shape[1],
+                                             rand.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(buf.data_ptr()), int(nbytes),
+                                             _stream()), "mpm_set_collisions")
+        self.coll_buf = buf        # the context reads it until the next call
+        self.n_pairs = n
+
+    def get_collisions(self, f):
+        """Records of frame f (substep f-1 -> f): (collision_idx (n_pairs,),
+        chosen points (n_pairs, 3), positions of i before each move (n_pairs, 3))."""
+        n = getattr(self, "n_pairs", 0)
+        idx = np.full(n, -1, np.int32)
+        pts = np.zeros((n, 3), np.float32)
+        pre = np.zeros((n, 3), np.float32)
+        if n:
+            _lib.check(self.L.mpm_get_collisions(self.ctx, int(f), idx.ctypes.data_as(ctypes.c_void_p),
+                                                 pts.ctypes.data_as(ctypes.c_void_p), pre.ctypes.data_as(ctypes.c_void_p),
+                                                 _stream()), "mpm_get_collisions")
+        return idx, pts, pre
+
     def action_grad(self, s, n_substeps):
         out = torch.zeros((max(1, self.n_prims), 7), dtype=torch.float32, device="cuda")
         _lib.check(self.L.mpm_action_grad(self.ctx, int(s), int(n_substeps), ctypes.c_void_p(out.data_ptr()), _stream()),

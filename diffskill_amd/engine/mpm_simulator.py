@@ -14,6 +14,32 @@ from ..config import derive
 from .core import Engine, cell_sort_perm, scene_desc
 
 
+NUM_PARTICLE_COLLISION = 600  # points sampled per collision pair (mpm_simulator.py:59)
+
+
+def collision_pairs(prim_cfgs):
+    """set_num_collision (mpm_simulator.py:69-80): (i, j) for every primitive
+    i and j with collision_group[j] > 0 in i's cfg, i-major.  Entries past
+    the end of a collision_group list (the default has three) count as 0."""
+    pairs = []
+    n = len(prim_cfgs)
+    for i in range(n):
+        grp = prim_cfgs[i].get("collision_group", None) if hasattr(prim_cfgs[i], "get") else None
+        if grp is None:
+            continue
+        for j in range(n):
+            if j < len(grp) and float(grp[j]) > 0:
+                pairs.append((i, j))
+    return pairs
+
+
+def collision_rand(n_pairs, n_points=NUM_PARTICLE_COLLISION):
+    """rand_num of MPMSimulator.initialize (mpm_simulator.py:87-96): the
+    legacy generator seeded with 42, uniform(-1.5, 1.5, (pairs, points, 3)),
+    stored as float32; the caller's global numpy random state is untouched."""
+    return np.random.RandomState(42).uniform(-1.5, 1.5, (n_pairs, n_points, 3)).astype(np.float32)
+
+
 class _Scalar:
     """Stand-in for a 0-d Taichi field read as ``field[None]``."""
 
@@ -99,11 +125,27 @@ class MPMSimulator:
         self._inv = None
         self.engine = None
         self._ctx = None
+        self.num_particle_collision = NUM_PARTICLE_COLLISION
+        self.set_num_collision()
+        self.rand_num = collision_rand(self.num_collision) if self.num_collision > 0 else None
         if hasattr(primitives, "_sim"):
             primitives._sim = self
         for j, p in enumerate(primitives):
             p._sim = self
             p._index = j
+
+    # ---------------------------------------------------------------- collisions
+    def set_num_collision(self):
+        """mpm_simulator.py:69-80."""
+        self.collision_pairs = [list(p) for p in collision_pairs([p.cfg for p in self.primitives])]
+        self.num_collision = len(self.collision_pairs)
+        return self.num_collision
+
+    def get_collisions(self, f):
+        """collision_idx of frame f (substep f-1 -> f) per pair (-1: no
+        contact), the chosen surface points (n_pairs, 3) and the positions of
+        the projected primitive before each pair's move."""
+        return self.engine.get_collisions(f)
 
     # ---------------------------------------------------------------- context
     def _prim_descs(self):
@@ -123,6 +165,8 @@ class MPMSimulator:
             self.engine.prim_state[0, :self.n_primitive] = keep
         for j, p in enumerate(self.primitives):
             p._bind(self, j)
+        if self.num_collision > 0:
+            self.engine.set_collisions(self.collision_pairs, self.rand_num)
 
     def _prim_state_view(self):
         return self.engine.prim_state

@@ -26,6 +26,10 @@
  *   mpm_action_grad                    set_velocity.grad + get_action_grad
  *                                      (function.py:180-190, primive_base.py:276-282)
  *   mpm_clear_grads                    GradModel.reset grad clearing (function.py:44-61)
+ *   mpm_collision_bytes / mpm_set_collisions / mpm_get_collisions
+ *                                      set_surface_points / set_collision_idx /
+ *                                      apply_collision_projection and their .grad
+ *                                      (mpm_simulator.py:286-305,316-320,337-339)
  *   mpm_last_error                     Python exceptions of the reference
  *
  * Threading: one context per stream; no internal locking (the reference is
@@ -168,6 +172,35 @@ int mpm_min_dist(mpm_ctx *ctx, int f, float *out, int n_cols, void *stream);
  * the current adjoint frame) and the primitive-state adjoint of frame f. */
 int mpm_min_dist_bwd(mpm_ctx *ctx, int f, const float *gout, int n_cols, void *stream);
 
+/* Tool-tool collision projection (MPMSimulator.set_num_collision,
+ * initialize and substep, mpm_simulator.py:69-96,286-320).  Every forward
+ * substep, after forward kinematics, each pair (i, j) samples n_points
+ * points of box j (rand_num[k] * size_j clamped to the box, moved by j's
+ * pose), takes the point with the smallest negative SDF of primitive i (the
+ * first on ties; none if no point is inside) and moves i by
+ * normalized(normal_i) * sdf_i at that point; all points and choices use the
+ * poses after kinematics, and the moves apply in pair order.  The adjoint
+ * (substep_grad, mpm_simulator.py:337-339) reaches i's position, rotation
+ * and gap; the points carry none (rand_points has no gradient).
+ *
+ * mpm_collision_bytes: device bytes the pairs' tables and per-frame records
+ * need (-1 on bad sizes).  mpm_set_collisions: pairs[2p], pairs[2p+1] = (i, j)
+ * as primitive indices of the context, both of one scene, grouped by scene
+ * in ascending order and in the reference's order within a scene; j must be
+ * a Box or Gripper (Box._project, primitives.py:369-372); rand_num is a host
+ * array n_pairs x n_points x 3 (the reference draws it with
+ * np.random.seed(42); uniform(-1.5, 1.5)); buffer is caller-owned device
+ * memory of at least mpm_collision_bytes that must outlive the context or
+ * the next call.  n_pairs = 0 switches the pass off.  Synchronizes stream.
+ * mpm_get_collisions: frame f's records (f in [1, max_frames), written by
+ * the substep f-1 -> f): idx[n_pairs] collision_idx (-1: no contact),
+ * points[n_pairs][3] the chosen points, pre[n_pairs][3] the position of i
+ * before the pair's move (host arrays; any may be NULL).  Synchronizes. */
+int64_t mpm_collision_bytes(mpm_ctx *ctx, int n_pairs, int n_points);
+int mpm_set_collisions(mpm_ctx *ctx, int n_pairs, const int *pairs, int n_points, const float *rand_num, void *buffer,
+                       int64_t buffer_bytes, void *stream);
+int mpm_get_collisions(mpm_ctx *ctx, int f, int *idx, float *points, float *pre, void *stream);
+
 /* Per-kernel timing with HIP events recorded on the launch stream around
  * every engine kernel (no counterpart in the reference; measurement only).
  * Phases (MPM_PH_COMPACT is kept for numbering and no longer launched: the
@@ -176,7 +209,8 @@ enum {
     MPM_PH_BIN = 0, MPM_PH_COMPACT = 1, MPM_PH_BLOCKS = 2, MPM_PH_SCATTER = 3, MPM_PH_STRESS = 4,
     MPM_PH_P2G = 5, MPM_PH_GRID = 6, MPM_PH_G2P = 7, MPM_PH_P2G_GRID = 8, MPM_PH_GRID_KEEP = 9,
     MPM_PH_G2P_BWD = 10, MPM_PH_GRID_BWD = 11, MPM_PH_P2G_BWD = 12, MPM_PH_STRESS_BWD = 13,
-    MPM_N_PHASES = 14
+    MPM_PH_COLLIDE = 14, MPM_PH_COLLIDE_BWD = 15,
+    MPM_N_PHASES = 16
 };
 int mpm_set_profiling(mpm_ctx *ctx, int on);
 

@@ -78,6 +78,16 @@ typedef struct {
     double action_scale[7];
 } orc_prim;
 
+/* tool-tool collision pairs of one scene and their per-substep records */
+typedef struct {
+    int n_pairs, n_points;
+    const int *pairs;   /* n_pairs x 2: (i, j), primitive i projected out of box j */
+    const real *rand;   /* n_pairs x n_points x 3 */
+    int *idx;           /* n_pairs: collision_idx (-1: none) */
+    real *pt;           /* n_pairs x 3: the chosen surface point */
+    real *pre;          /* n_pairs x 3: position of i before the pair's projection */
+} orc_coll;
+
 static int g_threads = 1;
 void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
 int orc_real_size(void) { return (int)sizeof(real); }
@@ -1120,17 +1130,130 @@ static void g2p(const consts_t *k, int np, const real *st, const real *gout, rea
     }
 }
 
-/* one substep s -> s+1 (mpm_simulator.py:307-323).  grid: n^3*4, gout: n^3*3 scratch */
-void orc_substep(const orc_sim *S, const orc_prim *prims, int np, const real *st, real *st_out,
-                 const real *ps_f, const real *pv_f, real *ps_f1, real *grid, real *gout) {
+/* ------------------------------------------------------------------ */
+/* tool-tool collision projection (mpm_simulator.py:286-305,316-320)   */
+/* ------------------------------------------------------------------ */
+/* Primitive.sdf and Primitive.normal of a whole primitive at world point x:
+ * a two-jaw gripper takes the smaller jaw SDF and the normal of jaw -1 when
+ * a <= b (primitives.py:486-496,618-628).  *jsdf is the jaw that receives
+ * the SDF's adjoint: Taichi's min sends it to the left operand only when it
+ * is strictly smaller. */
+static real prim_sdf_normal(const orc_prim *P, const real *ps, const real *x, real *n, int *jsdf, int *jn) {
+    unit_t u[2];
+    int m = prim_units(P, u);
+    if (m == 1) {
+        *jsdf = *jn = 0;
+        if (n) unit_normal(&u[0], ps, x, n);
+        return unit_sdf(&u[0], ps, x);
+    }
+    real a = unit_sdf(&u[0], ps, x), b = unit_sdf(&u[1], ps, x);
+    *jsdf = a < b ? 0 : 1;
+    *jn = a <= b ? 0 : 1;
+    if (n) unit_normal(&u[*jn], ps, x, n);
+    return tmin(a, b);
+}
+
+/* All pairs' surface points and collision indices are taken from the poses
+ * after forward kinematics, then the projections are applied in pair order:
+ *  - set_surface_points (mpm_simulator.py:286-292): point k of pair (i, j) is
+ *    rand[k] * size_j clamped to the box (Box._project, primitives.py:369-372)
+ *    and moved to the world by j's pose (Primitive.get_surface_pos,
+ *    primive_base.py:284-287);
+ *  - set_collision_idx (mpm_simulator.py:294-298, primive_base.py:135-143):
+ *    the point of smallest negative SDF of i, the first one on ties (what the
+ *    reference's loop computes run serially), -1 if none is inside;
+ *  - apply_collision_projection (mpm_simulator.py:300-305,
+ *    primive_base.py:145-150): position_i += normalized(normal_i) * sdf_i.
+ * pre[p] records position_i before pair p's projection. */
+static void tool_collide(const orc_prim *prims, const orc_coll *c, real *ps1) {
+    for (int p = 0; p < c->n_pairs; p++) {
+        int i = c->pairs[2 * p], j = c->pairs[2 * p + 1];
+        const orc_prim *Pi = &prims[i], *Pj = &prims[j];
+        const real *psi = ps1 + i * NPS, *psj = ps1 + j * NPS;
+        real best = 0, bw[3] = {0, 0, 0};
+        int bk = -1;
+        for (int k = 0; k < c->n_points; k++) {
+            const real *r = c->rand + ((size_t)p * c->n_points + k) * 3;
+            real loc[3], w[3];
+            for (int d = 0; d < 3; d++) loc[d] = clampv(r[d] * Pj->size[d], -Pj->size[d], Pj->size[d]);
+            qrot(psj + 3, loc, w);
+            for (int d = 0; d < 3; d++) w[d] += psj[d];
+            int js, jn;
+            real dist = prim_sdf_normal(Pi, psi, w, NULL, &js, &jn);
+            if (dist < best) { best = dist; bk = k; memcpy(bw, w, sizeof(bw)); }
+        }
+        c->idx[p] = bk;
+        memcpy(c->pt + 3 * p, bw, sizeof(bw));
+    }
+    for (int p = 0; p < c->n_pairs; p++) {
+        int i = c->pairs[2 * p];
+        real *psi = ps1 + i * NPS;
+        memcpy(c->pre + 3 * p, psi, 3 * sizeof(real));
+        if (c->idx[p] < 0) continue;
+        real n[3];
+        int js, jn;
+        real dist = prim_sdf_normal(&prims[i], psi, c->pt + 3 * p, n, &js, &jn);
+        real L = sqrt(dot3(n, n));
+        for (int d = 0; d < 3; d++) psi[d] += n[d] / L * dist;
+    }
+}
+
+/* Adjoint of the projections, pairs in reverse order.  The surface points
+ * carry no gradient (rand_points has no needs_grad, mpm_simulator.py:65), so
+ * only primitive i's state receives one: position_i' = position_i +
+ * n^(x_i) d(x_i) gives g_pos += J_pos^T g_pos and the rotation / gap
+ * adjoints J^T g_pos.  J is taken at the pose the projection started from
+ * (pre[p], rotation and gap unchanged by projections); the reference's
+ * in-place update makes its grad kernel read the projected pose instead,
+ * where the point lies on the tool surface and a box SDF's gradient is
+ * decided by rounding. */
+static void tool_collide_rev(const orc_prim *prims, const orc_coll *c, const real *ps1, real *gps1) {
+    for (int p = c->n_pairs - 1; p >= 0; p--) {
+        if (c->idx[p] < 0) continue;
+        int i = c->pairs[2 * p];
+        const orc_prim *Pi = &prims[i];
+        real ps[NPS];
+        memcpy(ps, ps1 + i * NPS, sizeof(ps));
+        memcpy(ps, c->pre + 3 * p, 3 * sizeof(real));
+        const real *x = c->pt + 3 * p;
+        real n[3];
+        int js, jn;
+        real dist = prim_sdf_normal(Pi, ps, x, n, &js, &jn);
+        real L = sqrt(dot3(n, n)), nh[3], g[3], gnh[3], gn[3];
+        real *gi = gps1 + i * NPS;
+        for (int d = 0; d < 3; d++) { nh[d] = n[d] / L; g[d] = gi[d]; gnh[d] = dist * g[d]; }
+        real gd = dot3(nh, g), t = dot3(nh, gnh);
+        for (int d = 0; d < 3; d++) gn[d] = (gnh[d] - nh[d] * t) / L;
+        unit_t u[2];
+        prim_units(Pi, u);
+        unit_sdf_rev(&u[js], ps, x, gd, gi);
+        unit_normal_rev(&u[jn], ps, x, gn, gi);
+    }
+}
+
+/* the collision pass alone on one frame's primitive states (in place) */
+void orc_tool_collide(const orc_prim *prims, const orc_coll *c, real *ps1) { tool_collide(prims, c, ps1); }
+void orc_tool_collide_rev(const orc_prim *prims, const orc_coll *c, const real *ps1, real *gps1) {
+    tool_collide_rev(prims, c, ps1, gps1);
+}
+
+/* one substep s -> s+1 (mpm_simulator.py:307-323).  grid: n^3*4, gout: n^3*3
+ * scratch; c (may be NULL) the tool-tool collision pairs and their records */
+void orc_substep_c(const orc_sim *S, const orc_prim *prims, int np, const real *st, real *st_out,
+                   const real *ps_f, const real *pv_f, real *ps_f1, real *grid, real *gout, const orc_coll *c) {
     consts_t k;
     make_consts(S, &k);
     size_t G = (size_t)k.n * k.n * k.n;
     memset(grid, 0, G * 4 * sizeof(real));
     p2g(&k, np, st, st_out, grid);
     orc_fk(prims, S->n_prims, ps_f, pv_f, ps_f1);
+    if (c && c->n_pairs > 0) tool_collide(prims, c, ps_f1);
     grid_op(&k, S, prims, ps_f, ps_f1, grid, gout);
     g2p(&k, np, st, gout, st_out);
+}
+void orc_substep(const orc_sim *S, const orc_prim *prims, int np, const real *st, real *st_out,
+                 const real *ps_f, const real *pv_f, real *ps_f1, real *grid, real *gout) {
+    orc_substep_c(S, prims, np, st, st_out, ps_f, pv_f, ps_f1, grid, gout, NULL);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1337,9 +1460,9 @@ static void p2g_rev(const consts_t *k, int np, const real *st, const real *ggrid
  * contribution, as in the reference's LIFO order); gpv: adjoint of the
  * per-substep primitive velocity at s.  Scratch: grid n^3*4, gout n^3*3,
  * ggout n^3*3, ggrid n^3*4. */
-void orc_substep_grad(const orc_sim *S, const orc_prim *prims, int np, const real *st, const real *adj1, real *adj0,
-                      const real *ps_f, const real *pv_f, const real *ps_f1, real *gps_f, real *gps_f1, real *gpv_f,
-                      real *grid, real *gout, real *ggout, real *ggrid) {
+void orc_substep_grad_c(const orc_sim *S, const orc_prim *prims, int np, const real *st, const real *adj1, real *adj0,
+                        const real *ps_f, const real *pv_f, const real *ps_f1, real *gps_f, real *gps_f1, real *gpv_f,
+                        real *grid, real *gout, real *ggout, real *ggrid, const orc_coll *c) {
     consts_t k;
     make_consts(S, &k);
     size_t G = (size_t)k.n * k.n * k.n;
@@ -1350,9 +1473,15 @@ void orc_substep_grad(const orc_sim *S, const orc_prim *prims, int np, const rea
     grid_op(&k, S, prims, ps_f, ps_f1, grid, gout);
     g2p_rev(&k, np, st, gout, adj1, adj0, ggout);
     grid_op_rev(&k, S, prims, ps_f, ps_f1, grid, ggout, ggrid, gps_f, gps_f1);
+    if (c && c->n_pairs > 0) tool_collide_rev(prims, c, ps_f1, gps_f1);
     for (int q = S->n_prims - 1; q >= 0; q--)
         prim_fk_rev(&prims[q], ps_f + q * NPS, pv_f + q * NPV, gps_f1 + q * NPS, gps_f + q * NPS, gpv_f + q * NPV);
     p2g_rev(&k, np, st, ggrid, adj1, adj0);
+}
+void orc_substep_grad(const orc_sim *S, const orc_prim *prims, int np, const real *st, const real *adj1, real *adj0,
+                      const real *ps_f, const real *pv_f, const real *ps_f1, real *gps_f, real *gps_f1, real *gpv_f,
+                      real *grid, real *gout, real *ggout, real *ggrid) {
+    orc_substep_grad_c(S, prims, np, st, adj1, adj0, ps_f, pv_f, ps_f1, gps_f, gps_f1, gpv_f, grid, gout, ggout, ggrid, NULL);
 }
 
 /* compute_grid_m_kernel (mpm_simulator.py:456-466): adds into m (n^3) */

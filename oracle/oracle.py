@@ -54,6 +54,14 @@ class OrcPrim(ctypes.Structure):
     ]
 
 
+class OrcColl(ctypes.Structure):
+    _fields_ = [
+        ("n_pairs", ctypes.c_int), ("n_points", ctypes.c_int),
+        ("pairs", ctypes.c_void_p), ("rand", ctypes.c_void_p),
+        ("idx", ctypes.c_void_p), ("pt", ctypes.c_void_p), ("pre", ctypes.c_void_p),
+    ]
+
+
 def build():
     """Compile liborc64.so / liborc32.so with the committed Makefile."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -199,7 +207,9 @@ def reset_state(x):
 class Oracle:
     """One scene advanced and differentiated by the C restatement."""
 
-    def __init__(self, sim, prims, bits=64, threads=1):
+    def __init__(self, sim, prims, bits=64, threads=1, collisions=None):
+        """collisions: optional (pairs, rand) -- tool-tool collision pairs
+        (i, j) in the reference's order and rand (n_pairs, n_points, 3)."""
         self.bits = bits
         self.dtype = np.float64 if bits == 64 else np.float32
         self.L = lib(bits)
@@ -218,6 +228,46 @@ class Oracle:
         self.ggout = np.zeros(G * 3, dt)
         self.ggrid = np.zeros(G * 4, dt)
         self.np_ = len(prims)
+        self.coll = None
+        self.last_coll = None
+        if collisions is not None and len(collisions[0]) > 0:
+            pairs = np.ascontiguousarray(np.asarray(collisions[0], np.int32).reshape(-1, 2))
+            rand = self.arr(collisions[1])
+            assert rand.ndim == 3 and rand.shape[0] == pairs.shape[0] and rand.shape[2] == 3
+            self.coll = (pairs, rand)
+
+    def _coll(self, rec=None):
+        """OrcColl over fresh (or the given) per-substep records."""
+        pairs, rand = self.coll
+        n = pairs.shape[0]
+        if rec is None:
+            rec = dict(idx=np.full(n, -1, np.int32), pt=np.zeros((n, 3), self.dtype), pre=np.zeros((n, 3), self.dtype))
+        c = OrcColl()
+        c.n_pairs = n
+        c.n_points = rand.shape[1]
+        c.pairs = pairs.ctypes.data
+        c.rand = rand.ctypes.data
+        c.idx = rec["idx"].ctypes.data
+        c.pt = rec["pt"].ctypes.data
+        c.pre = rec["pre"].ctypes.data
+        return c, rec
+
+    def tool_collide(self, ps1):
+        """The collision pass alone on one frame's primitive states: returns
+        (projected states, record dict idx / pt / pre)."""
+        k = max(1, self.np_)
+        ps1 = self.arr(ps1, (k, 8)).copy()
+        c, rec = self._coll()
+        self.L.orc_tool_collide(self.P, ctypes.byref(c), _ptr(ps1))
+        return ps1, rec
+
+    def tool_collide_rev(self, ps1, rec, gps1):
+        k = max(1, self.np_)
+        ps1 = self.arr(ps1, (k, 8))
+        gps1 = self.arr(gps1, (k, 8)).copy()
+        c, _ = self._coll(rec)
+        self.L.orc_tool_collide_rev(self.P, ctypes.byref(c), _ptr(ps1), _ptr(gps1))
+        return gps1
 
     def arr(self, a, shape=None):
         a = np.ascontiguousarray(a, dtype=self.dtype)
@@ -232,12 +282,20 @@ class Oracle:
         ps_f = self.arr(ps_f, (max(1, self.np_), 8))
         pv_f = self.arr(pv_f, (max(1, self.np_), 7))
         ps_f1 = np.zeros_like(ps_f)
-        self.L.orc_substep(ctypes.byref(self.S), self.P, ctypes.c_int(st.shape[0]), _ptr(st), _ptr(out),
-                           _ptr(ps_f), _ptr(pv_f), _ptr(ps_f1), _ptr(self.grid), _ptr(self.gout))
+        if self.coll is None:
+            self.L.orc_substep(ctypes.byref(self.S), self.P, ctypes.c_int(st.shape[0]), _ptr(st), _ptr(out),
+                               _ptr(ps_f), _ptr(pv_f), _ptr(ps_f1), _ptr(self.grid), _ptr(self.gout))
+            self.last_coll = None
+        else:
+            c, rec = self._coll()
+            self.L.orc_substep_c(ctypes.byref(self.S), self.P, ctypes.c_int(st.shape[0]), _ptr(st), _ptr(out),
+                                 _ptr(ps_f), _ptr(pv_f), _ptr(ps_f1), _ptr(self.grid), _ptr(self.gout), ctypes.byref(c))
+            self.last_coll = rec
         return out, ps_f1
 
-    def substep_grad(self, st, adj1, ps_f, pv_f, ps_f1, gps_f1):
-        """Returns (adj0, gps_f contribution, gps_f1 (updated), gpv_f)."""
+    def substep_grad(self, st, adj1, ps_f, pv_f, ps_f1, gps_f1, coll=None):
+        """Returns (adj0, gps_f contribution, gps_f1 (updated), gpv_f).
+        coll: the collision record of this substep's forward (last_coll)."""
         st = self.arr(st)
         adj1 = self.arr(adj1)
         adj0 = np.zeros_like(st)
@@ -248,9 +306,15 @@ class Oracle:
         gps_f1 = self.arr(gps_f1, (k, 8)).copy()
         gps_f = np.zeros((k, 8), self.dtype)
         gpv_f = np.zeros((k, 7), self.dtype)
-        self.L.orc_substep_grad(ctypes.byref(self.S), self.P, ctypes.c_int(st.shape[0]), _ptr(st), _ptr(adj1),
-                                _ptr(adj0), _ptr(ps_f), _ptr(pv_f), _ptr(ps_f1), _ptr(gps_f), _ptr(gps_f1),
-                                _ptr(gpv_f), _ptr(self.grid), _ptr(self.gout), _ptr(self.ggout), _ptr(self.ggrid))
+        args = [ctypes.byref(self.S), self.P, ctypes.c_int(st.shape[0]), _ptr(st), _ptr(adj1), _ptr(adj0), _ptr(ps_f),
+                _ptr(pv_f), _ptr(ps_f1), _ptr(gps_f), _ptr(gps_f1), _ptr(gpv_f), _ptr(self.grid), _ptr(self.gout),
+                _ptr(self.ggout), _ptr(self.ggrid)]
+        if self.coll is None:
+            self.L.orc_substep_grad(*args)
+        else:
+            assert coll is not None, "substep_grad of a colliding scene needs the forward's collision record"
+            c, _ = self._coll(coll)
+            self.L.orc_substep_grad_c(*args, ctypes.byref(c))
         return adj0, gps_f, gps_f1, gpv_f
 
     def grid_mass(self, st):
@@ -395,6 +459,7 @@ class OracleRollout:
         self.frames = [o.arr(st0)]
         self.ps = [o.arr(ps0, (k, 8))]
         self.pv = []
+        self.colls = []
         dims = [int(p.get("action_dim", 0)) for p in o.prims_d]
         offs = np.concatenate([[0], np.cumsum(dims)]).astype(int)
         for s in range(H):
@@ -405,6 +470,7 @@ class OracleRollout:
             for i in range(S):
                 self.pv.append(pv_step.copy())
                 st1, ps1 = o.substep(self.frames[-1], self.ps[-1], pv_step)
+                self.colls.append(o.last_coll)
                 self.frames.append(st1)
                 self.ps.append(ps1)
         self.offs = offs
@@ -429,7 +495,8 @@ class OracleRollout:
         if T in frame_adj:
             adj = adj + frame_adj[T]
         for f in range(T - 1, -1, -1):
-            adj0, g0, g1, gv = o.substep_grad(self.frames[f], adj, self.ps[f], self.pv[f], self.ps[f + 1], gps[f + 1])
+            adj0, g0, g1, gv = o.substep_grad(self.frames[f], adj, self.ps[f], self.pv[f], self.ps[f + 1], gps[f + 1],
+                                              self.colls[f])
             gps[f] += g0
             gps[f + 1] = g1
             gpv[f] = gv

@@ -127,6 +127,41 @@ def test_primitives_container_api():
         Primitives([dict(shape="Torus")])
 
 
+def test_collision_pairs_and_rand_num():
+    """set_num_collision (mpm_simulator.py:69-80) on the PRIMITIVES of
+    plb/envs/gather_move.yml (Gripper and Lifter Box with collision_group
+    (0, 0, 1), Obstacle Box without) and lift_spread.yml (only the Box has
+    one); rand_num of initialize (mpm_simulator.py:87-96) is what
+    np.random.seed(42); np.random.uniform(-1.5, 1.5, ...) draws, and the
+    caller's global random state is left as it was."""
+    from diffskill_amd.engine.mpm_simulator import collision_pairs, collision_rand
+    from diffskill_amd.engine.primitive import Primitives
+    gather = Primitives([
+        dict(shape="Gripper", size=(0.015, 0.09, 0.05), init_gap=0.4, minimal_gap=0.05, collision_group="(0, 0, 1)",
+             action=dict(dim=7, scale=(0.015, 0.0, 0.015, 0.0, 0.0, 0.1, 0.03))),
+        dict(shape="Box", size=(0.07, 0.07, 0.02), collision_group="(0, 0, 1)", action=dict(dim=6, scale=(0.01,) * 6)),
+        dict(shape="Box", size=(0.2, 0.28, 0.04)),
+    ])
+    assert collision_pairs([p.cfg for p in gather]) == [(0, 2), (1, 2)]
+    lift = Primitives([
+        dict(shape="RollingPinExt", h=0.3, r=0.03, action=dict(dim=6, scale=(0.7, 0.005, 0.005, 0.005, 0., 0.))),
+        dict(shape="Box", size=(0.1, 0.1, 0.02), collision_group=(0, 0, 1), action=dict(dim=6, scale=(0.01,) * 6)),
+        dict(shape="Box", size=(0.2, 0.28, 0.07)),
+    ])
+    assert collision_pairs([p.cfg for p in lift]) == [(1, 2)]
+    # five primitives with the three-entry default: missing entries count as 0
+    five = [dict(shape="Sphere", radius=0.05)] * 4 + [dict(shape="Box", collision_group=(0, 0, 0, 1, 0))]
+    assert collision_pairs([p.cfg for p in Primitives(five)]) == [(4, 3)]
+    np.random.seed(123)
+    before = np.random.get_state()[1].copy()
+    r = collision_rand(2)
+    assert np.array_equal(np.random.get_state()[1], before)
+    np.random.seed(42)
+    ref = np.random.uniform(-1.5, 1.5, (2, 600, 3))
+    assert r.dtype == np.float32 and r.shape == (2, 600, 3)
+    np.testing.assert_array_equal(r, ref.astype(np.float32))
+
+
 def test_workloads_are_seeded_and_sized():
     from diffskill_amd import workloads
     w = workloads.config4()
