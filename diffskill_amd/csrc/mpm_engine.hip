@@ -101,12 +101,15 @@ struct PTrace {
     float sig[3], sc[3], eh[3], ehn, dg, e[3], J;
     bool yield;
 };
-DEV void particle_stress(const SceneC& S, const m3& C, const m3& F, PTrace& t) {
+// F_tmp = (I + dt C) F (mpm_simulator.py:121-124)
+DEV m3 f_tmp(const SceneC& S, const m3& C, const m3& F) {
     m3 A;
 #pragma unroll
     for (int i = 0; i < 9; i++) A.a[i] = ((i % 4 == 0) ? 1.f : 0.f) + S.dt * C.a[i];
-    t.Ftmp = mmul(A, F);
-    svd3(t.Ftmp, t.U, t.sig, t.V);
+    return mmul(A, F);
+}
+// everything after the SVD: return map and stress, with t.Ftmp, t.U, t.sig, t.V set
+DEV void stress_from_svd(const SceneC& S, const m3& C, PTrace& t) {
     float eps[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) { t.sc[i] = tmax(t.sig[i], 0.05f); eps[i] = logf(t.sc[i]); }
@@ -136,6 +139,13 @@ DEV void particle_stress(const SceneC& S, const m3& C, const m3& F, PTrace& t) {
         t.affine.a[i] = S.stress_k * s + S.p_mass * C.a[i];
     }
 }
+DEV void particle_stress(const SceneC& S, const m3& C, const m3& F, PTrace& t) {
+    t.Ftmp = f_tmp(S, C, F);
+    svd3(t.Ftmp, t.U, t.sig, t.V);
+    stress_from_svd(S, C, t);
+}
+// rows of the stored SVD of F_tmp per frame slot: U (row-major), sigma, V
+constexpr int USV_ROWS = 21;
 
 // grid_op for one node with m > 1e-12 (mpm_simulator.py:230-262)
 struct NodeTr { v3 vu[MAX_UNITS]; v3 vw[3]; };
@@ -912,10 +922,11 @@ DEV m3 backward_svd(const m3& gu, const float gs[3], const m3& gv, const m3& u, 
 
 // constitutive update in particle-slot order (coalesced): F_tmp, register SVD,
 // return map, stress (mpm_simulator.py:121-129,165-182,205-216).  Writes the
-// affine matrix stress_k * stress + p_mass * C of frame f and, when st_out is
-// given, F of frame f+1.
+// affine matrix stress_k * stress + p_mass * C of frame f, the SVD of F_tmp
+// (U, sigma, V: what the adjoint needs instead of a second SVD) and, when
+// st_out is given, F of frame f+1.
 __global__ void __launch_bounds__(PPB) k_stress(ChunkArgs ca, const float* __restrict__ st, float* __restrict__ st_out,
-                                                float* __restrict__ aff) {
+                                                float* __restrict__ aff, float* __restrict__ usv) {
     int4 ch = ca.chunks[blockIdx.x];
     int i = threadIdx.x;
     if (i >= ch.z) return;
@@ -927,6 +938,9 @@ __global__ void __launch_bounds__(PPB) k_stress(ChunkArgs ca, const float* __res
     particle_stress(S, C, F, t);
     if (st_out) store9(st_out, ca.stride, 15, slot, t.Fn);
     store9(aff, ca.stride, 0, slot, t.affine);
+    store9(usv, ca.stride, 0, slot, t.U);
+    store3(usv, ca.stride, 9, slot, mk(t.sig[0], t.sig[1], t.sig[2]));
+    store9(usv, ca.stride, 12, slot, t.V);
 }
 
 // p2g scatter over work items (mpm_simulator.py:217-225): momentum and mass of
@@ -1053,10 +1067,11 @@ __global__ void __launch_bounds__(WG) k_p2g_bwd_gather(TileArgs ta, const float*
 
 // second half of the p2g adjoint in particle-slot order: stress, return map,
 // backward_svd, F_tmp (mpm_simulator.py:121-156,205-216); writes the adjoints
-// of C and F of frame f.  Workgroup 0 also runs the primitive FK adjoint of
-// substep f.
+// of C and F of frame f.  The SVD of F_tmp is the one the forward stored for
+// frame f.  Workgroup 0 also runs the primitive FK adjoint of substep f.
 __global__ void __launch_bounds__(PPB) k_stress_bwd(ChunkArgs ca, const float* __restrict__ st, const float* __restrict__ gaff_in,
                                                     const float* __restrict__ a1, float* __restrict__ a0,
+                                                    const float* __restrict__ usv,
                                                     const PrimC* __restrict__ prims, int n_prims_total,
                                                     const float* __restrict__ pstate, const float* __restrict__ pvel,
                                                     float* __restrict__ gpstate, float* __restrict__ gpvel, int f) {
@@ -1074,7 +1089,12 @@ __global__ void __launch_bounds__(PPB) k_stress_bwd(ChunkArgs ca, const float* _
     m3 C = load9(st, stride, 6, p);
     m3 F = load9(st, stride, 15, p);
     PTrace t;
-    particle_stress(S, C, F, t);
+    t.Ftmp = f_tmp(S, C, F);
+    t.U = load9(usv, stride, 0, p);
+    v3 sg = load3(usv, stride, 9, p);
+    t.sig[0] = sg.x; t.sig[1] = sg.y; t.sig[2] = sg.z;
+    t.V = load9(usv, stride, 12, p);
+    stress_from_svd(S, C, t);
     m3 gaff = load9(gaff_in, stride, 0, p);
     m3 gC, gst;
 #pragma unroll
@@ -1492,6 +1512,7 @@ struct mpm_ctx {
     float* ckpt = nullptr;
     float* ringbuf = nullptr;
     float* aff = nullptr;   // affine matrix (9 rows) per frame slot: checkpoints then ring
+    float* usv = nullptr;   // SVD of F_tmp (U, sigma, V: 21 rows) per frame slot, valid with aff
     float* gaff = nullptr;  // adjoint of the affine matrix (9 rows), one frame
     std::vector<int> aff_of; // frame whose affine matrix each slot holds (-1: none)
     float* adj[2] = {nullptr, nullptr};
@@ -1548,6 +1569,7 @@ struct mpm_ctx {
     float* adj_ptr(int f) const { return adj[f & 1]; }
     int slot_of(int f) const { return (f % K == 0) ? f / K : n_ckpt + (f % K) - 1; }
     float* aff_ptr(int f) const { return aff + (size_t)slot_of(f) * 9 * stride; }
+    float* usv_ptr(int f) const { return usv + (size_t)slot_of(f) * USV_ROWS * stride; }
     int bslot_of(int f) const { return keep ? slot_of(f) : 0; }
     void drop_kept() { for (auto& k : kept_of) k = -1; }
     ChunkArgs cargs() const { ChunkArgs a; a.scenes = d_scenes; a.chunks = d_chunks; a.n_chunks = n_chunks; a.stride = stride; return a; }
@@ -1603,7 +1625,7 @@ struct PhaseTimer {
 };
 
 struct Layout {
-    size_t off_scenes, off_prims, off_chunks, off_blk_scene, off_ckpt, off_ring, off_aff, off_adj, off_grid, off_blkints, off_cells,
+    size_t off_scenes, off_prims, off_chunks, off_blk_scene, off_ckpt, off_ring, off_aff, off_usv, off_adj, off_grid, off_blkints, off_cells,
         off_pints, off_bslots, off_keep, off_tiles, off_counters, off_prim_bufs, total;
     // one bin slot: order, items, blk_stamp, blk_item0, blk_nitems, nblocks, nmap, counters
     size_t bs_order, bs_items, bs_stamp, bs_item0, bs_nitems, bs_nblocks, bs_nmap, bs_counters, bs_bytes, keep_bytes;
@@ -1648,6 +1670,7 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     L.off_ckpt = o; o = align256(o + ff * L.n_ckpt);
     L.off_ring = o; o = align256(o + ff * L.ring);
     L.off_aff = o; o = align256(o + sizeof(float) * 9 * (size_t)stride * (L.n_ckpt + L.ring + 1));
+    L.off_usv = o; o = align256(o + sizeof(float) * USV_ROWS * (size_t)stride * (L.n_ckpt + L.ring));
     L.off_adj = o; o = align256(o + ff * 2);
     L.off_grid = o; o = align256(o + sizeof(float4) * (size_t)L.n_nodes * 3);
     L.off_blkints = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 3);   // block flags, node flags, occupied list
@@ -1724,7 +1747,7 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
         HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     }
     { PhaseTimer t(c, MPM_PH_STRESS, sst);
-      hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, sst, c->cargs(), in, out, aff); }
+      hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, sst, c->cargs(), in, out, aff, c->usv_ptr(f)); }
     if (fork) HIPCHK(hipEventRecord(c->ev_join, c->side));
     c->aff_of[c->slot_of(f)] = f;
     if (run_bin(c, f, 1, st)) return -1;
@@ -1778,9 +1801,9 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     GridArgs ga = c->gargs(f, bs, false);
     float* aff = c->aff_ptr(f);
     if (c->aff_of[c->slot_of(f)] != f) {
-        // the affine matrix of frame f is not stored (recomputed segment ended at f)
+        // the affine matrix and SVD of frame f are not stored (recomputed segment ended at f)
         PhaseTimer t(c, MPM_PH_STRESS, st);
-        hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, (float*)nullptr, aff);
+        hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, (float*)nullptr, aff, c->usv_ptr(f));
         c->aff_of[c->slot_of(f)] = f;
     }
     // rebuild the grid when it was not kept; a kept grid whose active list
@@ -1811,8 +1834,8 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     { PhaseTimer t(c, MPM_PH_P2G_BWD, st);
       hipLaunchKernelGGL(k_p2g_bwd_gather, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, a0, c->gaff, c->g_adjpm); }
     { PhaseTimer t(c, MPM_PH_STRESS_BWD, st);
-      hipLaunchKernelGGL(k_stress_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, (const float*)c->gaff, a1, a0, c->d_prims,
-                         c->n_prims, c->pstate, c->pvel, c->pstate_g, c->pvel_g, f); }
+      hipLaunchKernelGGL(k_stress_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, (const float*)c->gaff, a1, a0,
+                         (const float*)c->usv_ptr(f), c->d_prims, c->n_prims, c->pstate, c->pvel, c->pstate_g, c->pvel_g, f); }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1892,6 +1915,7 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->aff = (float*)(w + L.off_aff);
     c->gaff = c->aff + (size_t)9 * L.stride * (L.n_ckpt + L.ring);
     c->aff_of.assign(L.n_ckpt + L.ring, -1);
+    c->usv = (float*)(w + L.off_usv);
     c->adj[0] = (float*)(w + L.off_adj);
     c->adj[1] = c->adj[0] + c->frame_floats();
     float4* g = (float4*)(w + L.off_grid);
