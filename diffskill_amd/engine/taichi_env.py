@@ -62,10 +62,18 @@ class TaichiEnv:
 
     def initialize(self, cfg=None, target_path=None):
         """taichi_env.py:103-121."""
+        if self.has_loss and target_path is not None:
+            self.load_target_x(target_path)
         self.n_particles = len(self.init_particles)
         self.simulator.initialize(self.n_particles)
         self.primitives.initialize()
         self.simulator.reset(self.init_particles)
+
+    def load_target_x(self, path):
+        """taichi_env.py:123-125: the target particle positions of a dataset's
+        ``target/target_{i}.npy`` (read without unpickling)."""
+        self.target_x = np.load(path, allow_pickle=False)
+        self.tensor_target_x = torch.as_tensor(self.target_x, dtype=torch.float32, device=self.device)
 
     def step(self, action=None):
         if action is not None:
