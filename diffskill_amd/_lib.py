@@ -49,6 +49,7 @@ EXPORTS = [
     "mpm_update_prims", "mpm_copy_frame", "mpm_set_profiling", "mpm_profile_read", "mpm_last_error",
     "mpm_set_launch", "mpm_get_launch", "mpm_min_dist_cols", "mpm_min_dist", "mpm_min_dist_bwd",
     "mpm_collision_bytes", "mpm_set_collisions", "mpm_get_collisions",
+    "pc_nearest", "pc_last_error",
 ]
 
 PHASES = ["bin", "compact", "blocks", "scatter", "stress", "p2g", "grid_op", "g2p", "p2g_grid", "grid_op_keep",
@@ -103,6 +104,8 @@ def load():
         "mpm_get_collisions": (i32, [vp, i32, vp, vp, vp, vp]),
         "mpm_profile_read": (i32, [vp, vp, vp, i32]),
         "mpm_last_error": (ctypes.c_char_p, []),
+        "pc_nearest": (i32, [vp, i32, vp, i32, vp, vp, vp, vp]),
+        "pc_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

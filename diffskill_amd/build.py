@@ -4,7 +4,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", "mpm_engine.hip")]
+SRC = [os.path.join(HERE, "csrc", "mpm_engine.hip"), os.path.join(HERE, "csrc", "pointcloud.hip")]
 OUT = os.path.join(HERE, "libdiffskill_mpm.so")
 ARCH = os.environ.get("DSK_OFFLOAD_ARCH", "gfx950")
 

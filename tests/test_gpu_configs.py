@@ -132,19 +132,8 @@ def test_config2_action_grads_full_horizon():
     assert rel(g, g_ref) < max(1e-3, 1.5 * rel(g32, g_ref))
 
 
-def chamfer_torch(x, t, chunk=4096):
-    """One-way Chamfer L2: mean over particles of the squared distance to the
-    nearest target point."""
-    d = []
-    for i in range(0, x.shape[0], chunk):
-        xi = x[i:i + chunk]
-        d2 = torch.cdist(xi.detach(), t) ** 2
-        j = d2.argmin(dim=1)
-        d.append(((xi - t[j]) ** 2).sum(dim=1))
-    return torch.cat(d).mean()
-
-
 def _config3_grads(w, ckpt_k, H):
+    from diffskill_amd.pointcloud import chamfer_distance
     env, gm = gradmodel(w, ckpt_k=ckpt_k, output_grid=False)
     obs = gm.reset()
     acts = flat_actions(w)[:H]
@@ -152,10 +141,11 @@ def _config3_grads(w, ckpt_k, H):
     for s in range(H):
         obs = gm.forward(s, a[s], *obs)
     t = torch.as_tensor(w["target_points"], dtype=torch.float32, device="cuda")
-    loss = chamfer_torch(obs[0][:, :3], t)
+    # one-way Chamfer L2 (mean squared distance to the nearest target) on the GPU
+    loss = chamfer_distance(obs[0][:, :3], t)
     loss.backward()
     ws = env.simulator.engine.ws.numel()
-    return float(loss), a.grad.detach().cpu().double().numpy(), ws
+    return float(loss.detach()), a.grad.detach().cpu().double().numpy(), ws
 
 
 def test_config3_checkpointed_equals_store_all():
