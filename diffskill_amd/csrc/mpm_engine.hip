@@ -456,10 +456,11 @@ DEV float tsum32(float* r, int lane) {
 }
 DEV int tile_index(int l0, int l1, int l2, int n) { return ((l0 + n / 9) * 6 + (l1 + (n / 3) % 3)) * 6 + (l2 + n % 3); }
 
-// Segmented scatter of 27 x 4 per-particle node values into an LDS tile
+// Segmented scatter of 27 x NCOMP per-particle node values into an LDS tile
 // (component-major, TILE floats per component).  Lanes that share a cell are
 // summed through the transpose tree (at most MAXSEG distinct cells per wave),
-// the rest add directly.  VAL(lane_node, comp) gives a lane's value.
+// the rest add directly.  VAL(lane_node, comp) gives a lane's value.  Values
+// are packed densely, v = node * NCOMP + comp, 32 per transpose pass.
 template <int NCOMP, class ValFn>
 DEV void scatter_tile(float* tile, bool valid, int key, int l0, int l1, int l2, int lane, ValFn val) {
     unsigned long long act = __ballot(valid);
@@ -479,18 +480,18 @@ DEV void scatter_tile(float* tile, bool valid, int key, int l0, int l1, int l2, 
         unsigned long long m = __ballot(valid && key == key0);
         bool in = (m >> lane) & 1ull;
         int s0 = __shfl(l0, lead), s1 = __shfl(l1, lead), s2 = __shfl(l2, lead);
+        constexpr int NV = 27 * NCOMP;
 #pragma unroll
-        for (int pass = 0; pass < 4; pass++) {
+        for (int pass = 0; pass < (NV + 31) / 32; pass++) {
             float r[32];
 #pragma unroll
             for (int j = 0; j < 32; j++) {
-                int n = pass * 7 + j / 4, c = j & 3;
-                r[j] = (in && j < 28 && n < 27 && c < NCOMP) ? val(n, c) : 0.f;
+                int v = pass * 32 + j;
+                r[j] = (in && v < NV) ? val(v / NCOMP, v % NCOMP) : 0.f;
             }
             float s = tsum32(r, lane);
-            int j = lane >> 1;
-            int n = pass * 7 + j / 4, c = j & 3;
-            if (!(lane & 1) && j < 28 && n < 27 && c < NCOMP) atomicAdd(&tile[c * TILE + tile_index(s0, s1, s2, n)], s);
+            int v = pass * 32 + (lane >> 1);
+            if (!(lane & 1) && v < NV) atomicAdd(&tile[(v % NCOMP) * TILE + tile_index(s0, s1, s2, v / NCOMP)], s);
         }
         act &= ~m;
         seg++;
@@ -946,7 +947,7 @@ __global__ void __launch_bounds__(PPB) k_stress(ChunkArgs ca, const float* __res
 // p2g scatter over work items (mpm_simulator.py:217-225): momentum and mass of
 // every particle of the item into the block's LDS tile, flushed to the item's
 // tile buffer.  Reads x, v and the stored affine matrix.
-__global__ void __launch_bounds__(WG) k_p2g_scatter(TileArgs ta, const float* __restrict__ st, const float* __restrict__ aff,
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_p2g_scatter(TileArgs ta, const float* __restrict__ st, const float* __restrict__ aff,
                                                     float4* __restrict__ tiles, int only_overflow) {
     __shared__ float tile[4 * TILE];
     if (only_overflow && kept_fits(ta.counters, ta.keep_cap)) return;
