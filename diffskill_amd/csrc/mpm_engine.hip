@@ -659,14 +659,21 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __res
     int stride = ta.stride;
     for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
         int4 it = ta.items[e];
+        // the first batch's positions do not depend on the tile: their loads
+        // are issued first, so both latencies overlap
+        int k = threadIdx.x;
+        int p = ta.order[it.y + min(k, it.z - 1)];
+        v3 x = load3(st, stride, 0, p);
         const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
         int bi, bj, bk;
         block_origin(S, it.x, bi, bj, bk);
         load_tile(tv, S, bi, bj, bk, g_v);
         __syncthreads();
-        for (int k = threadIdx.x; k < it.z; k += WG) {
-            int p = ta.order[it.y + k];
-            v3 x = load3(st, stride, 0, p);
+        for (; k < it.z; k += WG) {
+            if (k >= WG) {
+                p = ta.order[it.y + k];
+                x = load3(st, stride, 0, p);
+            }
             Stencil sw;
             stencil(S, x, sw);
             int l0 = sw.base[0] - 4 * bi, l1 = sw.base[1] - 4 * bj, l2 = sw.base[2] - 4 * bk;
@@ -706,7 +713,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __res
 // reverse of g2p over work items: scatter the adjoint of grid_v_out into the
 // item's tile buffer, write the first part of the adjoint of x (frame f).
 // a1 = adjoint of frame f+1, st1 = frame f+1.
-__global__ void __launch_bounds__(WG) k_g2p_bwd_tile(TileArgs ta, const float* __restrict__ st, const float* __restrict__ st1,
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_g2p_bwd_tile(TileArgs ta, const float* __restrict__ st, const float* __restrict__ st1,
                                                      const float* __restrict__ a1, float* __restrict__ a0,
                                                      const float4* __restrict__ g_v, float4* __restrict__ tiles_v, int use_kept) {
     __shared__ float4 tv[TILE];
@@ -717,6 +724,14 @@ __global__ void __launch_bounds__(WG) k_g2p_bwd_tile(TileArgs ta, const float* _
     int stride = ta.stride;
     for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
         int4 it = ta.items[e];
+        // the first batch's particle data does not depend on the tile: its
+        // loads are issued first, so both latencies overlap
+        int p = ta.order[it.y + min((int)threadIdx.x, it.z - 1)];
+        v3 x = load3(st, stride, 0, p);
+        v3 x1 = load3(st1, stride, 0, p);
+        v3 gx1 = load3(a1, stride, 0, p);
+        v3 gv1 = load3(a1, stride, 3, p);
+        m3 gC = load9(a1, stride, 6, p);
         const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
         int bi, bj, bk;
         block_origin(S, it.x, bi, bj, bk);
@@ -727,12 +742,14 @@ __global__ void __launch_bounds__(WG) k_g2p_bwd_tile(TileArgs ta, const float* _
         for (int k0 = 0; k0 < it.z; k0 += WG) {
             int k = k0 + threadIdx.x;
             bool valid = k < it.z;
-            int p = valid ? ta.order[it.y + k] : ta.order[it.y];
-            v3 x = load3(st, stride, 0, p);
-            v3 x1 = load3(st1, stride, 0, p);
-            v3 gx1 = load3(a1, stride, 0, p);
-            v3 gv1 = load3(a1, stride, 3, p);
-            m3 gC = load9(a1, stride, 6, p);
+            if (k0 > 0) {
+                p = valid ? ta.order[it.y + k] : ta.order[it.y];
+                x = load3(st, stride, 0, p);
+                x1 = load3(st1, stride, 0, p);
+                gx1 = load3(a1, stride, 0, p);
+                gv1 = load3(a1, stride, 3, p);
+                gC = load9(a1, stride, 6, p);
+            }
             // clamp mask from the stored output position: the adjoint passes iff lower < x' < upper
             v3 gxn = mk((S.lower < x1.x && x1.x < S.upper) ? gx1.x : 0.f, (S.lower < x1.y && x1.y < S.upper) ? gx1.y : 0.f,
                         (S.lower < x1.z && x1.z < S.upper) ? gx1.z : 0.f);
@@ -1003,16 +1020,25 @@ __global__ void __launch_bounds__(WG) k_p2g_bwd_gather(TileArgs ta, const float*
     int stride = ta.stride;
     for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
         int4 it = ta.items[e];
+        // the first batch's particle data does not depend on the tile: its
+        // loads are issued first, so both latencies overlap
+        int k = threadIdx.x;
+        int p = ta.order[it.y + min(k, it.z - 1)];
+        v3 x = load3(st, stride, 0, p);
+        v3 v = load3(st, stride, 3, p);
+        m3 Am = load9(aff, stride, 0, p);
         const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
         int bi, bj, bk;
         block_origin(S, it.x, bi, bj, bk);
         load_tile(tg, S, bi, bj, bk, g_adjpm);
         __syncthreads();
-        for (int k = threadIdx.x; k < it.z; k += WG) {
-            int p = ta.order[it.y + k];
-            v3 x = load3(st, stride, 0, p);
-            v3 v = load3(st, stride, 3, p);
-            m3 Am = load9(aff, stride, 0, p);
+        for (; k < it.z; k += WG) {
+            if (k >= WG) {
+                p = ta.order[it.y + k];
+                x = load3(st, stride, 0, p);
+                v = load3(st, stride, 3, p);
+                Am = load9(aff, stride, 0, p);
+            }
             const float* A = Am.a;
             Stencil sw;
             stencil(S, x, sw);
