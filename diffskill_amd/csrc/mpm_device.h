@@ -218,8 +218,9 @@ DEV void givens_qr(T& a, T& b, T& c, T& s) {
     T r = sqrt(a * a + b * b);
     if (r > T(1e-30)) { c = a / r; s = b / r; } else { c = T(1); s = T(0); }
 }
-// Jacobi on A^T A (5 cyclic sweeps) for V, column sort, Givens QR of A V for
-// U and the singular values; computed in T, returned in fp32.
+// Jacobi on A^T A (at most 5 cyclic sweeps, fewer once the off-diagonal
+// part is below 1e-15 of the diagonal) for V, column sort, Givens QR of A V
+// for U and the singular values; computed in T, returned in fp32.
 template <typename T>
 DEV void svd3_t(const m3& Af, m3& U, float sig[3], m3& V) {
     T A[9];
@@ -242,6 +243,9 @@ DEV void svd3_t(const m3& Af, m3& U, float sig[3], m3& V) {
         jacobi_rot(s00, s22, s02, s01, s12, v00, v10, v20, v02, v12, v22);
         // (p,q) = (1,2), r = 0
         jacobi_rot(s11, s22, s12, s01, s02, v01, v11, v21, v02, v12, v22);
+        // converged: further rotations change nothing at this precision
+        T off = s01 * s01 + s02 * s02 + s12 * s12, on = s00 * s00 + s11 * s11 + s22 * s22;
+        if (off <= T(1e-30) * on) break;
     }
     // B = A V
     T b[9];
