@@ -10,12 +10,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdiffskill_mpm.so")
 
 MPM_SPHERE, MPM_CAPSULE, MPM_ROLLINGPIN, MPM_ROLLINGPIN_EXT, MPM_BOX, MPM_GRIPPER, \
-    MPM_PRISM, MPM_KNIFE, MPM_CHOPSTICKS, MPM_GRIPPER2 = range(10)
+    MPM_PRISM, MPM_KNIFE, MPM_CHOPSTICKS, MPM_GRIPPER2, MPM_CYLINDER, MPM_TORUS = range(12)
 
 SHAPE_CODES = {
     "Sphere": MPM_SPHERE, "Capsule": MPM_CAPSULE, "RollingPin": MPM_ROLLINGPIN,
     "RollingPinExt": MPM_ROLLINGPIN_EXT, "Box": MPM_BOX, "Gripper": MPM_GRIPPER,
     "Prism": MPM_PRISM, "Knife": MPM_KNIFE, "Chopsticks": MPM_CHOPSTICKS, "Gripper2": MPM_GRIPPER2,
+    "Cylinder": MPM_CYLINDER, "Torus": MPM_TORUS,
 }
 
 

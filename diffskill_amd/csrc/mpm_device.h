@@ -39,8 +39,8 @@ struct PrimC {
 };
 
 enum { P_SPHERE = 0, P_CAPSULE = 1, P_ROLLINGPIN = 2, P_ROLLINGPIN_EXT = 3, P_BOX = 4, P_GRIPPER = 5,
-       P_PRISM = 6, P_KNIFE = 7, P_CHOPSTICKS = 8, P_GRIPPER2 = 9 };
-enum { SH_SPHERE = 0, SH_CAPSULE = 1, SH_BOX = 2, SH_PRISM = 3, SH_KNIFE = 4, SH_CHOP = 5 };
+       P_PRISM = 6, P_KNIFE = 7, P_CHOPSTICKS = 8, P_GRIPPER2 = 9, P_CYLINDER = 10, P_TORUS = 11 };
+enum { SH_SPHERE = 0, SH_CAPSULE = 1, SH_BOX = 2, SH_PRISM = 3, SH_KNIFE = 4, SH_CHOP = 5, SH_CYL = 6, SH_TORUS = 7 };
 
 constexpr float LEN_EPS_SHAPE = 1e-14f;  // primitives.py:14-15
 constexpr float LEN_EPS_BASE = 1e-8f;    // primitive/utils.py:5-6
@@ -403,6 +403,106 @@ DEV float chop_sdf(const PrimC& P, float gap, v3 p, v3* g, float* ggap) {
     }
     return tmin(a, b);
 }
+// VJP of v / length(v) (length with the 1e-14 epsilon), 2-D and 3-D
+DEV void nrm2_rev(float v0, float v1, float go0, float go1, float* gv0, float* gv1) {
+    float L = sqrtf(v0 * v0 + v1 * v1 + LEN_EPS_SHAPE), d = v0 * go0 + v1 * go1, L3 = L * L * L;
+    *gv0 += go0 / L - v0 * d / L3;
+    *gv1 += go1 / L - v1 * d / L3;
+}
+DEV v3 nrm3_rev(v3 v, v3 go) {
+    float L = sqrtf(dot(v, v) + LEN_EPS_SHAPE), d = dot(v, go), L3 = L * L * L;
+    return mk(go.x / L - v.x * d / L3, go.y / L - v.y * d / L3, go.z / L - v.z * d / L3);
+}
+// Cylinder._sdf (primitives.py:300-304): 2-D box SDF of (|(x, z)|, |y|)
+// against (h, r): h is the disk radius, r the half height along y
+DEV float cylinder_sdf(float h, float r, v3 p, v3* g) {
+    float a = sqrtf(p.x * p.x + p.z * p.z + LEN_EPS_SHAPE);
+    float d0 = fabsf(a) - h, d1 = fabsf(p.y) - r;
+    float m = tmax(d0, d1);
+    float e0 = tmax(d0, 0.f), e1 = tmax(d1, 0.f);
+    float L = sqrtf(e0 * e0 + e1 * e1 + LEN_EPS_SHAPE);
+    if (g) {
+        float gd0 = (0.f < d0) ? e0 / L : 0.f, gd1 = (0.f < d1) ? e1 / L : 0.f;
+        if (m < 0.f) { if (d1 < d0) gd0 += 1.f; else gd1 += 1.f; }
+        float ga = gd0 * sgnf(a);
+        *g = mk(ga * p.x / a, gd1 * sgnf(p.y), ga * p.z / a);
+    }
+    return tmin(m, 0.f) + L;
+}
+// Cylinder._normal (primitives.py:306-320), analytic; the forward values the
+// VJP needs are recomputed by cyl_normal_tr
+struct CylTr { float l, d0, d1, n20, n21, m0, m1, q0, q1, s; v3 n3; };
+DEV CylTr cyl_normal_tr(float h, float r, v3 p) {
+    CylTr t;
+    t.l = sqrtf(p.x * p.x + p.z * p.z + LEN_EPS_SHAPE);
+    t.d0 = t.l - h;
+    t.d1 = fabsf(p.y) - r;
+    float f = t.d0 > t.d1 ? 1.f : 0.f, c = tmax(t.d0, t.d1) <= 0.f ? 1.f : 0.f;
+    t.n20 = tmax(t.d0, 0.f) + c * f;
+    t.n21 = tmax(t.d1, 0.f) + c * (1.f - f);
+    float L2 = sqrtf(t.n20 * t.n20 + t.n21 * t.n21 + LEN_EPS_SHAPE);
+    t.m0 = t.n20 / L2;
+    t.m1 = t.n21 / L2;
+    t.q0 = p.x / t.l;
+    t.q1 = p.z / t.l;
+    t.s = p.y >= 0.f ? 1.f : -1.f;
+    t.n3 = mk(t.q0 * t.m0, t.m1 * t.s, t.q1 * t.m0);
+    return t;
+}
+DEV v3 cylinder_normal(float h, float r, v3 p) {
+    CylTr t = cyl_normal_tr(h, r, p);
+    float L = sqrtf(dot(t.n3, t.n3) + LEN_EPS_SHAPE);
+    return mk(t.n3.x / L, t.n3.y / L, t.n3.z / L);
+}
+DEV v3 cylinder_normal_rev(float h, float r, v3 p, v3 gn) {
+    CylTr t = cyl_normal_tr(h, r, p);
+    v3 gn3 = nrm3_rev(t.n3, gn);
+    float gq0 = gn3.x * t.m0, gq1 = gn3.z * t.m0;
+    float gm0 = gn3.x * t.q0 + gn3.z * t.q1, gm1 = gn3.y * t.s;
+    float gn20 = 0.f, gn21 = 0.f;
+    nrm2_rev(t.n20, t.n21, gm0, gm1, &gn20, &gn21);
+    float gd0 = (0.f < t.d0) ? gn20 : 0.f, gd1 = (0.f < t.d1) ? gn21 : 0.f;
+    // q = (x, z) / l, d0 = l - h, d1 = |y| - r
+    float gl = gd0 - (gq0 * p.x + gq1 * p.z) / (t.l * t.l);
+    return mk(gq0 / t.l + gl * p.x / t.l, gd1 * sgnf(p.y), gq1 / t.l + gl * p.z / t.l);
+}
+// Torus._sdf (primitives.py:336-339): tx the ring radius, ty the tube radius
+DEV float torus_sdf(float tx, float ty, v3 p, v3* g) {
+    float a = sqrtf(p.x * p.x + p.z * p.z + LEN_EPS_SHAPE);
+    float q0 = a - tx, q1 = p.y;
+    float L = sqrtf(q0 * q0 + q1 * q1 + LEN_EPS_SHAPE);
+    if (g) {
+        float ga = q0 / L;
+        *g = mk(ga * p.x / a, q1 / L, ga * p.z / a);
+    }
+    return L - ty;
+}
+// Torus._normal (primitives.py:341-350), analytic
+DEV v3 torus_normal_n3(float tx, v3 p, float* l, float* q0, float* m0, float* m1) {
+    *l = sqrtf(p.x * p.x + p.z * p.z + LEN_EPS_SHAPE);
+    *q0 = *l - tx;
+    float L = sqrtf(*q0 * *q0 + p.y * p.y + LEN_EPS_SHAPE);
+    *m0 = *q0 / L;
+    *m1 = p.y / L;
+    return mk(p.x / *l * *m0, *m1, p.z / *l * *m0);
+}
+DEV v3 torus_normal(float tx, v3 p) {
+    float l, q0, m0, m1;
+    v3 n3 = torus_normal_n3(tx, p, &l, &q0, &m0, &m1);
+    float L = sqrtf(dot(n3, n3) + LEN_EPS_SHAPE);
+    return mk(n3.x / L, n3.y / L, n3.z / L);
+}
+DEV v3 torus_normal_rev(float tx, v3 p, v3 gn) {
+    float l, q0, m0, m1;
+    v3 n3 = torus_normal_n3(tx, p, &l, &q0, &m0, &m1);
+    v3 gn3 = nrm3_rev(n3, gn);
+    float gx0 = gn3.x * m0, gx1 = gn3.z * m0;
+    float gm0 = gn3.x * p.x / l + gn3.z * p.z / l, gm1 = gn3.y;
+    float gq0 = 0.f, gy = 0.f;
+    nrm2_rev(q0, p.y, gm0, gm1, &gq0, &gy);
+    float gl = gq0 - (gx0 * p.x + gx1 * p.z) / (l * l);
+    return mk(gx0 / l + gl * p.x / l, gy, gx1 / l + gl * p.z / l);
+}
 DEV float local_sdf(const PrimC& P, int sh, float gap, v3 p, v3* g, float* ggap) {
     if (ggap) *ggap = 0.f;
     switch (sh) {
@@ -417,6 +517,8 @@ DEV float local_sdf(const PrimC& P, int sh, float gap, v3 p, v3* g, float* ggap)
         return tmax(a, b);
     }
     case SH_CHOP: return chop_sdf(P, gap, p, g, ggap);
+    case SH_CYL: return cylinder_sdf(P.h, P.r, p, g);
+    case SH_TORUS: return torus_sdf(P.h, P.r, p, g);
     default: {  // Sphere.sdf (primitives.py:28-30), world offset p = x - position
         float L = sqrtf(dot(p, p) + LEN_EPS_SHAPE);
         if (g) *g = mk(p.x / L, p.y / L, p.z / L);
@@ -524,10 +626,13 @@ DEV v3 fd_normal_rev_d(const PrimC& P, int sh, v3 pf, v3 gnf) {
     return mk((float)gp[0], (float)gp[1], (float)gp[2]);
 }
 
-// normals: analytic for sphere / capsule / chopsticks, central FD with the
-// float32 step 1e-4 for box / prism / knife (primitives.py:382-393,720-731,796-807)
+// normals: analytic for sphere / capsule / chopsticks / cylinder / torus,
+// central FD with the float32 step 1e-4 for box / prism / knife
+// (primitives.py:382-393,720-731,796-807)
 DEV v3 local_normal(const PrimC& P, int sh, float gap, v3 p) {
     if (sh == SH_CAPSULE) return capsule_normal(P.h, p);
+    if (sh == SH_CYL) return cylinder_normal(P.h, P.r, p);
+    if (sh == SH_TORUS) return torus_normal(P.h, p);
     if (sh == SH_SPHERE) { float L = sqrtf(dot(p, p) + LEN_EPS_SHAPE); return mk(p.x / L, p.y / L, p.z / L); }
     if (sh == SH_CHOP) {
         v3 pp = mk(p.x, p.y + P.h / 2.f, p.z);
@@ -539,6 +644,8 @@ DEV v3 local_normal(const PrimC& P, int sh, float gap, v3 p) {
 }
 DEV v3 local_normal_rev(const PrimC& P, int sh, float gap, v3 p, v3 gn, float* ggap) {
     if (sh == SH_CAPSULE) return capsule_normal_rev(P.h, p, gn);
+    if (sh == SH_CYL) return cylinder_normal_rev(P.h, P.r, p, gn);
+    if (sh == SH_TORUS) return torus_normal_rev(P.h, p, gn);
     if (sh == SH_SPHERE) {
         float L = sqrtf(dot(p, p) + LEN_EPS_SHAPE), dd = dot(p, gn), L3 = L * L * L;
         return mk(gn.x / L - p.x * dd / L3, gn.y / L - p.y * dd / L3, gn.z / L - p.z * dd / L3);

@@ -1879,6 +1879,8 @@ PrimC to_primc(const mpm_prim_desc& d) {
     case MPM_PRISM: p.unit_shape = SH_PRISM; break;
     case MPM_KNIFE: p.unit_shape = SH_KNIFE; break;
     case MPM_CHOPSTICKS: p.unit_shape = SH_CHOP; break;
+    case MPM_CYLINDER: p.unit_shape = SH_CYL; break;
+    case MPM_TORUS: p.unit_shape = SH_TORUS; break;
     default: p.unit_shape = SH_BOX;
     }
     p.n_units = (d.type == MPM_GRIPPER || d.type == MPM_GRIPPER2) ? 2 : 1;

@@ -307,17 +307,33 @@ class Knife(Primitive):
 
 
 class Cylinder(Primitive):
+    """Cylinder (primitives.py:293-326): h the disk radius, r the half height."""
     shape = "Cylinder"
 
-    def __init__(self, *a, **k):
-        raise NotImplementedError("Cylinder primitives are not implemented by the MI355X engine")
+    @classmethod
+    def default_config(cls):
+        c = Primitive.default_config()
+        c.h = 0.2
+        c.r = 0.1
+        return c
+
+    def shape_params(self):
+        return dict(h=float(self.cfg.h), r=float(self.cfg.r))
 
 
 class Torus(Primitive):
+    """Torus (primitives.py:329-356): tx the ring radius, ty the tube radius."""
     shape = "Torus"
 
-    def __init__(self, *a, **k):
-        raise NotImplementedError("Torus primitives are not implemented by the MI355X engine")
+    @classmethod
+    def default_config(cls):
+        c = Primitive.default_config()
+        c.tx = 0.2
+        c.ty = 0.1
+        return c
+
+    def shape_params(self):
+        return dict(h=float(self.cfg.tx), r=float(self.cfg.ty))
 
 
 CLASSES = {c.shape: c for c in (Sphere, Capsule, RollingPin, RollingPinExt, Chopsticks, Box, Gripper, Gripper2,
@@ -332,8 +348,10 @@ class Primitives:
         self.action_dims = [0]
         for c in cfgs:
             c = CfgNode(c) if not isinstance(c, CfgNode) else c
-            cls = CLASSES[c.shape]
-            p = cls(cfg=c, max_timesteps=max_timesteps)
+            if c.shape not in CLASSES:
+                raise NotImplementedError("primitive shape %r is not one of the reference's classes %s"
+                                          % (c.shape, sorted(CLASSES)))
+            p = CLASSES[c.shape](cfg=c, max_timesteps=max_timesteps)
             self.primitives.append(p)
             self.action_dims.append(self.action_dims[-1] + p.action_dim)
         self.n = len(self.primitives)

@@ -51,7 +51,7 @@ extern "C" {
 enum {
     MPM_SPHERE = 0, MPM_CAPSULE = 1, MPM_ROLLINGPIN = 2, MPM_ROLLINGPIN_EXT = 3,
     MPM_BOX = 4, MPM_GRIPPER = 5, MPM_PRISM = 6, MPM_KNIFE = 7,
-    MPM_CHOPSTICKS = 8, MPM_GRIPPER2 = 9
+    MPM_CHOPSTICKS = 8, MPM_GRIPPER2 = 9, MPM_CYLINDER = 10, MPM_TORUS = 11
 };
 
 /* One scene: the SIMULATOR constants of MPMSimulator.__init__
@@ -74,7 +74,8 @@ typedef struct {
     int action_dim;
     double friction, softness;
     double size[3];        /* Box/Gripper half extents; Knife box size */
-    double h, r;           /* Capsule family h (segment length), r; Sphere radius in r */
+    double h, r;           /* Capsule family h (segment length), r; Sphere radius in r;
+                              Cylinder h (disk radius), r (half height); Torus tx in h, ty in r */
     double ph[2];          /* Prism / Knife prism h */
     double prot[4];        /* Prism / Knife prism rotation (w, x, y, z) */
     double lo[3], hi[3];   /* xyz_limit */

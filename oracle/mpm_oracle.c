@@ -62,7 +62,7 @@ typedef struct {
 enum {
     ORC_SPHERE = 0, ORC_CAPSULE = 1, ORC_ROLLINGPIN = 2, ORC_ROLLINGPIN_EXT = 3,
     ORC_BOX = 4, ORC_GRIPPER = 5, ORC_PRISM = 6, ORC_KNIFE = 7,
-    ORC_CHOPSTICKS = 8, ORC_GRIPPER2 = 9
+    ORC_CHOPSTICKS = 8, ORC_GRIPPER2 = 9, ORC_CYLINDER = 10, ORC_TORUS = 11
 };
 
 typedef struct {
@@ -70,7 +70,8 @@ typedef struct {
     int action_dim;
     double friction, softness;
     double size[3];   /* Box / Gripper half extents; Knife box size */
-    double h, r;      /* Capsule family: segment length h, radius r; Sphere radius r */
+    double h, r;      /* Capsule family: segment length h, radius r; Sphere radius r;
+                         Cylinder h (disk radius), r (half height); Torus tx in h, ty in r */
     double ph[2];     /* Prism / Knife prism h */
     double prot[4];   /* Prism / Knife prism rotation (w,x,y,z) */
     double lo[3], hi[3];  /* xyz_limit */
@@ -328,8 +329,109 @@ static real knife_sdf(const orc_prim *P, const real *p, real *g) {
     return tmax(a, b);
 }
 
+/* v / length(v), length with the 1e-14 epsilon, and its VJP (adds into gv) */
+static void nrm2(const real *v, real *o) {
+    real L = sqrt(v[0] * v[0] + v[1] * v[1] + LEN_EPS_SHAPE);
+    o[0] = v[0] / L; o[1] = v[1] / L;
+}
+static void nrm2_rev(const real *v, const real *go, real *gv) {
+    real L = sqrt(v[0] * v[0] + v[1] * v[1] + LEN_EPS_SHAPE), d = v[0] * go[0] + v[1] * go[1];
+    for (int i = 0; i < 2; i++) gv[i] += go[i] / L - v[i] * d / (L * L * L);
+}
+static void nrm3_rev(const real *v, const real *go, real *gv) {
+    real L = sqrt(dot3(v, v) + LEN_EPS_SHAPE), d = dot3(v, go);
+    for (int i = 0; i < 3; i++) gv[i] += go[i] / L - v[i] * d / (L * L * L);
+}
+
+/* Cylinder._sdf (primitives.py:300-304): the 2-D box SDF of (|(x, z)|, |y|)
+ * against (h, r) -- h is the disk radius, r the half height along y */
+static real cylinder_sdf(real h, real r, const real *p, real *g) {
+    real a = sqrt(p[0] * p[0] + p[2] * p[2] + LEN_EPS_SHAPE);
+    real d0 = fabs(a) - h, d1 = fabs(p[1]) - r;
+    real m = tmax(d0, d1), t = tmin(m, 0);
+    real e0 = tmax(d0, 0), e1 = tmax(d1, 0);
+    real L = sqrt(e0 * e0 + e1 * e1 + LEN_EPS_SHAPE);
+    if (g) {
+        real gd0 = (0 < d0) ? e0 / L : 0, gd1 = (0 < d1) ? e1 / L : 0;
+        if (m < 0) { if (d1 < d0) gd0 += 1; else gd1 += 1; }
+        real ga = gd0 * sgn(a);
+        g[0] = ga * p[0] / a; g[1] = gd1 * sgn(p[1]); g[2] = ga * p[2] / a;
+    }
+    return t + L;
+}
+/* Cylinder._normal (primitives.py:306-320), analytic */
+typedef struct { real pp[2], l, d0, d1, n2[2], n2n[2], p2[2], s, n3[3]; } cyl_tr;
+static void cylinder_normal_tr(real h, real r, const real *p, cyl_tr *t) {
+    t->pp[0] = p[0]; t->pp[1] = p[2];
+    t->l = sqrt(p[0] * p[0] + p[2] * p[2] + LEN_EPS_SHAPE);
+    t->d0 = t->l - h; t->d1 = fabs(p[1]) - r;
+    real f = t->d0 > t->d1 ? 1 : 0, c = tmax(t->d0, t->d1) <= 0 ? 1 : 0;
+    t->n2[0] = tmax(t->d0, 0) + c * f; t->n2[1] = tmax(t->d1, 0) + c * (1 - f);
+    nrm2(t->n2, t->n2n);
+    t->p2[0] = t->pp[0] / t->l; t->p2[1] = t->pp[1] / t->l;
+    t->s = p[1] >= 0 ? 1 : -1;
+    t->n3[0] = t->p2[0] * t->n2n[0]; t->n3[1] = t->n2n[1] * t->s; t->n3[2] = t->p2[1] * t->n2n[0];
+}
+static void cylinder_normal(real h, real r, const real *p, real *n) {
+    cyl_tr t;
+    cylinder_normal_tr(h, r, p, &t);
+    real L = sqrt(dot3(t.n3, t.n3) + LEN_EPS_SHAPE);
+    for (int i = 0; i < 3; i++) n[i] = t.n3[i] / L;
+}
+static void cylinder_normal_rev(real h, real r, const real *p, const real *gn, real *gp) {
+    cyl_tr t;
+    cylinder_normal_tr(h, r, p, &t);
+    real gn3[3] = {0, 0, 0}, gn2n[2], gn2[2] = {0, 0}, gp2[2];
+    nrm3_rev(t.n3, gn, gn3);
+    gp2[0] = gn3[0] * t.n2n[0]; gp2[1] = gn3[2] * t.n2n[0];
+    gn2n[0] = gn3[0] * t.p2[0] + gn3[2] * t.p2[1]; gn2n[1] = gn3[1] * t.s;
+    nrm2_rev(t.n2, gn2n, gn2);
+    real gd0 = (0 < t.d0) ? gn2[0] : 0, gd1 = (0 < t.d1) ? gn2[1] : 0;
+    /* p2 = pp / l; d0 = l - h; d1 = |y| - r */
+    real gl = gd0 - (gp2[0] * t.pp[0] + gp2[1] * t.pp[1]) / (t.l * t.l);
+    real gpp0 = gp2[0] / t.l + gl * t.pp[0] / t.l, gpp1 = gp2[1] / t.l + gl * t.pp[1] / t.l;
+    gp[0] += gpp0; gp[1] += gd1 * sgn(p[1]); gp[2] += gpp1;
+}
+/* Torus._sdf (primitives.py:336-339): tx the ring radius, ty the tube radius */
+static real torus_sdf(real tx, real ty, const real *p, real *g) {
+    real a = sqrt(p[0] * p[0] + p[2] * p[2] + LEN_EPS_SHAPE);
+    real q[2] = {a - tx, p[1]};
+    real L = sqrt(q[0] * q[0] + q[1] * q[1] + LEN_EPS_SHAPE);
+    if (g) {
+        real ga = q[0] / L;
+        g[0] = ga * p[0] / a; g[1] = q[1] / L; g[2] = ga * p[2] / a;
+    }
+    return L - ty;
+}
+/* Torus._normal (primitives.py:341-350), analytic */
+static void torus_normal_tr(real tx, const real *p, real *x, real *l, real *q, real *n2, real *x2, real *n3) {
+    x[0] = p[0]; x[1] = p[2];
+    *l = sqrt(x[0] * x[0] + x[1] * x[1] + LEN_EPS_SHAPE);
+    q[0] = *l - tx; q[1] = p[1];
+    nrm2(q, n2);
+    x2[0] = x[0] / *l; x2[1] = x[1] / *l;
+    n3[0] = x2[0] * n2[0]; n3[1] = n2[1]; n3[2] = x2[1] * n2[0];
+}
+static void torus_normal(real tx, const real *p, real *n) {
+    real x[2], l, q[2], n2[2], x2[2], n3[3];
+    torus_normal_tr(tx, p, x, &l, q, n2, x2, n3);
+    real L = sqrt(dot3(n3, n3) + LEN_EPS_SHAPE);
+    for (int i = 0; i < 3; i++) n[i] = n3[i] / L;
+}
+static void torus_normal_rev(real tx, const real *p, const real *gn, real *gp) {
+    real x[2], l, q[2], n2[2], x2[2], n3[3];
+    torus_normal_tr(tx, p, x, &l, q, n2, x2, n3);
+    real gn3[3] = {0, 0, 0}, gx2[2], gn2[2], gq[2] = {0, 0};
+    nrm3_rev(n3, gn, gn3);
+    gx2[0] = gn3[0] * n2[0]; gx2[1] = gn3[2] * n2[0];
+    gn2[0] = gn3[0] * x2[0] + gn3[2] * x2[1]; gn2[1] = gn3[1];
+    nrm2_rev(q, gn2, gq);
+    real gl = gq[0] - (gx2[0] * x[0] + gx2[1] * x[1]) / (l * l);
+    gp[0] += gx2[0] / l + gl * x[0] / l; gp[1] += gq[1]; gp[2] += gx2[1] / l + gl * x[1] / l;
+}
+
 /* local SDF of one "collision unit" shape (the shape whose SDF the unit tests) */
-enum { SH_SPHERE, SH_CAPSULE, SH_BOX, SH_PRISM, SH_KNIFE, SH_CHOP };
+enum { SH_SPHERE, SH_CAPSULE, SH_BOX, SH_PRISM, SH_KNIFE, SH_CHOP, SH_CYL, SH_TORUS };
 static int unit_shape(int type) {
     switch (type) {
     case ORC_SPHERE: return SH_SPHERE;
@@ -338,6 +440,8 @@ static int unit_shape(int type) {
     case ORC_PRISM: return SH_PRISM;
     case ORC_KNIFE: return SH_KNIFE;
     case ORC_CHOPSTICKS: return SH_CHOP;
+    case ORC_CYLINDER: return SH_CYL;
+    case ORC_TORUS: return SH_TORUS;
     }
     return SH_BOX;
 }
@@ -368,6 +472,8 @@ static real local_sdf(const orc_prim *P, int sh, real gap, const real *p, real *
     }
     case SH_KNIFE: return knife_sdf(P, p, g);
     case SH_CHOP: return chop_sdf(P, gap, p, g, ggap);
+    case SH_CYL: return cylinder_sdf(P->h, P->r, p, g);
+    case SH_TORUS: return torus_sdf(P->h, P->r, p, g);
     case SH_SPHERE: { /* Sphere.sdf (primitives.py:28-30), world-frame offset */
         real L = sqrt(dot3(p, p) + LEN_EPS_SHAPE);
         if (g) for (int i = 0; i < 3; i++) g[i] = p[i] / L;
@@ -377,10 +483,12 @@ static real local_sdf(const orc_prim *P, int sh, real gap, const real *p, real *
     return 0;
 }
 
-/* normals: analytic (Sphere, Capsule, Chopsticks) or central FD with a
+/* normals: analytic (Sphere, Capsule, Chopsticks, Cylinder, Torus) or central FD with a
  * float32 step 1e-4 (Box/Prism/Knife, primitives.py:382-393,720-731,796-807) */
 static void local_normal(const orc_prim *P, int sh, real gap, const real *p, real *n) {
     if (sh == SH_CAPSULE) { capsule_normal(P->h, p, n); return; }
+    if (sh == SH_CYL) { cylinder_normal(P->h, P->r, p, n); return; }
+    if (sh == SH_TORUS) { torus_normal(P->h, p, n); return; }
     if (sh == SH_SPHERE) { /* Sphere.normal = normalize(grid_pos - position) */
         real L = sqrt(dot3(p, p) + LEN_EPS_SHAPE);
         for (int i = 0; i < 3; i++) n[i] = p[i] / L;
@@ -406,6 +514,8 @@ static void local_normal(const orc_prim *P, int sh, real gap, const real *p, rea
 }
 static void local_normal_rev(const orc_prim *P, int sh, real gap, const real *p, const real *gn, real *gp, real *ggap) {
     if (sh == SH_CAPSULE) { capsule_normal_rev(P->h, p, gn, gp); return; }
+    if (sh == SH_CYL) { cylinder_normal_rev(P->h, P->r, p, gn, gp); return; }
+    if (sh == SH_TORUS) { torus_normal_rev(P->h, p, gn, gp); return; }
     if (sh == SH_SPHERE) {
         real L = sqrt(dot3(p, p) + LEN_EPS_SHAPE), d = dot3(p, gn);
         for (int i = 0; i < 3; i++) gp[i] += gn[i] / L - p[i] * d / (L * L * L);

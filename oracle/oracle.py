@@ -27,7 +27,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 PRIM_TYPES = {
     "Sphere": 0, "Capsule": 1, "RollingPin": 2, "RollingPinExt": 3, "Box": 4,
-    "Gripper": 5, "Prism": 6, "Knife": 7, "Chopsticks": 8, "Gripper2": 9,
+    "Gripper": 5, "Prism": 6, "Knife": 7, "Chopsticks": 8, "Gripper2": 9, "Cylinder": 10, "Torus": 11,
 }
 
 
@@ -135,6 +135,8 @@ PRIM_DEFAULTS = {
     "Chopsticks": dict(h=0.06, r=0.03, minimal_gap=0.06, init_gap=0.06),
     "Prism": dict(h=(0.1, 0.1), prot=(1.0, 0.0, 0.0, 0.0)),
     "Knife": dict(h=(0.1, 0.1), size=(0.1, 0.1, 0.1), prot=(1.0, 0.0, 0.0, 0.0)),
+    "Cylinder": dict(h=0.2, r=0.1),
+    "Torus": dict(tx=0.2, ty=0.1),
 }
 
 
@@ -170,8 +172,10 @@ def prim_from_cfg(cfg, softness=666.0):
              action_scale=tuple(act.get("scale", ())))
     if shape == "Sphere":
         o["r"] = d["radius"]
-    if shape in ("Capsule", "RollingPin", "RollingPinExt", "Gripper2", "Chopsticks"):
+    if shape in ("Capsule", "RollingPin", "RollingPinExt", "Gripper2", "Chopsticks", "Cylinder"):
         o["h"], o["r"] = d["h"], d["r"]
+    if shape == "Torus":
+        o["h"], o["r"] = d["tx"], d["ty"]
     if shape in ("Box", "Gripper", "Knife"):
         o["size"] = tuple(d["size"])
     if shape in ("Prism", "Knife"):

@@ -29,6 +29,8 @@ SHAPES = {
                      action=dict(dim=7, scale=(0.01, 0.01, 0.01, 0.05, 0.05, 0.05, 0.01))),
     "Chopsticks": dict(shape="Chopsticks", h=0.1, r=0.02, minimal_gap=0.02, init_gap=0.1,
                        action=dict(dim=7, scale=(0.01, 0.01, 0.01, 0.05, 0.05, 0.05, 0.01))),
+    "Cylinder": dict(shape="Cylinder", h=0.04, r=0.02, action=dict(dim=6, scale=(0.01, 0.01, 0.01, 0.05, 0.05, 0.05))),
+    "Torus": dict(shape="Torus", tx=0.05, ty=0.015, action=dict(dim=6, scale=(0.01, 0.01, 0.01, 0.05, 0.05, 0.05))),
 }
 
 

@@ -127,8 +127,13 @@ def test_primitives_container_api():
     # state kept on the host until a simulator context exists
     prims[0].set_state(3, [0.1, 0.2, 0.3])
     np.testing.assert_allclose(prims[0].get_state(3)[:3], [0.1, 0.2, 0.3])
+    cyl, tor = Primitives([dict(shape="Cylinder", h=0.05, r=0.02), dict(shape="Torus")])
+    d = cyl.desc()
+    assert d.type == 10 and (d.h, d.r) == (0.05, 0.02)
+    d = tor.desc()
+    assert d.type == 11 and (d.h, d.r) == (0.2, 0.1)     # Torus default tx, ty (primitives.py:352-356)
     with pytest.raises(NotImplementedError):
-        Primitives([dict(shape="Torus")])
+        Primitives([dict(shape="Cone")])
 
 
 def test_collision_pairs_and_rand_num():

@@ -265,6 +265,60 @@ def test_sdf_closed_forms_and_normals():
     assert d[0] == pytest.approx(0.2, abs=1e-6)
 
 
+def _surface_distance(x, surf):
+    from scipy.spatial import cKDTree
+    return cKDTree(surf).query(x)[0]
+
+
+def test_cylinder_and_torus_sdfs_and_normals():
+    """Cylinder (h the disk radius, r the half height; primitives.py:300-320)
+    and Torus (tx the ring radius, ty the tube radius; :336-350): hand-computed
+    points, distances of outside points to a densely sampled surface, and the
+    analytic normals against the normalised central difference of the SDF,
+    in a rotated frame."""
+    rng = np.random.default_rng(7)
+    cyl = prim_from_cfg(dict(shape="Cylinder", h=0.1, r=0.05))
+    tor = prim_from_cfg(dict(shape="Torus", tx=0.2, ty=0.05))
+    sim, _ = sim_from_cfg({})
+    o = Oracle(sim, [cyl, tor])
+    c = np.array([0.5, 0.4, 0.5])
+    ps = np.zeros(8)
+    ps[:3] = c
+    ps[3] = 1
+    pts = c + np.array([[0, 0.2, 0], [0.3, 0, 0], [0.2, 0.1, 0], [0.05, 0.01, 0], [0, -0.3, 0]])
+    np.testing.assert_allclose(o.prim_sdf(0, ps, pts), [0.15, 0.2, np.hypot(0.1, 0.05), -0.04, 0.25], atol=1e-6)
+    np.testing.assert_allclose(o.prim_normal(0, ps, pts[[0, 1, 3, 4]]), [[0, 1, 0], [1, 0, 0], [0, 1, 0], [0, -1, 0]], atol=1e-6)
+    pts = c + np.array([[0.2, 0.1, 0], [0.4, 0, 0], [0, 0, -0.2], [0.25, 0, 0]])
+    np.testing.assert_allclose(o.prim_sdf(1, ps, pts), [0.05, 0.15, -0.05, 0.0], atol=1e-6)
+    np.testing.assert_allclose(o.prim_normal(1, ps, pts[:2]), [[0, 1, 0], [1, 0, 0]], atol=1e-6)
+    # sampled surfaces (local frame)
+    u = rng.uniform(0, 2 * np.pi, 400000)
+    v = rng.uniform(0, 1, 400000)
+    side = np.stack([0.1 * np.cos(u), -0.05 + 0.1 * v, 0.1 * np.sin(u)], 1)
+    rad = 0.1 * np.sqrt(v)
+    caps = np.stack([rad * np.cos(u), np.where(np.arange(len(u)) % 2 == 0, 0.05, -0.05), rad * np.sin(u)], 1)
+    cyl_surf = np.concatenate([side, caps])
+    w = rng.uniform(0, 2 * np.pi, 400000)
+    tor_surf = np.stack([(0.2 + 0.05 * np.cos(w)) * np.cos(u), 0.05 * np.sin(w), (0.2 + 0.05 * np.cos(w)) * np.sin(u)], 1)
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    R = pytest.importorskip("scipy.spatial.transform").Rotation.from_quat([q[1], q[2], q[3], q[0]])
+    ps[3:7] = q
+    for j, surf, scale in ((0, cyl_surf, 0.25), (1, tor_surf, 0.35)):
+        loc = rng.uniform(-scale, scale, (300, 3))
+        x = c + R.apply(loc)
+        d = o.prim_sdf(j, ps, x)
+        out = d > 0.01
+        np.testing.assert_allclose(d[out], _surface_distance(loc[out], surf), atol=2e-3)
+        # normal = direction of the SDF gradient (central differences in fp64)
+        n = o.prim_normal(j, ps, x[out])
+        e = 1e-6
+        g = np.stack([(o.prim_sdf(j, ps, x[out] + e * np.eye(3)[k]) - o.prim_sdf(j, ps, x[out] - e * np.eye(3)[k])) / (2 * e)
+                      for k in range(3)], 1)
+        g /= np.linalg.norm(g, axis=1, keepdims=True)
+        np.testing.assert_allclose(n, g, atol=1e-5)
+
+
 def test_forward_kinematics_matches_scipy_rotations():
     """forward_kinematics (primive_base.py:152-156): rot' = w2quat(w) (x) rot, i.e.
     a world-frame rotation by the rotation vector w; pos' = clamp(pos + v)."""
@@ -310,6 +364,8 @@ FD_SHAPES = {
                     action=dict(dim=7, scale=(0.01, 0.01, 0.01, 0.05, 0.05, 0.05, 0.01))),
     "Chopsticks": dict(shape="Chopsticks", h=0.1, r=0.02, minimal_gap=0.02, init_gap=0.1,
                        action=dict(dim=7, scale=(0.01, 0.01, 0.01, 0.05, 0.05, 0.05, 0.01))),
+    "Cylinder": dict(shape="Cylinder", h=0.04, r=0.02, action=dict(dim=6, scale=(0.01, 0.01, 0.01, 0.05, 0.05, 0.05))),
+    "Torus": dict(shape="Torus", tx=0.05, ty=0.015, action=dict(dim=6, scale=(0.01, 0.01, 0.01, 0.05, 0.05, 0.05))),
 }
 
 
