@@ -53,6 +53,52 @@ class _Scalar:
         self.v = v
 
 
+class _ParticleField:
+    """A per-frame particle field of MPMSimulator (x, v, F or C, ``[frame][particle]``,
+    mpm_simulator.py:40-43), read from the engine's frames in the caller's
+    particle order: ``field[f, p]``, ``field.to_numpy()`` (frames not simulated
+    read as zeros).  ``field.grad.fill(0)`` clears the engine's adjoints, what
+    GradModel.reset does for every field (function.py:44-61); ``field.grad[f, p]``
+    reads the adjoint of the frame the backward pass is at."""
+
+    _INDEX = {"x": 0, "v": 1, "F": 2, "C": 3}
+
+    def __init__(self, sim, name, grad=False):
+        self.sim, self.name, self.is_grad = sim, name, grad
+
+    @property
+    def grad(self):
+        return _ParticleField(self.sim, self.name, True)
+
+    def _frame(self, f):
+        k = self._INDEX[self.name]
+        if self.is_grad:
+            x, v, F, C = self.sim.get_adjoint(f)
+            return (x, v, F, C)[k]
+        return self.sim._frame_tensors(f)[k]
+
+    def __getitem__(self, key):
+        f, p = key
+        return self._frame(int(f))[int(p)].detach().double().cpu().numpy()
+
+    def to_numpy(self):
+        n = self.sim.n_particles[None]
+        shape = (3,) if self.name in ("x", "v") else (3, 3)
+        out = np.zeros((self.sim.max_steps, n) + shape)
+        for f in range(self.sim.max_steps):
+            try:
+                out[f] = self._frame(f).detach().double().cpu().numpy()
+            except RuntimeError:
+                pass
+        return out
+
+    def fill(self, value):
+        if self.is_grad and float(value) == 0.0:
+            self.sim.clear_grads()
+            return
+        raise NotImplementedError("%s.fill: set particle states with set_state / reset" % self.name)
+
+
 class _GridField:
     """``grid_m`` with the field methods callers use (fill / to_torch / to_numpy / grad.from_torch)."""
 
@@ -120,6 +166,10 @@ class MPMSimulator:
         self.ckpt_k = ckpt_k
         self.keep_blocks = keep_blocks
         self.grid_m = _GridField(self)
+        self.x = _ParticleField(self, "x")
+        self.v = _ParticleField(self, "v")
+        self.F = _ParticleField(self, "F")
+        self.C = _ParticleField(self, "C")
         # particles are stored in cell order: slot i holds the caller's particle perm[i]
         self._perm = None
         self._inv = None

@@ -21,6 +21,70 @@ def _tuple(v, n=None, fill=0.0):
     return v
 
 
+class _PrimField:
+    """One per-frame field of a primitive (primive_base.py:37-52): ``position``,
+    ``rotation``, ``gap`` in the frame state, ``v``, ``w``, ``gap_vel`` in the
+    per-substep velocity, read and written through the engine's buffers;
+    ``field[f]``, ``field[f] = value``, ``to_numpy()``, ``fill(value)``, and the
+    adjoint as ``field.grad`` with the same methods."""
+
+    def __init__(self, prim, which, cols, grad=False):
+        self.prim, self.which, self.cols, self.is_grad = prim, which, cols, grad
+
+    @property
+    def grad(self):
+        return _PrimField(self.prim, self.which, self.cols, True)
+
+    def _buf(self):
+        sim = self.prim._sim
+        if sim is None or sim.engine is None:
+            raise RuntimeError("primitive fields live in the simulator's buffers: initialize the simulator first")
+        e = sim.engine
+        if self.which == "state":
+            return e.prim_state_grad if self.is_grad else e.prim_state
+        return e.prim_vel_grad if self.is_grad else e.prim_vel
+
+    def __getitem__(self, f):
+        v = self._buf()[int(f), self.prim._index, self.cols[0]:self.cols[1]].detach().double().cpu().numpy()
+        return v[0] if len(v) == 1 else v
+
+    def __setitem__(self, f, value):
+        b = self._buf()
+        b[int(f), self.prim._index, self.cols[0]:self.cols[1]] = torch.as_tensor(
+            np.asarray(value, np.float32).reshape(-1), device=b.device)
+
+    def to_numpy(self):
+        v = self._buf()[:, self.prim._index, self.cols[0]:self.cols[1]].detach().double().cpu().numpy()
+        return v[:, 0] if v.shape[1] == 1 else v
+
+    def fill(self, value):
+        self._buf()[:, self.prim._index, self.cols[0]:self.cols[1]] = float(value)
+
+
+class _ActionBuffer:
+    """``action_buffer`` (primive_base.py:49): the clipped action of each env
+    step s as ``action_buffer[s]``; its adjoint ``action_buffer.grad[s]``
+    (get_action_grad, primive_base.py:276-282)."""
+
+    def __init__(self, prim, grad=False):
+        self.prim, self.is_grad = prim, grad
+
+    @property
+    def grad(self):
+        return _ActionBuffer(self.prim, True)
+
+    def __getitem__(self, s):
+        p = self.prim
+        if self.is_grad:
+            return p.get_action_grad(int(s), 1)[0]
+        return np.asarray(p._sim.primitives._table(int(s))[p._index, :p.action_dim], np.float64)
+
+    def fill(self, value):
+        if self.is_grad and float(value) == 0.0:
+            return  # action adjoints are derived from the velocity adjoints, which the grad fills clear
+        raise NotImplementedError("action_buffer.fill: set actions with set_action")
+
+
 class Primitive:
     """Primitive (primive_base.py:13-329)."""
 
@@ -47,6 +111,12 @@ class Primitive:
         self._index = None
         self._host_states = {}
         self._xyz_limit = (_tuple(self.cfg.lower_bound, 3), _tuple(self.cfg.upper_bound, 3))
+        # per-frame fields and their adjoints (engine buffers once bound)
+        self.position = _PrimField(self, "state", (0, 3))
+        self.rotation = _PrimField(self, "state", (3, 7))
+        self.v = _PrimField(self, "vel", (0, 3))
+        self.w = _PrimField(self, "vel", (3, 6))
+        self.action_buffer = _ActionBuffer(self)
 
     # ------------------------------------------------------------ config
     @classmethod
@@ -197,6 +267,15 @@ class RollingPinExt(Capsule):
 
 class _GapMixin:
     state_dim = 8
+
+    @property
+    def gap(self):
+        """Per-frame gap (primitives.py:433) and its adjoint."""
+        return _PrimField(self, "state", (7, 8))
+
+    @property
+    def gap_vel(self):
+        return _PrimField(self, "vel", (6, 7))
 
     @property
     def init_state(self):
