@@ -1,10 +1,10 @@
 """TaichiEnv: the scene-assembly and state API of plb/engine/taichi_env.py
 (initialize, step, get_state / set_state, get_obs) over the HIP engine.
 
-Rendering, pykeops / geomloss point-cloud losses and the Taichi loss module
-are not part of this engine.  Initial particles come from ``SHAPES`` boxes and
-spheres sampled like plb/engine/shapes/shape_maker.py, or from
-``init_particles`` passed explicitly.
+Rendering, the pykeops / geomloss Sinkhorn loss and the Taichi loss module
+are not part of this engine.  Initial particles come from the ``SHAPES`` list
+(shapes.Shapes, plb/engine/shapes/shape_maker.py) or from ``init_particles``
+passed explicitly.
 """
 import numpy as np
 import torch
@@ -12,30 +12,7 @@ import torch
 from ..config import CfgNode, get_cfg_defaults
 from .mpm_simulator import MPMSimulator
 from .primitive import Primitives
-
-
-def sample_shapes(shapes, seed=None):
-    """Box / sphere sampling of shape_maker.py:81-105 (30000 particles per 0.1^3)."""
-    rng = np.random.RandomState(seed) if seed is not None else np.random
-    objs = []
-    for s in shapes:
-        s = CfgNode(s) if not isinstance(s, CfgNode) else s
-        kind = s.get("shape", "box")
-        if kind == "box":
-            width = np.array(s.get("width", (0.1, 0.1, 0.1)), dtype=np.float64)
-            n = s.get("n_particles", None) or max(int(np.prod(width) / (0.1 ** 3) * 30000), 1)
-            p = (rng.random_sample((int(n), 3)) * 2 - 1) * (0.5 * width) + np.array(s.init_pos)
-        elif kind == "sphere":
-            r = float(s.get("radius", 0.05))
-            n = s.get("n_particles", None) or max(int((r ** 3) * 4 * np.pi / 3 / (0.1 ** 3) * 30000), 1)
-            p = rng.normal(size=(int(n), 3))
-            p /= np.linalg.norm(p, axis=-1, keepdims=True)
-            u = rng.random_sample(size=(int(n), 1)) ** (1. / 3)
-            p = p * u * r + np.array(s.init_pos)
-        else:
-            raise NotImplementedError("shape %r is not supported by this sampler" % kind)
-        objs.append(p)
-    return np.concatenate(objs).astype(np.float64)
+from .shapes import Shapes
 
 
 class TaichiEnv:
@@ -48,7 +25,12 @@ class TaichiEnv:
         self.env_name = self.cfg.get("env_name", None)
         self.primitives = Primitives(full.PRIMITIVES)
         if init_particles is None:
-            init_particles = sample_shapes(full.SHAPES)
+            # TaichiEnv.__init__ (taichi_env.py:65-66): Shapes(cfg.SHAPES).get()
+            self.shapes = Shapes(full.SHAPES)
+            init_particles, self.particle_colors = self.shapes.get()
+        else:
+            self.shapes = None
+            self.particle_colors = None
         self.init_particles = np.asarray(init_particles, np.float64)
         self.n_particles = len(self.init_particles)
         self.simulator = MPMSimulator(full.SIMULATOR, self.primitives, ckpt_k=ckpt_k)
@@ -61,7 +43,16 @@ class TaichiEnv:
         self._is_copy = is_copy
 
     def initialize(self, cfg=None, target_path=None):
-        """taichi_env.py:103-121."""
+        """taichi_env.py:103-121: a new cfg rebuilds the particles from its
+        SHAPES and updates the primitives' cfgs."""
+        if cfg is not None:
+            full = get_cfg_defaults()
+            full.merge(cfg)
+            self.full_cfg = full
+            self.cfg = full.ENV
+            self.shapes = Shapes(full.SHAPES)
+            self.init_particles, self.particle_colors = self.shapes.get()
+            self.primitives.update_cfgs(full.PRIMITIVES)
         if self.has_loss and target_path is not None:
             self.load_target_x(target_path)
         self.n_particles = len(self.init_particles)
