@@ -548,6 +548,7 @@ DEV int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 // metadata is wave-uniform (scalar loads, issued together); the first tile of
 // every covering block is loaded before any is summed, so the dependent-load
 // chain is stamp -> item range -> tile, not eight of them in a row.
+template <int BATCH>
 DEV float4 gather_tiles(const GridArgs& ga, const SceneC& S, int B0, int B1, int B2, int lane, const float4* __restrict__ tiles,
                         int stamp) {
     int l0 = lane >> 4, l1 = (lane >> 2) & 3, l2 = lane & 3;
@@ -579,12 +580,30 @@ DEV float4 gather_tiles(const GridArgs& ga, const SceneC& S, int B0, int B1, int
 #pragma unroll
     for (int d = 0; d < 8; d++) {
         acc.x += v[d].x; acc.y += v[d].y; acc.z += v[d].z; acc.w += v[d].w;
-        // further work items of a block with more than wi particles
-        if (tidx[d] >= 0)
-            for (int j = 1; j < ni[d]; j++) {
-                float4 u = tiles[(size_t)(i0[d] + j) * TILE + tidx[d]];
+        // further work items of a block with more than wi particles: their
+        // tiles are loaded BATCH at a time before being summed (in item
+        // order), so the loads' latencies overlap instead of chaining
+        if (BATCH == 1) {
+            if (tidx[d] >= 0)
+                for (int j = 1; j < ni[d]; j++) {
+                    float4 u = tiles[(size_t)(i0[d] + j) * TILE + tidx[d]];
+                    acc.x += u.x; acc.y += u.y; acc.z += u.z; acc.w += u.w;
+                }
+        } else if (tidx[d] >= 0) {
+            const float4* tp = tiles + (size_t)i0[d] * TILE + tidx[d];
+            int j = 1;
+            for (; j + BATCH <= ni[d]; j += BATCH) {
+                float4 u[BATCH];
+#pragma unroll
+                for (int k = 0; k < BATCH; k++) u[k] = tp[(size_t)(j + k) * TILE];
+#pragma unroll
+                for (int k = 0; k < BATCH; k++) { acc.x += u[k].x; acc.y += u[k].y; acc.z += u[k].z; acc.w += u[k].w; }
+            }
+            for (; j < ni[d]; j++) {
+                float4 u = tp[(size_t)j * TILE];
                 acc.x += u.x; acc.y += u.y; acc.z += u.z; acc.w += u.w;
             }
+        }
     }
     return acc;
 }
@@ -607,7 +626,7 @@ __global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, const float4* __re
         int B0, B1, B2, I0, I1, I2;
         long long node;
         node_coords(S, gb, lane, B0, B1, B2, I0, I1, I2, node);
-        float4 pm = gather_tiles(ga, S, B0, B1, B2, lane, tiles, stamp);
+        float4 pm = gather_tiles<8>(ga, S, B0, B1, B2, lane, tiles, stamp);
         v3 v = mk(0.f, 0.f, 0.f);
         if (pm.w > 1e-12f && in_grid(S, I0, I1, I2)) {
             const float* ps0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
@@ -827,7 +846,7 @@ __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* 
         long long node;
         node_coords(S, gb, lane, B0, B1, B2, I0, I1, I2, node);
         float4 pm = kept ? ga.keep_pm[(size_t)e * NODES_PER_BLK + lane] : g_pm[node];
-        float4 gv4 = gather_tiles(ga, S, B0, B1, B2, lane, tiles_v, stamp);
+        float4 gv4 = gather_tiles<1>(ga, S, B0, B1, B2, lane, tiles_v, stamp);
         bool act = pm.w > 1e-12f && in_grid(S, I0, I1, I2);
         const float* ps0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
         const float* ps1 = ga.pstate + ((size_t)(ga.f + 1) * ga.n_prims_total + S.prim_off) * 8;
