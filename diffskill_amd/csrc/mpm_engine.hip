@@ -700,24 +700,42 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __res
             v3 nv = mk(0.f, 0.f, 0.f);
             m3 nC = mzero();
             if (in_tile) {
+                // new_v = sum_n wt g and new_C[:, j] = c4 sum_n wt (n_j - fx_j) g:
+                // the weights factor over the axes, so the sums are taken axis by
+                // axis (pw = w (n - fx) along one axis, w along the others).
+                float pw[3][3];
 #pragma unroll
-                for (int a = 0; a < 3; a++)
+                for (int o = 0; o < 3; o++)
 #pragma unroll
-                    for (int b = 0; b < 3; b++)
+                    for (int d = 0; d < 3; d++) pw[o][d] = sw.w[o][d] * ((float)o - sw.fx[d]);
+                v3 G0 = mk(0.f, 0.f, 0.f), Gx = G0, Gy = G0, Gz = G0;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    v3 J = mk(0.f, 0.f, 0.f), Jy = J, Jz = J;
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        v3 H = mk(0.f, 0.f, 0.f), Hz = H;
 #pragma unroll
                         for (int c = 0; c < 3; c++) {
-                            float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
-                            float4 g = tv[((l0 + a) * 6 + (l1 + b)) * 6 + (l2 + c)];
-                            float dp[3] = {(float)a - sw.fx[0], (float)b - sw.fx[1], (float)c - sw.fx[2]};
-                            nv += wt * mk(g.x, g.y, g.z);
-                            float cw = S.c4 * wt;
-#pragma unroll
-                            for (int j = 0; j < 3; j++) {
-                                nC.a[j] += cw * g.x * dp[j];
-                                nC.a[3 + j] += cw * g.y * dp[j];
-                                nC.a[6 + j] += cw * g.z * dp[j];
-                            }
+                            float4 g4 = tv[((l0 + a) * 6 + (l1 + b)) * 6 + (l2 + c)];
+                            v3 g = mk(g4.x, g4.y, g4.z);
+                            H += sw.w[c][2] * g;
+                            Hz += pw[c][2] * g;
                         }
+                        J += sw.w[b][1] * H;
+                        Jy += pw[b][1] * H;
+                        Jz += sw.w[b][1] * Hz;
+                    }
+                    G0 += sw.w[a][0] * J;
+                    Gx += pw[a][0] * J;
+                    Gy += sw.w[a][0] * Jy;
+                    Gz += sw.w[a][0] * Jz;
+                }
+                float c4 = S.c4;
+                nv = G0;
+                nC.a[0] = c4 * Gx.x, nC.a[1] = c4 * Gy.x, nC.a[2] = c4 * Gz.x;
+                nC.a[3] = c4 * Gx.y, nC.a[4] = c4 * Gy.y, nC.a[5] = c4 * Gz.y;
+                nC.a[6] = c4 * Gx.z, nC.a[7] = c4 * Gy.z, nC.a[8] = c4 * Gz.z;
             }
             store3(st_out, stride, 3, p, nv);
             store9(st_out, stride, 6, p, nC);
@@ -778,37 +796,57 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
             int l0 = sw.base[0] - 4 * bi, l1 = sw.base[1] - 4 * bj, l2 = sw.base[2] - 4 * bk;
             bool in_tile = (unsigned)l0 < 4u && (unsigned)l1 < 4u && (unsigned)l2 < 4u;
             float gfx[3] = {0.f, 0.f, 0.f};
+            // Per node n = (a, b, c) with dp = n - fx, the adjoint of the node's
+            // velocity is wt u, u = gnv + c4 gC dp = u0 + a K0 + b K1 + c K2
+            // (Kj = c4 gC[:, j], u0 = gnv - c4 gC fx).
+            float c4 = S.c4;
+            v3 K0 = mk(c4 * gC.a[0], c4 * gC.a[3], c4 * gC.a[6]);
+            v3 K1 = mk(c4 * gC.a[1], c4 * gC.a[4], c4 * gC.a[7]);
+            v3 K2 = mk(c4 * gC.a[2], c4 * gC.a[5], c4 * gC.a[8]);
+            v3 u0 = gnv - (sw.fx[0] * K0 + sw.fx[1] * K1 + sw.fx[2] * K2);
             if (in_tile) {
+                // The weight's adjoint is gw = g . u, and gfx[d] sums gw times the
+                // derivative of the weight along d minus c4 (gC^T sum_n wt g)[d].
+                // Both factor over the three axes, so they are accumulated axis by axis.
+                v3 ua = u0;
+                v3 G0 = mk(0.f, 0.f, 0.f);
+                float X0 = 0.f, X1 = 0.f, X2 = 0.f;
 #pragma unroll
-                for (int a = 0; a < 3; a++)
+                for (int a = 0; a < 3; a++) {
+                    v3 ub = ua, Ja = mk(0.f, 0.f, 0.f);
+                    float Ra = 0.f, Sa = 0.f, Ta = 0.f;
 #pragma unroll
-                    for (int b = 0; b < 3; b++)
+                    for (int b = 0; b < 3; b++) {
+                        v3 u = ub, H = mk(0.f, 0.f, 0.f);
+                        float P = 0.f, Q = 0.f;
 #pragma unroll
                         for (int c = 0; c < 3; c++) {
                             float4 g4 = tv[((l0 + a) * 6 + (l1 + b)) * 6 + (l2 + c)];
                             v3 g = mk(g4.x, g4.y, g4.z);
-                            float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
-                            float dp[3] = {(float)a - sw.fx[0], (float)b - sw.fx[1], (float)c - sw.fx[2]};
-                            v3 Cd = mk(gC.a[0] * dp[0] + gC.a[1] * dp[1] + gC.a[2] * dp[2],
-                                       gC.a[3] * dp[0] + gC.a[4] * dp[1] + gC.a[5] * dp[2],
-                                       gC.a[6] * dp[0] + gC.a[7] * dp[1] + gC.a[8] * dp[2]);
-                            v3 CTg = mk(gC.a[0] * g.x + gC.a[3] * g.y + gC.a[6] * g.z, gC.a[1] * g.x + gC.a[4] * g.y + gC.a[7] * g.z,
-                                        gC.a[2] * g.x + gC.a[5] * g.y + gC.a[8] * g.z);
-                            float gw = dot(gnv, g) + S.c4 * dot(g, Cd);
-                            int off[3] = {a, b, c};
-#pragma unroll
-                            for (int d = 0; d < 3; d++) {
-                                float dwd = sw.dw[off[d]][d];
-#pragma unroll
-                                for (int e2 = 0; e2 < 3; e2++)
-                                    if (e2 != d) dwd *= sw.w[off[e2]][e2];
-                                gfx[d] += gw * dwd - S.c4 * wt * comp(CTg, d);
-                            }
+                            float gw = dot(g, u);
+                            P += sw.w[c][2] * gw;
+                            Q += sw.dw[c][2] * gw;
+                            H += sw.w[c][2] * g;
+                            u += K2;
                         }
+                        Ja += sw.w[b][1] * H;
+                        Ra += sw.w[b][1] * P;
+                        Sa += sw.dw[b][1] * P;
+                        Ta += sw.w[b][1] * Q;
+                        ub += K1;
+                    }
+                    G0 += sw.w[a][0] * Ja;
+                    X0 += sw.dw[a][0] * Ra;
+                    X1 += sw.w[a][0] * Sa;
+                    X2 += sw.w[a][0] * Ta;
+                    ua += K0;
+                }
+                gfx[0] = X0 - c4 * (gC.a[0] * G0.x + gC.a[3] * G0.y + gC.a[6] * G0.z);
+                gfx[1] = X1 - c4 * (gC.a[1] * G0.x + gC.a[4] * G0.y + gC.a[7] * G0.z);
+                gfx[2] = X2 - c4 * (gC.a[2] * G0.x + gC.a[5] * G0.y + gC.a[8] * G0.z);
             }
             if (valid) store3(a0, stride, 0, p, gxn + S.inv_dx * mk(gfx[0], gfx[1], gfx[2]));
             int key = (l0 << 4) | (l1 << 2) | l2;
-            float c4 = S.c4;
             auto val = [&](int n, int c) -> float {
                 int a = n / 9, b = (n / 3) % 3, cc = n % 3;
                 float wt = sw.w[a][0] * sw.w[b][1] * sw.w[cc][2];
@@ -1068,39 +1106,68 @@ __global__ void __launch_bounds__(WG) k_p2g_bwd_gather(TileArgs ta, const float*
             float gfx[3] = {0.f, 0.f, 0.f};
             v3 mv = S.p_mass * v;
             if (in_tile) {
+                // Per node n = (a, b, c) with dpos = dx (n - fx): the node's
+                // momentum is wt u, u = mv + A dpos = u0 + a K0 + b K1 + c K2
+                // (Kj = dx A[:, j]), and the weight's adjoint gw = g . u + g_m pm.
+                // The adjoints are sums over n of wt g (v), wt g dpos^T (affine),
+                // gw times the weight's derivative along d and A^T wt g (x); the
+                // weights factor over the axes, so all of them are accumulated axis
+                // by axis (pw = w (n - fx) along one axis).
+                float dx = S.dx, pm = S.p_mass;
+                float pw[3][3];
 #pragma unroll
-                for (int a = 0; a < 3; a++)
+                for (int o = 0; o < 3; o++)
 #pragma unroll
-                    for (int b = 0; b < 3; b++)
+                    for (int d = 0; d < 3; d++) pw[o][d] = sw.w[o][d] * ((float)o - sw.fx[d]);
+                v3 K0 = mk(dx * A[0], dx * A[3], dx * A[6]);
+                v3 K1 = mk(dx * A[1], dx * A[4], dx * A[7]);
+                v3 K2 = mk(dx * A[2], dx * A[5], dx * A[8]);
+                v3 ua = mv - (sw.fx[0] * K0 + sw.fx[1] * K1 + sw.fx[2] * K2);
+                v3 G0 = mk(0.f, 0.f, 0.f), Gx = G0, Gy = G0, Gz = G0;
+                float X0 = 0.f, X1 = 0.f, X2 = 0.f;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    v3 ub = ua, J = mk(0.f, 0.f, 0.f), Jy = J, Jz = J;
+                    float Ra = 0.f, Sa = 0.f, Ta = 0.f;
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        v3 u = ub, H = mk(0.f, 0.f, 0.f), Hz = H;
+                        float P = 0.f, Q = 0.f;
 #pragma unroll
                         for (int c = 0; c < 3; c++) {
                             float4 gg = tg[((l0 + a) * 6 + (l1 + b)) * 6 + (l2 + c)];
-                            float wt = sw.w[a][0] * sw.w[b][1] * sw.w[c][2];
-                            float dp[3] = {((float)a - sw.fx[0]) * S.dx, ((float)b - sw.fx[1]) * S.dx, ((float)c - sw.fx[2]) * S.dx};
-                            v3 contrib = mk(mv.x + A[0] * dp[0] + A[1] * dp[1] + A[2] * dp[2], mv.y + A[3] * dp[0] + A[4] * dp[1] + A[5] * dp[2],
-                                            mv.z + A[6] * dp[0] + A[7] * dp[1] + A[8] * dp[2]);
                             v3 g = mk(gg.x, gg.y, gg.z);
-                            float gw = dot(g, contrib) + gg.w * S.p_mass;
-                            gv += (wt * S.p_mass) * g;
-                            v3 wg = wt * g;
-#pragma unroll
-                            for (int j = 0; j < 3; j++) {
-                                gaff.a[j] += wg.x * dp[j];
-                                gaff.a[3 + j] += wg.y * dp[j];
-                                gaff.a[6 + j] += wg.z * dp[j];
-                            }
-                            v3 gdpos = mk(wg.x * A[0] + wg.y * A[3] + wg.z * A[6], wg.x * A[1] + wg.y * A[4] + wg.z * A[7],
-                                          wg.x * A[2] + wg.y * A[5] + wg.z * A[8]);
-                            int off[3] = {a, b, c};
-#pragma unroll
-                            for (int d = 0; d < 3; d++) {
-                                float dwd = sw.dw[off[d]][d];
-#pragma unroll
-                                for (int e2 = 0; e2 < 3; e2++)
-                                    if (e2 != d) dwd *= sw.w[off[e2]][e2];
-                                gfx[d] += gw * dwd - comp(gdpos, d) * S.dx;
-                            }
+                            float gw = dot(g, u) + gg.w * pm;
+                            P += sw.w[c][2] * gw;
+                            Q += sw.dw[c][2] * gw;
+                            H += sw.w[c][2] * g;
+                            Hz += pw[c][2] * g;
+                            u += K2;
                         }
+                        J += sw.w[b][1] * H;
+                        Jy += pw[b][1] * H;
+                        Jz += sw.w[b][1] * Hz;
+                        Ra += sw.w[b][1] * P;
+                        Sa += sw.dw[b][1] * P;
+                        Ta += sw.w[b][1] * Q;
+                        ub += K1;
+                    }
+                    G0 += sw.w[a][0] * J;
+                    Gx += pw[a][0] * J;
+                    Gy += sw.w[a][0] * Jy;
+                    Gz += sw.w[a][0] * Jz;
+                    X0 += sw.dw[a][0] * Ra;
+                    X1 += sw.w[a][0] * Sa;
+                    X2 += sw.w[a][0] * Ta;
+                    ua += K0;
+                }
+                gv = pm * G0;
+                gaff.a[0] = dx * Gx.x, gaff.a[1] = dx * Gy.x, gaff.a[2] = dx * Gz.x;
+                gaff.a[3] = dx * Gx.y, gaff.a[4] = dx * Gy.y, gaff.a[5] = dx * Gz.y;
+                gaff.a[6] = dx * Gx.z, gaff.a[7] = dx * Gy.z, gaff.a[8] = dx * Gz.z;
+                gfx[0] = X0 - dx * (A[0] * G0.x + A[3] * G0.y + A[6] * G0.z);
+                gfx[1] = X1 - dx * (A[1] * G0.x + A[4] * G0.y + A[7] * G0.z);
+                gfx[2] = X2 - dx * (A[2] * G0.x + A[5] * G0.y + A[8] * G0.z);
             }
             v3 gx0 = load3(a0, stride, 0, p) + S.inv_dx * mk(gfx[0], gfx[1], gfx[2]);
             store3(a0, stride, 0, p, gx0);
