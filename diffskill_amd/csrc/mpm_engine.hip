@@ -322,7 +322,17 @@ __global__ void __launch_bounds__(PPB) k_bin(BinArgs a, const float* __restrict_
 // node block's flag, as in k_bin) with their list positions in nmap.  A
 // workgroup takes 4 blocks at a time and claims their ranges with one 64-bit
 // atomic on the (particles, items) counter pair and one on the node count.
-__global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nblocks, int* __restrict__ nmap) {
+// Node-block records: for active node block e < rec_cap, NREC ints
+// {gb, scene, -, -, item0[8], nitems[8]} with the work-item ranges of the 8 cell
+// blocks whose 6x6x6 footprint covers it (cell block B - (d>>2&1, d>>1&1, d&1);
+// 0 items when it is not occupied this pass), so the grid phases read one record
+// instead of chasing block list -> scene -> block stamps -> item ranges.
+// k_blocks writes gb, scene and the 8 covering block ids (-1: outside the grid)
+// into slots 4..11; k_scatter, once every block is stamped, replaces those with
+// the item ranges.
+constexpr int NREC = 20;
+__global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nblocks, int* __restrict__ nmap, int* __restrict__ nrec,
+                                                int rec_cap) {
     __shared__ unsigned long long s_tot[4];
     __shared__ unsigned long long s_base;
     __shared__ int s_nn[4];
@@ -347,12 +357,13 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nbl
         int total = __shfl(incl, 63);
         int nit = (total + a.wi - 1) / a.wi;
         // node blocks of the footprint reached first by this pass
-        int nb = -1;
+        int nb = -1, sc = 0, ni = 0, nj = 0, nk = 0;
         if (has && lane < 8) {
-            const SceneC& S = a.scenes[a.blk_scene[gb]];
+            sc = a.blk_scene[gb];
+            const SceneC& S = a.scenes[sc];
             int bi, bj, bk;
             block_origin(S, gb, bi, bj, bk);
-            int ni = bi + ((lane >> 2) & 1), nj = bj + ((lane >> 1) & 1), nk = bk + (lane & 1);
+            ni = bi + ((lane >> 2) & 1), nj = bj + ((lane >> 1) & 1), nk = bk + (lane & 1);
             if (ni < S.nb && nj < S.nb && nk < S.nb) nb = S.blk_off + (ni * S.nb + nj) * S.nb + nk;
         }
         bool fresh = nb >= 0 && a.node_flag[nb] != stamp && atomicExch(&a.node_flag[nb], stamp) != stamp;
@@ -376,6 +387,17 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nbl
             int pos = noff + __popcll(fm & lanemask_lt(lane));
             nblocks[pos] = nb;
             nmap[nb] = pos;
+            if (pos < rec_cap) {
+                const SceneC& S = a.scenes[sc];
+                int* r = nrec + (size_t)pos * NREC;
+                r[0] = nb;
+                r[1] = sc;
+#pragma unroll
+                for (int d = 0; d < 8; d++) {
+                    int P0 = ni - ((d >> 2) & 1), P1 = nj - ((d >> 1) & 1), P2 = nk - (d & 1);
+                    r[4 + d] = (P0 < 0 || P1 < 0 || P2 < 0) ? -1 : S.blk_off + (P0 * S.nb + P1) * S.nb + P2;
+                }
+            }
         }
         if (!has) continue;
         a.cell_start[gb * 64 + lane] = start + incl - cnt;
@@ -390,10 +412,32 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nbl
 
 // index scatter; the pass's last kernel also advances the stamp base and
 // resets the occupied-block count for the next pass
-__global__ void __launch_bounds__(PPB) k_scatter(BinArgs a) {
+__global__ void __launch_bounds__(PPB) k_scatter(BinArgs a, int* __restrict__ nrec, int rec_cap) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.gstamp[G_STAMP] += 1;
         a.gstamp[G_PBLK] = 0;
+    }
+    // node-block records: covering block ids -> their item ranges this pass
+    int n_rec = min(a.counters[CNT_NBLK], rec_cap);
+    int stamp = a.counters[CNT_STAMP];
+    for (int t = blockIdx.x * PPB + threadIdx.x; t < n_rec; t += gridDim.x * PPB) {
+        int* r = nrec + (size_t)t * NREC;
+        int pgb[8], bst[8], i0[8], nit[8];
+#pragma unroll
+        for (int d = 0; d < 8; d++) pgb[d] = r[4 + d];
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            int q = max(pgb[d], 0);
+            bst[d] = a.blk_stamp[q];
+            i0[d] = a.blk_item0[q];
+            nit[d] = a.blk_nitems[q];
+        }
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            bool hit = pgb[d] >= 0 && bst[d] == stamp;
+            r[4 + d] = hit ? i0[d] : 0;
+            r[12 + d] = hit ? nit[d] : 0;
+        }
     }
     int4 ch = a.chunks[blockIdx.x];
     int i = threadIdx.x;
@@ -453,6 +497,16 @@ DEV float tsum32(float* r, int lane) {
     tstep_dpp<2, 4, DPP_ROW_HALF_MIRROR>(r, lane);
     tstep_dpp<1, 2, DPP_QUAD_XOR2>(r, lane);
     return r[0] + f_of(__builtin_amdgcn_mov_dpp(u_of(r[0]), DPP_QUAD_XOR1, 0xf, 0xf, false));
+}
+// 16 values: four halving steps (bits 5..2), then the quad sum; afterwards
+// every lane of quad j holds the total of value j
+DEV float tsum16(float* r, int lane) {
+    tstep_swap32<8>(r);
+    tstep_swap16<4>(r);
+    tstep_dpp<2, 8, DPP_ROW_MIRROR>(r, lane);
+    tstep_dpp<1, 4, DPP_ROW_HALF_MIRROR>(r, lane);
+    float s = r[0] + f_of(__builtin_amdgcn_mov_dpp(u_of(r[0]), DPP_QUAD_XOR2, 0xf, 0xf, false));
+    return s + f_of(__builtin_amdgcn_mov_dpp(u_of(s), DPP_QUAD_XOR1, 0xf, 0xf, false));
 }
 DEV int tile_index(int l0, int l1, int l2, int n) { return ((l0 + n / 9) * 6 + (l1 + (n / 3) % 3)) * 6 + (l2 + n % 3); }
 
@@ -521,6 +575,8 @@ struct GridArgs {
     const int* blk_stamp;
     const int* blk_item0;
     const int* blk_nitems;
+    const int* nrec;       // node-block records (NREC ints each) for e < rec_cap
+    int rec_cap;
     const float* pstate;   // [frame][n_prims_total][8]
     int n_prims_total;
     int f;
@@ -543,67 +599,81 @@ DEV void node_coords(const SceneC& S, int gb, int lane, int& B0, int& B1, int& B
 
 DEV int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// owner-computes gather: the node's value is the sum, in a fixed order, of the
-// tiles of the (up to 8) cell blocks whose footprint covers it.  The block
-// metadata is wave-uniform (scalar loads, issued together); the first tile of
-// every covering block is loaded before any is summed, so the dependent-load
-// chain is stamp -> item range -> tile, not eight of them in a row.
-template <int BATCH>
-DEV float4 gather_tiles(const GridArgs& ga, const SceneC& S, int B0, int B1, int B2, int lane, const float4* __restrict__ tiles,
-                        int stamp) {
-    int l0 = lane >> 4, l1 = (lane >> 2) & 3, l2 = lane & 3;
-    int pgb[8], i0[8], ni[8];
+// The active node block e of this pass: its global block id, scene and the
+// work-item ranges (i0, ni) of the 8 covering cell blocks -- from its record
+// when e < rec_cap, else from the block list and the block stamps.  All of it
+// is wave-uniform (scalar loads).
+struct BlockMeta { int gb, sc, i0[8], ni[8]; };
+DEV void read_rec(const GridArgs& ga, int e, BlockMeta& m) {
+    const int* r = ga.nrec + (size_t)e * NREC;
+    m.gb = uniform(r[0]);
+    m.sc = uniform(r[1]);
+#pragma unroll
+    for (int d = 0; d < 8; d++) { m.i0[d] = uniform(r[4 + d]); m.ni[d] = uniform(r[12 + d]); }
+}
+DEV void block_meta(const GridArgs& ga, int e, int stamp, BlockMeta& m) {
+    if (e < ga.rec_cap) {
+        read_rec(ga, e, m);
+        return;
+    }
+    m.gb = uniform(ga.nblocks[e]);
+    m.sc = uniform(ga.blk_scene[m.gb]);
+    const SceneC& S = ga.scenes[m.sc];
+    int B0, B1, B2;
+    block_origin(S, m.gb, B0, B1, B2);
+    int pgb[8], st[8];
 #pragma unroll
     for (int d = 0; d < 8; d++) {
         int P0 = B0 - ((d >> 2) & 1), P1 = B1 - ((d >> 1) & 1), P2 = B2 - (d & 1);
         pgb[d] = (P0 < 0 || P1 < 0 || P2 < 0) ? -1 : S.blk_off + (P0 * S.nb + P1) * S.nb + P2;
     }
-    int st[8];
 #pragma unroll
     for (int d = 0; d < 8; d++) st[d] = pgb[d] >= 0 ? ga.blk_stamp[pgb[d]] : stamp - 1;
 #pragma unroll
     for (int d = 0; d < 8; d++) {
         bool hit = st[d] == stamp;
-        i0[d] = hit ? ga.blk_item0[pgb[d]] : 0;
-        ni[d] = hit ? ga.blk_nitems[pgb[d]] : 0;
+        m.i0[d] = uniform(hit ? ga.blk_item0[pgb[d]] : 0);
+        m.ni[d] = uniform(hit ? ga.blk_nitems[pgb[d]] : 0);
     }
-    float4 v[8];
-    int tidx[8];
+}
+
+// owner-computes gather: the node's value is the sum, in a fixed order, of the
+// tiles of the work items of the (up to 8) cell blocks whose footprint covers
+// it, loaded BATCH at a time across blocks.
+template <int BATCH>
+DEV float4 gather_tiles(const BlockMeta& m, int lane, const float4* __restrict__ tiles) {
+    int l0 = lane >> 4, l1 = (lane >> 2) & 3, l2 = lane & 3;
+    const int* i0 = m.i0;
+    const int* ni = m.ni;
+    // the (block d, item j) pairs in (d, j) order, flattened: pair k belongs to
+    // the block d with pre[d] <= k < pre[d + 1].  BATCH tiles are loaded before
+    // they are summed, in pair order, so at most a few load latencies chain.
+    int pre[8];
+    pre[0] = 0;
 #pragma unroll
-    for (int d = 0; d < 8; d++) {
-        int t0 = l0 + 4 * ((d >> 2) & 1), t1 = l1 + 4 * ((d >> 1) & 1), t2 = l2 + 4 * (d & 1);
-        bool in = t0 < 6 && t1 < 6 && t2 < 6;
-        tidx[d] = in ? (t0 * 6 + t1) * 6 + t2 : -1;
-        v[d] = (ni[d] > 0 && in) ? tiles[(size_t)i0[d] * TILE + tidx[d]] : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    for (int d = 1; d < 8; d++) pre[d] = pre[d - 1] + ni[d - 1];
+    int total = pre[7] + ni[7];
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k0 = 0; k0 < total; k0 += BATCH) {
+        float4 u[BATCH];
 #pragma unroll
-    for (int d = 0; d < 8; d++) {
-        acc.x += v[d].x; acc.y += v[d].y; acc.z += v[d].z; acc.w += v[d].w;
-        // further work items of a block with more than wi particles: their
-        // tiles are loaded BATCH at a time before being summed (in item
-        // order), so the loads' latencies overlap instead of chaining
-        if (BATCH == 1) {
-            if (tidx[d] >= 0)
-                for (int j = 1; j < ni[d]; j++) {
-                    float4 u = tiles[(size_t)(i0[d] + j) * TILE + tidx[d]];
-                    acc.x += u.x; acc.y += u.y; acc.z += u.z; acc.w += u.w;
-                }
-        } else if (tidx[d] >= 0) {
-            const float4* tp = tiles + (size_t)i0[d] * TILE + tidx[d];
-            int j = 1;
-            for (; j + BATCH <= ni[d]; j += BATCH) {
-                float4 u[BATCH];
+        for (int s = 0; s < BATCH; s++) {
+            int k = k0 + s;
+            int d = 0;
 #pragma unroll
-                for (int k = 0; k < BATCH; k++) u[k] = tp[(size_t)(j + k) * TILE];
+            for (int q = 1; q < 8; q++) d += (k >= pre[q]) ? 1 : 0;
+            int item = i0[0] + k;
 #pragma unroll
-                for (int k = 0; k < BATCH; k++) { acc.x += u[k].x; acc.y += u[k].y; acc.z += u[k].z; acc.w += u[k].w; }
-            }
-            for (; j < ni[d]; j++) {
-                float4 u = tp[(size_t)j * TILE];
-                acc.x += u.x; acc.y += u.y; acc.z += u.z; acc.w += u.w;
-            }
+            for (int q = 1; q < 8; q++)
+                if (d == q) item = i0[q] + (k - pre[q]);
+            d = uniform(d);
+            item = uniform(item);
+            int t0 = l0 + 4 * ((d >> 2) & 1), t1 = l1 + 4 * ((d >> 1) & 1), t2 = l2 + 4 * (d & 1);
+            bool in = k < total && t0 < 6 && t1 < 6 && t2 < 6;
+            u[s] = in ? tiles[(size_t)item * TILE + (t0 * 6 + t1) * 6 + t2] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
+#pragma unroll
+        for (int s = 0; s < BATCH; s++) { acc.x += u[s].x; acc.y += u[s].y; acc.z += u[s].z; acc.w += u[s].w; }
     }
     return acc;
 }
@@ -615,18 +685,25 @@ DEV float4 gather_tiles(const GridArgs& ga, const SceneC& S, int B0, int B1, int
 // adjoint then recomputes the grid instead of reading the kept one).
 __global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
                                                  float4* __restrict__ g_v, int only_overflow) {
+    int lane = threadIdx.x & 63;
+    int wave = threadIdx.x >> 6;
+    int e = uniform(blockIdx.x * 4 + wave);
+    // the first block's record is read together with the counters (a record
+    // beyond the active count is stale and left unused)
+    BlockMeta m;
+    bool have = e < ga.rec_cap;
+    if (have) read_rec(ga, e, m);
     if (only_overflow && kept_fits(ga.counters, ga.keep_cap)) return;
     int n_active = ga.counters[CNT_NBLK];
     int stamp = ga.counters[CNT_STAMP];
-    int lane = threadIdx.x & 63;
-    int wave = threadIdx.x >> 6;
-    for (int e = uniform(blockIdx.x * 4 + wave); e < n_active; e += gridDim.x * 4) {
-        int gb = uniform(ga.nblocks[e]);
-        const SceneC& S = ga.scenes[uniform(ga.blk_scene[gb])];
+    for (; e < n_active; e += gridDim.x * 4) {
+        if (!have) block_meta(ga, e, stamp, m);
+        have = false;
+        const SceneC& S = ga.scenes[m.sc];
         int B0, B1, B2, I0, I1, I2;
         long long node;
-        node_coords(S, gb, lane, B0, B1, B2, I0, I1, I2, node);
-        float4 pm = gather_tiles<8>(ga, S, B0, B1, B2, lane, tiles, stamp);
+        node_coords(S, m.gb, lane, B0, B1, B2, I0, I1, I2, node);
+        float4 pm = gather_tiles<8>(m, lane, tiles);
         v3 v = mk(0.f, 0.f, 0.f);
         if (pm.w > 1e-12f && in_grid(S, I0, I1, I2)) {
             const float* ps0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
@@ -862,29 +939,31 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
     }
 }
 
-DEV float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
 // reverse of grid_op: adjoint of (momentum, mass) per node, plus the adjoint
 // of the primitive states of frames f and f+1 (wave-reduced, then atomics).
 __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* __restrict__ g_pm, const float4* __restrict__ tiles_v,
                                                      float4* __restrict__ g_adjpm, float* __restrict__ gpstate, int use_kept) {
+    int lane = threadIdx.x & 63;
+    int wave = threadIdx.x >> 6;
+    int e = uniform(blockIdx.x * 4 + wave);
+    // the first block's record is read together with the counters (a record
+    // beyond the active count is stale and left unused)
+    BlockMeta m;
+    bool have = e < ga.rec_cap;
+    if (have) read_rec(ga, e, m);
     int n_active = ga.counters[CNT_NBLK];
     bool kept = use_kept && kept_fits(ga.counters, ga.keep_cap);
     int stamp = ga.counters[CNT_STAMP];
-    int lane = threadIdx.x & 63;
-    int wave = threadIdx.x >> 6;
-    for (int e = uniform(blockIdx.x * 4 + wave); e < n_active; e += gridDim.x * 4) {
-        int gb = uniform(ga.nblocks[e]);
-        const SceneC& S = ga.scenes[uniform(ga.blk_scene[gb])];
+    for (; e < n_active; e += gridDim.x * 4) {
+        float4 pm = kept ? ga.keep_pm[(size_t)e * NODES_PER_BLK + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!have) block_meta(ga, e, stamp, m);
+        have = false;
+        const SceneC& S = ga.scenes[m.sc];
         int B0, B1, B2, I0, I1, I2;
         long long node;
-        node_coords(S, gb, lane, B0, B1, B2, I0, I1, I2, node);
-        float4 pm = kept ? ga.keep_pm[(size_t)e * NODES_PER_BLK + lane] : g_pm[node];
-        float4 gv4 = gather_tiles<1>(ga, S, B0, B1, B2, lane, tiles_v, stamp);
+        node_coords(S, m.gb, lane, B0, B1, B2, I0, I1, I2, node);
+        if (!kept) pm = g_pm[node];
+        float4 gv4 = gather_tiles<8>(m, lane, tiles_v);
         bool act = pm.w > 1e-12f && in_grid(S, I0, I1, I2);
         const float* ps0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
         const float* ps1 = ga.pstate + ((size_t)(ga.f + 1) * ga.n_prims_total + S.prim_off) * 8;
@@ -950,11 +1029,10 @@ __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* 
 #pragma unroll
                 for (int k = 0; k < 16; k++) any |= (vals[k] != 0.f);
                 if (__ballot(any) == 0ull) continue;
-#pragma unroll
-                for (int k = 0; k < 16; k++) {
-                    float s = wave_sum(vals[k]);
-                    if (lane == 0 && s != 0.f) unsafeAtomicAdd((k < 8 ? gps0 : gps1) + q * 8 + (k & 7), s);
-                }
+                // the 16 wave sums at once; quad k adds value k
+                float s = tsum16(vals, lane);
+                int k = lane >> 2;
+                if ((lane & 3) == 0 && s != 0.f) unsafeAtomicAdd((k < 8 ? gps0 : gps1) + q * 8 + (k & 7), s);
             }
         }
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1639,13 +1717,14 @@ struct mpm_ctx {
     // slot, so the adjoint of a substep the forward ran reuses both instead of
     // binning and rebuilding the grid again; otherwise one shared slot.
     struct BinSlot {
-        int* order; int4* items; int* blk_stamp; int* blk_item0; int* blk_nitems; int* nblocks; int* nmap; int* counters;
+        int* order; int4* items; int* blk_stamp; int* blk_item0; int* blk_nitems; int* nblocks; int* nmap; int* counters; int* nrec;
         float4* keep_pm; float4* keep_v;
     };
     std::vector<BinSlot> bslots;
     std::vector<int> kept_of;  // substep whose bins + grid each bin slot holds (-1: none)
     bool keep = false;
     int keep_cap = 0;          // node blocks a kept grid can hold
+    int rec_cap = 0;           // node-block records per bin slot
     float4 *tiles_pm = nullptr, *tiles_v = nullptr;
     int max_items = 0;
     int wi = WI;          // particles per work item
@@ -1714,6 +1793,7 @@ struct mpm_ctx {
         GridArgs g;
         g.scenes = d_scenes; g.prims = d_prims; g.blk_scene = d_blk_scene; g.nblocks = b.nblocks; g.counters = b.counters;
         g.blk_stamp = b.blk_stamp; g.blk_item0 = b.blk_item0; g.blk_nitems = b.blk_nitems;
+        g.nrec = b.nrec; g.rec_cap = rec_cap;
         g.pstate = pstate; g.n_prims_total = n_prims; g.f = f;
         g.keep_pm = b.keep_pm;
         g.keep_v = b.keep_v;
@@ -1746,8 +1826,8 @@ struct Layout {
     size_t off_scenes, off_prims, off_chunks, off_blk_scene, off_ckpt, off_ring, off_aff, off_usv, off_adj, off_grid, off_blkints, off_cells,
         off_pints, off_bslots, off_keep, off_tiles, off_counters, off_prim_bufs, total;
     // one bin slot: order, items, blk_stamp, blk_item0, blk_nitems, nblocks, nmap, counters
-    size_t bs_order, bs_items, bs_stamp, bs_item0, bs_nitems, bs_nblocks, bs_nmap, bs_counters, bs_bytes, keep_bytes;
-    int stride, n_chunks, n_blocks, n_ckpt, ring, K, max_items, n_bslots, keep_cap;
+    size_t bs_order, bs_items, bs_stamp, bs_item0, bs_nitems, bs_nblocks, bs_nmap, bs_counters, bs_nrec, bs_bytes, keep_bytes;
+    int stride, n_chunks, n_blocks, n_ckpt, ring, K, max_items, n_bslots, keep_cap, rec_cap;
     long long n_nodes;
     int n_prims;
 };
@@ -1802,6 +1882,9 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     // list does not fit falls back to recomputing its grid in the adjoint
     int cap = keep_blocks < 0 ? stride / 64 + 64 * n : keep_blocks;
     L.keep_cap = std::min(cap, L.n_blocks);
+    // node-block records: the same generous count (passes with more active
+    // node blocks look the rest up from the block stamps)
+    L.rec_cap = std::min(stride / 64 + 64 * n, L.n_blocks);
     L.n_bslots = L.keep_cap > 0 ? L.n_ckpt + L.ring : 1;
     size_t b = 0;
     L.bs_order = b; b = align256(b + sizeof(int) * (size_t)stride);
@@ -1812,6 +1895,7 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     L.bs_nblocks = b; b = align256(b + sizeof(int) * (size_t)L.n_blocks);
     L.bs_nmap = b; b = align256(b + sizeof(int) * (size_t)L.n_blocks);
     L.bs_counters = b; b = align256(b + sizeof(int) * 64);
+    L.bs_nrec = b; b = align256(b + sizeof(int) * NREC * (size_t)L.rec_cap);
     L.bs_bytes = b;
     L.keep_bytes = (size_t)L.keep_cap * NODES_PER_BLK * sizeof(float4) * 2;
     L.off_bslots = o; o = align256(o + L.bs_bytes * L.n_bslots);
@@ -1842,9 +1926,9 @@ int run_bin(mpm_ctx* c, int f, int do_fk, hipStream_t st) {
     { PhaseTimer t(c, MPM_PH_BIN, st);
       hipLaunchKernelGGL(k_bin, dim3(c->n_chunks), dim3(PPB), 0, st, ba, in, do_fk, c->d_prims, c->n_prims, c->pstate, c->pvel, f); }
     { PhaseTimer t(c, MPM_PH_BLOCKS, st);
-      hipLaunchKernelGGL(k_blocks, dim3(c->grid_ctas), dim3(256), 0, st, ba, B.nblocks, B.nmap); }
+      hipLaunchKernelGGL(k_blocks, dim3(c->grid_ctas), dim3(256), 0, st, ba, B.nblocks, B.nmap, B.nrec, c->rec_cap); }
     { PhaseTimer t(c, MPM_PH_SCATTER, st);
-      hipLaunchKernelGGL(k_scatter, dim3(c->n_chunks), dim3(PPB), 0, st, ba); }
+      hipLaunchKernelGGL(k_scatter, dim3(c->n_chunks), dim3(PPB), 0, st, ba, B.nrec, c->rec_cap); }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -2088,6 +2172,7 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->bin_rank = c->bin_q + L.stride;
     c->keep = L.keep_cap > 0;
     c->keep_cap = L.keep_cap;
+    c->rec_cap = L.rec_cap;
     for (int k = 0; k < L.n_bslots; k++) {
         char* b = w + L.off_bslots + (size_t)k * L.bs_bytes;
         mpm_ctx::BinSlot S;
@@ -2099,6 +2184,7 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
         S.nblocks = (int*)(b + L.bs_nblocks);
         S.nmap = (int*)(b + L.bs_nmap);
         S.counters = (int*)(b + L.bs_counters);
+        S.nrec = (int*)(b + L.bs_nrec);
         S.keep_pm = S.keep_v = nullptr;
         if (c->keep) {
             S.keep_pm = (float4*)(w + L.off_keep + (size_t)k * L.keep_bytes);
