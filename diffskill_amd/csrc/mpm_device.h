@@ -101,6 +101,36 @@ DEV void conj_n_rev(q4 q, q4 g, q4* gq) {
     gq->y -= g.y / n - c.y * d / n3;
     gq->z -= g.z / n - c.z * d / n3;
 }
+// rotation matrix (row-major) <-> unit quaternion (w, x, y, z), the storage form
+// of the stored SVD's U and V: Shepperd's method divides by the largest of
+// 4w^2, 4x^2, 4y^2, 4z^2; the matrix of a quaternion is normalized by |q|^2
+DEV q4 rot_to_quat(const float* R) {
+    // t = (4w^2, 4x^2, 4y^2, 4z^2); k = the largest; every lane takes the same
+    // instruction path (selects, one square root, one reciprocal)
+    float t0 = 1.f + R[0] + R[4] + R[8], t1 = 1.f + R[0] - R[4] - R[8];
+    float t2 = 1.f - R[0] + R[4] - R[8], t3 = 1.f - R[0] - R[4] + R[8];
+    float a = R[7] - R[5], b = R[2] - R[6], c = R[3] - R[1];   // 4wx, 4wy, 4wz
+    float d = R[1] + R[3], e = R[2] + R[6], g = R[5] + R[7];   // 4xy, 4xz, 4yz
+    int k = 0;
+    float tm = t0;
+    if (t1 > tm) { k = 1; tm = t1; }
+    if (t2 > tm) { k = 2; tm = t2; }
+    if (t3 > tm) { k = 3; tm = t3; }
+    float s = sqrtf(tm) * 2.f;   // 4 |q_k|
+    float h = 0.25f * s, inv = __builtin_amdgcn_rcpf(s);
+    if (k == 0) return mkq(h, a * inv, b * inv, c * inv);
+    if (k == 1) return mkq(a * inv, h, d * inv, e * inv);
+    if (k == 2) return mkq(b * inv, d * inv, h, g * inv);
+    return mkq(c * inv, e * inv, g * inv, h);
+}
+DEV void quat_to_rot(q4 q, float* R) {
+    float s = 2.f / (q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+    float xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z;
+    float xy = q.x * q.y, xz = q.x * q.z, yz = q.y * q.z, wx = q.w * q.x, wy = q.w * q.y, wz = q.w * q.z;
+    R[0] = 1.f - s * (yy + zz); R[1] = s * (xy - wz);        R[2] = s * (xz + wy);
+    R[3] = s * (xy + wz);        R[4] = 1.f - s * (xx + zz); R[5] = s * (yz - wx);
+    R[6] = s * (xz - wy);        R[7] = s * (yz + wx);        R[8] = 1.f - s * (xx + yy);
+}
 // qmul (utils.py:18-25): Hamilton product, renormalized
 DEV q4 qmul_raw(q4 q, q4 r) {
     return mkq(r.w * q.w - r.x * q.x - r.y * q.y - r.z * q.z,

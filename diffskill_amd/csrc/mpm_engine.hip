@@ -144,8 +144,24 @@ DEV void particle_stress(const SceneC& S, const m3& C, const m3& F, PTrace& t) {
     svd3(t.Ftmp, t.U, t.sig, t.V);
     stress_from_svd(S, C, t);
 }
-// rows of the stored SVD of F_tmp per frame slot: U (row-major), sigma, V
-constexpr int USV_ROWS = 21;
+// rows of the stored SVD of F_tmp per frame slot: U as a unit quaternion (U and
+// V are proper rotations: products of Givens / Jacobi rotations, column swaps
+// with a sign flip), sigma, V as a unit quaternion
+constexpr int USV_ROWS = 11;
+DEV void store_usv(float* usv, int stride, int p, const PTrace& t) {
+    q4 qu = rot_to_quat(t.U.a), qv = rot_to_quat(t.V.a);
+    float r[USV_ROWS] = {qu.w, qu.x, qu.y, qu.z, t.sig[0], t.sig[1], t.sig[2], qv.w, qv.x, qv.y, qv.z};
+#pragma unroll
+    for (int k = 0; k < USV_ROWS; k++) usv[(size_t)k * stride + p] = r[k];
+}
+DEV void load_usv(const float* usv, int stride, int p, PTrace& t) {
+    float r[USV_ROWS];
+#pragma unroll
+    for (int k = 0; k < USV_ROWS; k++) r[k] = usv[(size_t)k * stride + p];
+    quat_to_rot(mkq(r[0], r[1], r[2], r[3]), t.U.a);
+    t.sig[0] = r[4]; t.sig[1] = r[5]; t.sig[2] = r[6];
+    quat_to_rot(mkq(r[7], r[8], r[9], r[10]), t.V.a);
+}
 
 // grid_op for one node with m > 1e-12 (mpm_simulator.py:230-262)
 struct NodeTr { v3 vu[MAX_UNITS]; v3 vw[3]; };
@@ -1091,9 +1107,7 @@ __global__ void __launch_bounds__(PPB) k_stress(ChunkArgs ca, const float* __res
     particle_stress(S, C, F, t);
     if (st_out) store9(st_out, ca.stride, 15, slot, t.Fn);
     store9(aff, ca.stride, 0, slot, t.affine);
-    store9(usv, ca.stride, 0, slot, t.U);
-    store3(usv, ca.stride, 9, slot, mk(t.sig[0], t.sig[1], t.sig[2]));
-    store9(usv, ca.stride, 12, slot, t.V);
+    store_usv(usv, ca.stride, slot, t);
 }
 
 // p2g scatter over work items (mpm_simulator.py:217-225): momentum and mass of
@@ -1281,10 +1295,7 @@ __global__ void __launch_bounds__(PPB) k_stress_bwd(ChunkArgs ca, const float* _
     m3 F = load9(st, stride, 15, p);
     PTrace t;
     t.Ftmp = f_tmp(S, C, F);
-    t.U = load9(usv, stride, 0, p);
-    v3 sg = load3(usv, stride, 9, p);
-    t.sig[0] = sg.x; t.sig[1] = sg.y; t.sig[2] = sg.z;
-    t.V = load9(usv, stride, 12, p);
+    load_usv(usv, stride, p, t);
     stress_from_svd(S, C, t);
     m3 gaff = load9(gaff_in, stride, 0, p);
     m3 gC, gst;
@@ -1703,7 +1714,7 @@ struct mpm_ctx {
     float* ckpt = nullptr;
     float* ringbuf = nullptr;
     float* aff = nullptr;   // affine matrix (9 rows) per frame slot: checkpoints then ring
-    float* usv = nullptr;   // SVD of F_tmp (U, sigma, V: 21 rows) per frame slot, valid with aff
+    float* usv = nullptr;   // SVD of F_tmp (quaternion of U, sigma, quaternion of V: 11 rows) per frame slot, valid with aff
     float* gaff = nullptr;  // adjoint of the affine matrix (9 rows), one frame
     std::vector<int> aff_of; // frame whose affine matrix each slot holds (-1: none)
     float* adj[2] = {nullptr, nullptr};
