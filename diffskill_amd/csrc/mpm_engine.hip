@@ -1752,11 +1752,6 @@ struct mpm_ctx {
     // event fork and join, so the two overlap (parallel branches in a graph)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    // adjoint lookahead: the overflow-fallback launches of substep f-1 run on
-    // the side stream while substep f finishes (ev_fb_fork after f's grid
-    // adjoint, ev_fb_join after the fallbacks); fb_issued is that f-1
-    hipEvent_t ev_fb_fork = nullptr, ev_fb_join = nullptr;
-    int fb_issued = -1;
     // tool-tool collision pairs (mpm_set_collisions); tables and per-frame
     // records live in a caller-owned device buffer
     int n_pairs = 0, n_cgroups = 0;
@@ -1996,13 +1991,6 @@ int ensure_frame(mpm_ctx* c, int f, hipStream_t st) {
     return 0;
 }
 
-// true when frame f is resident without recomputation
-bool frame_resident(const mpm_ctx* c, int f) {
-    if (f < 0 || f > c->valid_hi) return false;
-    if (f % c->K == 0) return true;
-    return c->seg_valid == f / c->K && c->seg_lo <= f && f <= c->seg_hi;
-}
-
 // the kept-grid overflow fallbacks of substep f (p2g then grid_op, each a
 // no-op unless the pass's active node blocks outgrew the kept grid)
 int launch_fallbacks(mpm_ctx* c, int f, hipStream_t s) {
@@ -2018,7 +2006,7 @@ int launch_fallbacks(mpm_ctx* c, int f, hipStream_t s) {
     return 0;
 }
 
-int run_bwd(mpm_ctx* c, int f, hipStream_t st, int f_stop) {
+int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     if (ensure_frame(c, f + 1, st)) return -1;
     if (ensure_frame(c, f, st)) return -1;
     const float* in = c->frame_ptr(f);
@@ -2043,11 +2031,11 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st, int f_stop) {
     }
     // rebuild the grid when it was not kept; a kept grid whose active list
     // overflowed the capacity is rebuilt on the device (only_overflow launches,
-    // small persistent grids so that their no-op case dispatches quickly),
-    // possibly already issued on the side stream by substep f+1's adjoint
-    if (c->fb_issued == f && kept) {
-        HIPCHK(hipStreamWaitEvent(st, c->ev_fb_join, 0));
-    } else if (!kept) {
+    // small persistent grids so that their no-op case dispatches quickly).
+    // They stay on this stream: moved to the side stream (overlapping the rest
+    // of substep f+1) the graph's cross-stream fork and join cost more idle
+    // time than the two no-op launches themselves (80.4 vs 79.2 ms per pass).
+    if (!kept) {
         { PhaseTimer t(c, MPM_PH_P2G_GRID, st);
           hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm, 0); }
         { PhaseTimer t(c, MPM_PH_GRID_KEEP, st);
@@ -2055,24 +2043,12 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st, int f_stop) {
     } else if (c->keep_cap < c->n_blocks) {
         if (launch_fallbacks(c, f, st)) return -1;
     }
-    c->fb_issued = -1;
     int use_kept = kept ? 1 : 0;
     { PhaseTimer t(c, MPM_PH_G2P_BWD, st);
       hipLaunchKernelGGL(k_g2p_bwd_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, in1, a1, a0, c->g_v, c->tiles_v, use_kept); }
     { PhaseTimer t(c, MPM_PH_GRID_BWD, st);
       hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->g_pm, c->tiles_v, c->g_adjpm, c->pstate_g,
                          use_kept); }
-    // substep f-1's fallbacks read frame f-1 and write only the dense grid,
-    // which this substep no longer uses: run them beside the rest of it
-    int g = f - 1;
-    if (!c->prof && g >= f_stop && c->keep && c->keep_cap < c->n_blocks && c->kept_of[c->bslot_of(g)] == g &&
-        c->aff_of[c->slot_of(g)] == g && frame_resident(c, g)) {
-        HIPCHK(hipEventRecord(c->ev_fb_fork, st));
-        HIPCHK(hipStreamWaitEvent(c->side, c->ev_fb_fork, 0));
-        if (launch_fallbacks(c, g, c->side)) return -1;
-        HIPCHK(hipEventRecord(c->ev_fb_join, c->side));
-        c->fb_issued = g;
-    }
     if (c->n_pairs > 0) {
         // between the grid adjoint (which adds into frame f+1's primitive
         // adjoint) and the kinematics adjoint (which consumes it)
@@ -2287,9 +2263,7 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->tile_ctas = std::max(64, ncu * 4);
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fb_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fb_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
         mpm_destroy(c);
         return fail("mpm_create: stream/event creation failed");
     }
@@ -2302,8 +2276,6 @@ int mpm_destroy(mpm_ctx* ctx) {
     if (ctx->side) { hipStreamSynchronize(ctx->side); hipStreamDestroy(ctx->side); }
     if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
-    if (ctx->ev_fb_fork) hipEventDestroy(ctx->ev_fb_fork);
-    if (ctx->ev_fb_join) hipEventDestroy(ctx->ev_fb_join);
     for (auto& r : ctx->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
     for (auto e : ctx->ev_pool) hipEventDestroy(e);
     delete ctx;
@@ -2480,9 +2452,8 @@ int mpm_substep_bwd(mpm_ctx* ctx, int s, int count, void* stream) {
         return fail("mpm_substep_bwd: adjoint order violated (current adjoint frame " + std::to_string(ctx->adj_top) + ", expected " +
                     std::to_string(s + count) + ")");
     hipStream_t st = S_(stream);
-    ctx->fb_issued = -1;
     for (int f = s + count - 1; f >= s; f--) {
-        if (run_bwd(ctx, f, st, s)) return -1;
+        if (run_bwd(ctx, f, st)) return -1;
         ctx->adj_top = f;
     }
     return 0;
