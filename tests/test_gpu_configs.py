@@ -155,15 +155,19 @@ def test_config3_checkpointed_equals_store_all():
     difference| < 1e-4 and loss allclose, the reference's own check of its
     checkpointed recompute (long_term_gradient.ipynb:255-257).  Float atomics
     make even two identical runs differ in rounding, which 1000 contact-rich
-    substeps amplify; the relative difference is bounded by 1e-3 or 5x that
-    measured run-to-run spread, whichever is larger."""
+    substeps amplify: identical store-all runs, and checkpointed against
+    store-all runs, have measured relative gradient differences anywhere from
+    6e-5 to 2e-3 (the same K differing by 30x between repeats), so the
+    relative difference is bounded by 5e-3 or 5x the spread of two repeated
+    store-all runs, whichever is larger."""
     from diffskill_amd import workloads
     w = workloads.config3()
     H = w["H"]
     loss0, g0, ws0 = _config3_grads(w, None, H)
     loss0b, g0b, _ = _config3_grads(w, None, H)
-    noise = rel(g0b, g0)
-    print("store-all run-to-run: loss %.8g vs %.8g, grad rel %.3g" % (loss0, loss0b, noise))
+    loss0c, g0c, _ = _config3_grads(w, None, H)
+    noise = max(rel(g0b, g0), rel(g0c, g0))
+    print("store-all run-to-run: loss %.8g vs %.8g vs %.8g, grad rel %.3g" % (loss0, loss0b, loss0c, noise))
     assert np.isfinite(g0).all() and np.abs(g0).max() > 0
     for K in (10, 50, 100):
         print("case config3 ckpt_k=%d" % K, flush=True)
@@ -172,7 +176,7 @@ def test_config3_checkpointed_equals_store_all():
         print("K %d: max abs grad difference %.3g (max |grad| %.3g)" % (K, np.abs(g - g0).max(), np.abs(g0).max()))
         assert np.isclose(loss, loss0, rtol=1e-5)
         assert np.abs(g - g0).max() < 1e-4
-        assert rel(g, g0) < max(1e-3, 5 * noise)
+        assert rel(g, g0) < max(5e-3, 5 * noise)
         assert ws < 0.5 * ws0
 
 
