@@ -107,8 +107,10 @@ class BatchedTrajOpt:
 
     One ``iteration()`` = restart -> H env steps x ``substeps`` forward
     substeps (all scenes) -> mass grid -> L1 adjoint -> all adjoint substeps
-    -> action adjoints -> Adam step.  With ``use_graph`` the device part is
-    captured once as a HIP graph and replayed.
+    -> action adjoints -> Adam step.  With ``use_graph`` the whole iteration,
+    the Adam step included (``capturable`` Adam, its state created up front),
+    is captured once as a HIP graph and replayed, so no host work sits
+    between iterations.
     """
 
     def __init__(self, batch, H, substeps, targets, lr=1e-2, init_actions=None, use_graph=True):
@@ -121,7 +123,15 @@ class BatchedTrajOpt:
         a = torch.zeros((self.H, P, 7), dtype=torch.float32, device="cuda") if init_actions is None else \
             init_actions.to("cuda", torch.float32).clone()
         self.actions = torch.nn.Parameter(a)
-        self.opt = torch.optim.Adam([self.actions], lr=lr)
+        self.actions.grad = torch.zeros_like(a)
+        self.opt = torch.optim.Adam([self.actions], lr=lr, capturable=bool(use_graph))
+        if use_graph:
+            # the state Adam would create lazily on its first step, created
+            # now so that the captured step only updates it in place
+            st = self.opt.state[self.actions]
+            st["step"] = torch.zeros((), dtype=torch.float32, device="cuda")
+            st["exp_avg"] = torch.zeros_like(a)
+            st["exp_avg_sq"] = torch.zeros_like(a)
         n = batch.consts[0]["n_grid"]
         self.mass = torch.zeros((batch.n_scenes, n, n, n), dtype=torch.float32, device="cuda")
         self.gmass = torch.zeros_like(self.mass)
@@ -156,8 +166,13 @@ class BatchedTrajOpt:
             raise RuntimeError("autotune must run before the pass is captured into a graph")
         return autotune(self.b.engine, self._device_pass, candidates, reps)
 
+    def _step(self):
+        torch.mul(self.grads, self.b.action_mask, out=self.actions.grad)
+        self.opt.step()
+
     def _capture(self):
-        # warm up on a side stream, then capture
+        # warm up the device pass on a side stream (the Adam step is not
+        # warmed up: that would move the actions), then capture pass + step
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
@@ -166,6 +181,7 @@ class BatchedTrajOpt:
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self._device_pass()
+            self._step()
         self.graph = g
 
     def iteration(self):
@@ -175,8 +191,7 @@ class BatchedTrajOpt:
             self.graph.replay()
         else:
             self._device_pass()
-        self.actions.grad = self.grads * self.b.action_mask
-        self.opt.step()
+            self._step()
 
     def losses(self):
         return (self.mass - self.targets).abs().sum(dim=(1, 2, 3))
