@@ -1888,9 +1888,10 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     // list does not fit falls back to recomputing its grid in the adjoint
     int cap = keep_blocks < 0 ? stride / 64 + 64 * n : keep_blocks;
     L.keep_cap = std::min(cap, L.n_blocks);
-    // node-block records: the same generous count (passes with more active
-    // node blocks look the rest up from the block stamps)
-    L.rec_cap = std::min(stride / 64 + 64 * n, L.n_blocks);
+    // node-block records: the same generous count, or a caller-given kept-grid
+    // capacity (passes with more active node blocks look the rest up from the
+    // block stamps, the path a small capacity exercises)
+    L.rec_cap = std::min(keep_blocks > 0 ? keep_blocks : stride / 64 + 64 * n, L.n_blocks);
     L.n_bslots = L.keep_cap > 0 ? L.n_ckpt + L.ring : 1;
     size_t b = 0;
     L.bs_order = b; b = align256(b + sizeof(int) * (size_t)stride);
