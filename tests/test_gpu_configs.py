@@ -97,7 +97,7 @@ def _grid_l1_grads(w, target, ckpt_k=None, H=None):
         obs = gm.forward(s, a[s], *obs)
     loss = (obs[2] - target).abs().sum()
     loss.backward()
-    return float(loss), a.grad.detach().cpu().double().numpy()
+    return float(loss.detach()), a.grad.detach().cpu().double().numpy()
 
 
 def _oracle_grid_l1(w, target, H=None, bits=64):
