@@ -50,7 +50,7 @@ EXPORTS = [
     "mpm_update_prims", "mpm_copy_frame", "mpm_set_profiling", "mpm_profile_read", "mpm_last_error",
     "mpm_set_launch", "mpm_get_launch", "mpm_min_dist_cols", "mpm_min_dist", "mpm_min_dist_bwd",
     "mpm_collision_bytes", "mpm_set_collisions", "mpm_get_collisions",
-    "pc_nearest", "pc_last_error",
+    "pc_nearest", "pc_last_error", "pc_sinkhorn_work_bytes", "pc_softmin", "pc_plan_grad",
 ]
 
 PHASES = ["bin", "compact", "blocks", "scatter", "stress", "p2g", "grid_op", "g2p", "p2g_grid", "grid_op_keep",
@@ -106,6 +106,9 @@ def load():
         "mpm_profile_read": (i32, [vp, vp, vp, i32]),
         "mpm_last_error": (ctypes.c_char_p, []),
         "pc_nearest": (i32, [vp, i32, vp, i32, vp, vp, vp, vp]),
+        "pc_sinkhorn_work_bytes": (ctypes.c_longlong, [i32, i32]),
+        "pc_softmin": (i32, [vp, i32, vp, i32, vp, ctypes.c_float, i32, vp, vp, vp]),
+        "pc_plan_grad": (i32, [vp, i32, vp, i32, vp, vp, ctypes.c_float, i32, vp, vp, vp]),
         "pc_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():

@@ -10,10 +10,14 @@
 // non-negative, so their bit patterns order like the values, and equal
 // distances keep the smaller index.
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <string>
 
 #include "../../include/diffskill_pointcloud.h"
+
+#define DEV_PC __device__ __forceinline__
 
 namespace {
 
@@ -81,9 +85,183 @@ __global__ void __launch_bounds__(PC_WG) k_pc_unpack(const unsigned long long* _
     d2[i] = none ? __int_as_float(0x7f800000) : __uint_as_float((unsigned)(b >> 32));
 }
 
+// ---------------------------------------------------------------- Sinkhorn
+// The soft-min of the Sinkhorn divergence (geomloss SamplesLoss('sinkhorn'),
+// the EMD loss of plb/engine/taichi_env.py:23-26) and the gradient of its
+// transport plan, over all (row, column) pairs:
+//   softmin_i = -eps log sum_j exp(h_j - C(x_i, y_j) / eps)
+//   grad_i    = sum_j exp(h_j - C(x_i, y_j) / eps + r_i) dC/dx_i
+// with C = |x - y| (p = 1) or |x - y|^2 / 2 (p = 2).  Base-2 arithmetic: the
+// host passes h and r in log2 units (h * log2 e), the kernels scale the cost
+// by log2(e) / eps and exponentiate with v_exp_f32.  A workgroup takes 256
+// rows (one per lane) and one column range (a "split") staged through LDS;
+// each lane keeps a running (max, scaled sum) per block of 4 columns, and the
+// splits' partials are combined in split order by a second kernel, so the
+// result does not depend on scheduling.
+constexpr int SK_CHUNK = 2048;   // columns per LDS pass (32 KB)
+
+DEV_PC float sk_cost(float dx, float dy, float dz, int p) {
+    float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    return p == 1 ? sqrtf(d2) : 0.5f * d2;
+}
+
+__global__ void __launch_bounds__(PC_WG) k_sk_softmin(const float* __restrict__ x, int n, const float* __restrict__ y, int m,
+                                                     const float* __restrict__ h2, float k2, int p, int per_split,
+                                                     float2* __restrict__ part) {
+    __shared__ float sx[SK_CHUNK], sy[SK_CHUNK], sz[SK_CHUNK], sh[SK_CHUNK];
+    int i = blockIdx.x * PC_WG + threadIdx.x;
+    int c0 = blockIdx.y * per_split, c1 = min(m, c0 + per_split);
+    float xi = 0.f, yi = 0.f, zi = 0.f;
+    if (i < n) { xi = x[(size_t)i * 3]; yi = x[(size_t)i * 3 + 1]; zi = x[(size_t)i * 3 + 2]; }
+    float mx = -INFINITY, sm = 0.f;
+    for (int b0 = c0; b0 < c1; b0 += SK_CHUNK) {
+        int nc = min(SK_CHUNK, c1 - b0);
+        __syncthreads();
+        for (int k = threadIdx.x; k < nc; k += PC_WG) {
+            const float* q = y + (size_t)(b0 + k) * 3;
+            sx[k] = q[0]; sy[k] = q[1]; sz[k] = q[2]; sh[k] = h2[b0 + k];
+        }
+        __syncthreads();
+        int k = 0;
+        for (; k + 4 <= nc; k += 4) {
+            float z[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) z[u] = sh[k + u] - k2 * sk_cost(xi - sx[k + u], yi - sy[k + u], zi - sz[k + u], p);
+            float bm = fmaxf(fmaxf(z[0], z[1]), fmaxf(z[2], z[3]));
+            float nm = fmaxf(mx, bm);
+            float acc = 0.f;
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc += __builtin_amdgcn_exp2f(z[u] - nm);
+            sm = sm * __builtin_amdgcn_exp2f(mx - nm) + acc;
+            mx = nm;
+        }
+        for (; k < nc; k++) {
+            float z = sh[k] - k2 * sk_cost(xi - sx[k], yi - sy[k], zi - sz[k], p);
+            float nm = fmaxf(mx, z);
+            sm = sm * __builtin_amdgcn_exp2f(mx - nm) + __builtin_amdgcn_exp2f(z - nm);
+            mx = nm;
+        }
+    }
+    if (i < n) part[(size_t)blockIdx.y * n + i] = make_float2(mx, sm);
+}
+
+// combine the splits: softmin_i = -eps ln2 (M + log2 sum_s s_s 2^(m_s - M))
+__global__ void __launch_bounds__(PC_WG) k_sk_softmin_combine(const float2* __restrict__ part, int n, int splits, float eps,
+                                                             float* __restrict__ out) {
+    int i = blockIdx.x * PC_WG + threadIdx.x;
+    if (i >= n) return;
+    float M = -INFINITY;
+    for (int s = 0; s < splits; s++) M = fmaxf(M, part[(size_t)s * n + i].x);
+    float S = 0.f;
+    for (int s = 0; s < splits; s++) {
+        float2 v = part[(size_t)s * n + i];
+        S += v.y * __builtin_amdgcn_exp2f(v.x - M);
+    }
+    out[i] = -eps * 0.69314718055994531f * (M + __builtin_amdgcn_logf(S));
+}
+
+__global__ void __launch_bounds__(PC_WG) k_sk_plan_grad(const float* __restrict__ x, int n, const float* __restrict__ y, int m,
+                                                       const float* __restrict__ h2, const float* __restrict__ r2, float k2, int p,
+                                                       int per_split, float4* __restrict__ part) {
+    __shared__ float sx[SK_CHUNK], sy[SK_CHUNK], sz[SK_CHUNK], sh[SK_CHUNK];
+    int i = blockIdx.x * PC_WG + threadIdx.x;
+    int c0 = blockIdx.y * per_split, c1 = min(m, c0 + per_split);
+    float xi = 0.f, yi = 0.f, zi = 0.f, ri = 0.f;
+    if (i < n) { xi = x[(size_t)i * 3]; yi = x[(size_t)i * 3 + 1]; zi = x[(size_t)i * 3 + 2]; ri = r2[i]; }
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    for (int b0 = c0; b0 < c1; b0 += SK_CHUNK) {
+        int nc = min(SK_CHUNK, c1 - b0);
+        __syncthreads();
+        for (int k = threadIdx.x; k < nc; k += PC_WG) {
+            const float* q = y + (size_t)(b0 + k) * 3;
+            sx[k] = q[0]; sy[k] = q[1]; sz[k] = q[2]; sh[k] = h2[b0 + k];
+        }
+        __syncthreads();
+        for (int k = 0; k < nc; k++) {
+            float dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
+            float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            float c, w;
+            if (p == 1) {
+                float d = sqrtf(d2);
+                c = d;
+                w = d > 0.f ? 1.f / d : 0.f;   // dC/dx = (x - y) / |x - y|, 0 at coincident points
+            } else {
+                c = 0.5f * d2;
+                w = 1.f;
+            }
+            float e = __builtin_amdgcn_exp2f(sh[k] + ri - k2 * c) * w;
+            gx = fmaf(e, dx, gx); gy = fmaf(e, dy, gy); gz = fmaf(e, dz, gz);
+        }
+    }
+    if (i < n) part[(size_t)blockIdx.y * n + i] = make_float4(gx, gy, gz, 0.f);
+}
+
+__global__ void __launch_bounds__(PC_WG) k_sk_grad_combine(const float4* __restrict__ part, int n, int splits, float* __restrict__ g) {
+    int i = blockIdx.x * PC_WG + threadIdx.x;
+    if (i >= n) return;
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    for (int s = 0; s < splits; s++) {
+        float4 v = part[(size_t)s * n + i];
+        gx += v.x; gy += v.y; gz += v.z;
+    }
+    g[(size_t)i * 3] = gx; g[(size_t)i * 3 + 1] = gy; g[(size_t)i * 3 + 2] = gz;
+}
+
+// column splits: enough workgroups to fill the GPU (about 2048), at least
+// 256 columns each
+int sk_splits(int n, int m) {
+    int gx = (n + PC_WG - 1) / PC_WG;
+    int s = (2048 + gx - 1) / gx;
+    s = std::min(s, (m + 255) / 256);
+    return std::max(1, std::min(s, 65535));
+}
+
 }  // namespace
 
 extern "C" {
+
+long long pc_sinkhorn_work_bytes(int n, int m) {
+    if (n < 0 || m < 0) return -1;
+    return (long long)sk_splits(n, m) * std::max(n, 1) * (long long)sizeof(float4);
+}
+
+int pc_softmin(const float* x, int n, const float* y, int m, const float* h2, float eps, int p, void* work, float* out, void* stream) {
+    if (n < 0 || m <= 0) return pc_fail("pc_softmin: need n >= 0 rows and m > 0 columns");
+    if (!(eps > 0.f)) return pc_fail("pc_softmin: eps must be positive");
+    if (p != 1 && p != 2) return pc_fail("pc_softmin: p must be 1 or 2");
+    if (n == 0) return 0;
+    if (!x || !y || !h2 || !work || !out) return pc_fail("pc_softmin: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    int splits = sk_splits(n, m);
+    int per = (m + splits - 1) / splits;
+    int gx = (n + PC_WG - 1) / PC_WG;
+    float k2 = 1.4426950408889634f / eps;
+    hipLaunchKernelGGL(k_sk_softmin, dim3(gx, splits), dim3(PC_WG), 0, st, x, n, y, m, h2, k2, p, per, (float2*)work);
+    hipLaunchKernelGGL(k_sk_softmin_combine, dim3(gx), dim3(PC_WG), 0, st, (const float2*)work, n, splits, eps, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return pc_fail(std::string("pc_softmin: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int pc_plan_grad(const float* x, int n, const float* y, int m, const float* h2, const float* r2, float eps, int p, void* work,
+                 float* g, void* stream) {
+    if (n < 0 || m <= 0) return pc_fail("pc_plan_grad: need n >= 0 rows and m > 0 columns");
+    if (!(eps > 0.f)) return pc_fail("pc_plan_grad: eps must be positive");
+    if (p != 1 && p != 2) return pc_fail("pc_plan_grad: p must be 1 or 2");
+    if (n == 0) return 0;
+    if (!x || !y || !h2 || !r2 || !work || !g) return pc_fail("pc_plan_grad: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    int splits = sk_splits(n, m);
+    int per = (m + splits - 1) / splits;
+    int gx = (n + PC_WG - 1) / PC_WG;
+    float k2 = 1.4426950408889634f / eps;
+    hipLaunchKernelGGL(k_sk_plan_grad, dim3(gx, splits), dim3(PC_WG), 0, st, x, n, y, m, h2, r2, k2, p, per, (float4*)work);
+    hipLaunchKernelGGL(k_sk_grad_combine, dim3(gx), dim3(PC_WG), 0, st, (const float4*)work, n, splits, g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return pc_fail(std::string("pc_plan_grad: ") + hipGetErrorString(e));
+    return 0;
+}
+
 
 const char* pc_last_error(void) { return g_pc_err.c_str(); }
 

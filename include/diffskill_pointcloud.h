@@ -8,6 +8,14 @@
  * built by create_chamfer_loss (plb/engine/taichi_env.py:28-32, a
  * third-party dependency absent here).  The loss values and their adjoints
  * are assembled from its outputs by the host layer.
+ *
+ * pc_softmin / pc_plan_grad are the pairwise kernels of the Sinkhorn
+ * divergence behind the reference's EMD loss, create_emd_loss's
+ * geomloss.SamplesLoss(loss='sinkhorn', p=1, blur=0.001)
+ * (plb/engine/taichi_env.py:23-26, evaluated at :269-270 and :303; geomloss
+ * is a third-party dependency absent here): the soft-min of its dual
+ * updates and the transport-plan gradient of its last step.  The annealing
+ * loop runs on the host.
  */
 #ifndef DISKILL_POINTCLOUD_H
 #define DISKILL_POINTCLOUD_H
@@ -26,6 +34,29 @@ extern "C" {
  * or -1 with pc_last_error() describing the problem. */
 int pc_nearest(const float *q, int nq, const float *t, int nt, unsigned long long *work, int *idx, float *d2, void *stream);
 const char *pc_last_error(void);
+
+/* Bytes of device scratch that pc_softmin and pc_plan_grad need for n rows
+ * and m columns (-1 for negative counts). */
+long long pc_sinkhorn_work_bytes(int n, int m);
+
+/* out[i] = -eps log sum_j exp(h[j] - C(x_i, y_j) / eps), the weights h
+ * given in log2 units (h2 = h * log2(e)), for the n rows x (n x 3
+ * floats, device) against the m columns y (m x 3), C = |x - y| (p = 1) or
+ * |x - y|^2 / 2 (p = 2); every pair is evaluated (n * m work), the columns
+ * split over workgroups whose partial (max, sum) are combined in a fixed
+ * order.  work: pc_sinkhorn_work_bytes(n, m) bytes of device scratch.
+ * Asynchronous on stream, no allocation.  Returns 0, or -1 with
+ * pc_last_error() (m must be positive, eps positive, p 1 or 2). */
+int pc_softmin(const float *x, int n, const float *y, int m, const float *h2, float eps, int p, void *work, float *out,
+               void *stream);
+
+/* g[i] (n x 3, overwritten) = sum_j exp(h[j] + r[i] - C(x_i, y_j) / eps)
+ * dC(x_i, y_j)/dx_i with h = h2 / log2(e), r = r2 / log2(e) (log2 units, as
+ * pc_softmin), dC/dx = (x - y) / |x - y| for p = 1 (0 where x = y) and
+ * x - y for p = 2: the transport plan's pull on every row point.  Same work
+ * buffer, stream and error rules as pc_softmin. */
+int pc_plan_grad(const float *x, int n, const float *y, int m, const float *h2, const float *r2, float eps, int p, void *work,
+                 float *g, void *stream);
 
 #ifdef __cplusplus
 }
