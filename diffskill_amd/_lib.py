@@ -51,6 +51,7 @@ EXPORTS = [
     "mpm_set_launch", "mpm_get_launch", "mpm_min_dist_cols", "mpm_min_dist", "mpm_min_dist_bwd",
     "mpm_collision_bytes", "mpm_set_collisions", "mpm_get_collisions",
     "pc_nearest", "pc_last_error", "pc_sinkhorn_work_bytes", "pc_softmin", "pc_plan_grad",
+    "pc_sinkhorn_step_work_bytes", "pc_sinkhorn_step",
 ]
 
 PHASES = ["bin", "compact", "blocks", "scatter", "stress", "p2g", "grid_op", "g2p", "p2g_grid", "grid_op_keep",
@@ -109,6 +110,8 @@ def load():
         "pc_sinkhorn_work_bytes": (ctypes.c_longlong, [i32, i32]),
         "pc_softmin": (i32, [vp, i32, vp, i32, vp, ctypes.c_float, i32, vp, vp, vp]),
         "pc_plan_grad": (i32, [vp, i32, vp, i32, vp, vp, ctypes.c_float, i32, vp, vp, vp]),
+        "pc_sinkhorn_step_work_bytes": (ctypes.c_longlong, [i32, i32]),
+        "pc_sinkhorn_step": (i32, [vp, i32, vp, i32, vp, ctypes.c_float, i32, i32, vp, vp]),
         "pc_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():

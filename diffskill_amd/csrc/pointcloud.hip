@@ -207,6 +207,102 @@ __global__ void __launch_bounds__(PC_WG) k_sk_grad_combine(const float4* __restr
     g[(size_t)i * 3] = gx; g[(size_t)i * 3 + 1] = gy; g[(size_t)i * 3 + 2] = gz;
 }
 
+// One annealing step of the divergence, all four soft-mins in one launch
+// (blockIdx.z = problem) and their combination in a second: the potentials
+// live in one device array pot = [a_x, b_x, a_y, b_y, pa_x, pb_x, pa_y, pb_y]
+// (lengths n, n, m, m, n, n, m, m), and the column weights h are formed while
+// the columns are staged (uniform log weights plus potential / eps):
+//   problem 0: a_y <- softmin over x of (log 1/n + b_x / eps)   rows y, cols x
+//   problem 1: b_x <- softmin over y of (log 1/m + a_y / eps)   rows x, cols y
+//   problem 2: a_x <- softmin over x of (log 1/n + a_x / eps)   rows x, cols x
+//   problem 3: b_y <- softmin over y of (log 1/m + b_y / eps)   rows y, cols y
+// mode 0 (first eps): h = log weights only, the result replaces the potential;
+// mode 1: symmetric update, potential <- (potential + result) / 2;
+// mode 2 (last extrapolation): the potentials move to pa_x.. and the results
+// replace them.
+struct SkProb { const float* rows; int nr; const float* cols; int nc; const float* hsrc; float lw; float* dst; float* prev; };
+DEV_PC SkProb sk_problem(int z, const float* x, int n, const float* y, int m, float* pot) {
+    float* a_x = pot; float* b_x = pot + n; float* a_y = pot + 2 * n; float* b_y = pot + 2 * n + m;
+    float* pa_x = pot + 2 * n + 2 * m; float* pb_x = pa_x + n; float* pa_y = pa_x + 2 * n; float* pb_y = pa_y + m;
+    SkProb q;
+    if (z == 0) { q.rows = y; q.nr = m; q.cols = x; q.nc = n; q.hsrc = b_x; q.dst = a_y; q.prev = pa_y; }
+    else if (z == 1) { q.rows = x; q.nr = n; q.cols = y; q.nc = m; q.hsrc = a_y; q.dst = b_x; q.prev = pb_x; }
+    else if (z == 2) { q.rows = x; q.nr = n; q.cols = x; q.nc = n; q.hsrc = a_x; q.dst = a_x; q.prev = pa_x; }
+    else { q.rows = y; q.nr = m; q.cols = y; q.nc = m; q.hsrc = b_y; q.dst = b_y; q.prev = pb_y; }
+    q.lw = -logf((float)q.nc);
+    return q;
+}
+
+__global__ void __launch_bounds__(PC_WG) k_sk_step(const float* __restrict__ x, int n, const float* __restrict__ y, int m,
+                                                  float* __restrict__ pot, float eps, int p, int mode, int splits_xy, int splits_xx,
+                                                  int splits_yy, float2* __restrict__ part, size_t part_stride) {
+    __shared__ float sx[SK_CHUNK], sy[SK_CHUNK], sz[SK_CHUNK], sh[SK_CHUNK];
+    int z = blockIdx.z;
+    SkProb q = sk_problem(z, x, n, y, m, pot);
+    int splits = z < 2 ? splits_xy : (z == 2 ? splits_xx : splits_yy);
+    if ((int)blockIdx.y >= splits || (int)blockIdx.x * PC_WG >= q.nr) return;
+    int per = (q.nc + splits - 1) / splits;
+    int i = blockIdx.x * PC_WG + threadIdx.x;
+    int c0 = blockIdx.y * per, c1 = min(q.nc, c0 + per);
+    float k2 = 1.4426950408889634f / eps, inv = 1.f / eps;
+    float xi = 0.f, yi = 0.f, zi = 0.f;
+    if (i < q.nr) { xi = q.rows[(size_t)i * 3]; yi = q.rows[(size_t)i * 3 + 1]; zi = q.rows[(size_t)i * 3 + 2]; }
+    float mx = -INFINITY, sm = 0.f;
+    for (int b0 = c0; b0 < c1; b0 += SK_CHUNK) {
+        int nc = min(SK_CHUNK, c1 - b0);
+        __syncthreads();
+        for (int k = threadIdx.x; k < nc; k += PC_WG) {
+            const float* c = q.cols + (size_t)(b0 + k) * 3;
+            sx[k] = c[0]; sy[k] = c[1]; sz[k] = c[2];
+            float h = q.lw + (mode == 0 ? 0.f : q.hsrc[b0 + k] * inv);
+            sh[k] = h * 1.4426950408889634f;
+        }
+        __syncthreads();
+        int k = 0;
+        for (; k + 4 <= nc; k += 4) {
+            float zz[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) zz[u] = sh[k + u] - k2 * sk_cost(xi - sx[k + u], yi - sy[k + u], zi - sz[k + u], p);
+            float bm = fmaxf(fmaxf(zz[0], zz[1]), fmaxf(zz[2], zz[3]));
+            float nm = fmaxf(mx, bm);
+            float acc = 0.f;
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc += __builtin_amdgcn_exp2f(zz[u] - nm);
+            sm = sm * __builtin_amdgcn_exp2f(mx - nm) + acc;
+            mx = nm;
+        }
+        for (; k < nc; k++) {
+            float zz = sh[k] - k2 * sk_cost(xi - sx[k], yi - sy[k], zi - sz[k], p);
+            float nm = fmaxf(mx, zz);
+            sm = sm * __builtin_amdgcn_exp2f(mx - nm) + __builtin_amdgcn_exp2f(zz - nm);
+            mx = nm;
+        }
+    }
+    if (i < q.nr) part[(size_t)z * part_stride + (size_t)blockIdx.y * q.nr + i] = make_float2(mx, sm);
+}
+
+__global__ void __launch_bounds__(PC_WG) k_sk_step_combine(const float* __restrict__ x, int n, const float* __restrict__ y, int m,
+                                                          float* __restrict__ pot, float eps, int mode, int splits_xy, int splits_xx,
+                                                          int splits_yy, const float2* __restrict__ part, size_t part_stride) {
+    int z = blockIdx.y;
+    SkProb q = sk_problem(z, x, n, y, m, pot);
+    int splits = z < 2 ? splits_xy : (z == 2 ? splits_xx : splits_yy);
+    int i = blockIdx.x * PC_WG + threadIdx.x;
+    if (i >= q.nr) return;
+    const float2* pp = part + (size_t)z * part_stride;
+    float M = -INFINITY;
+    for (int s = 0; s < splits; s++) M = fmaxf(M, pp[(size_t)s * q.nr + i].x);
+    float S = 0.f;
+    for (int s = 0; s < splits; s++) {
+        float2 v = pp[(size_t)s * q.nr + i];
+        S += v.y * __builtin_amdgcn_exp2f(v.x - M);
+    }
+    float r = -eps * 0.69314718055994531f * (M + __builtin_amdgcn_logf(S));
+    if (mode == 1) r = 0.5f * (q.dst[i] + r);
+    if (mode == 2) q.prev[i] = q.dst[i];
+    q.dst[i] = r;
+}
+
 // column splits: enough workgroups to fill the GPU (about 2048), at least
 // 256 columns each
 int sk_splits(int n, int m) {
@@ -223,6 +319,35 @@ extern "C" {
 long long pc_sinkhorn_work_bytes(int n, int m) {
     if (n < 0 || m < 0) return -1;
     return (long long)sk_splits(n, m) * std::max(n, 1) * (long long)sizeof(float4);
+}
+
+long long pc_sinkhorn_step_work_bytes(int n, int m) {
+    if (n <= 0 || m <= 0) return -1;
+    long long rows = std::max(n, m);
+    int s = std::max(std::max(sk_splits(n, m), sk_splits(m, n)), std::max(sk_splits(n, n), sk_splits(m, m)));
+    return 4LL * s * rows * (long long)sizeof(float2);
+}
+
+int pc_sinkhorn_step(const float* x, int n, const float* y, int m, float* pot, float eps, int p, int mode, void* work, void* stream) {
+    if (n <= 0 || m <= 0) return pc_fail("pc_sinkhorn_step: both clouds must be non-empty");
+    if (!(eps > 0.f)) return pc_fail("pc_sinkhorn_step: eps must be positive");
+    if (p != 1 && p != 2) return pc_fail("pc_sinkhorn_step: p must be 1 or 2");
+    if (mode < 0 || mode > 2) return pc_fail("pc_sinkhorn_step: mode must be 0, 1 or 2");
+    if (!x || !y || !pot || !work) return pc_fail("pc_sinkhorn_step: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    // splits per problem (rows, cols): y-x and x-y share max(splits(m, n), splits(n, m)) slots
+    int sxy = std::max(sk_splits(m, n), sk_splits(n, m)), sxx = sk_splits(n, n), syy = sk_splits(m, m);
+    int rows = std::max(n, m);
+    size_t stride = (size_t)std::max(std::max(sxy, sxx), syy) * rows;
+    int gx = (rows + PC_WG - 1) / PC_WG;
+    int gy = std::max(std::max(sxy, sxx), syy);
+    hipLaunchKernelGGL(k_sk_step, dim3(gx, gy, 4), dim3(PC_WG), 0, st, x, n, y, m, pot, eps, p, mode, sxy, sxx, syy, (float2*)work,
+                       stride);
+    hipLaunchKernelGGL(k_sk_step_combine, dim3(gx, 4), dim3(PC_WG), 0, st, x, n, y, m, pot, eps, mode, sxy, sxx, syy,
+                       (const float2*)work, stride);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return pc_fail(std::string("pc_sinkhorn_step: ") + hipGetErrorString(e));
+    return 0;
 }
 
 int pc_softmin(const float* x, int n, const float* y, int m, const float* h2, float eps, int p, void* work, float* out, void* stream) {

@@ -179,17 +179,27 @@ class _SinkhornDivergence(torch.autograd.Function):
         la = torch.full((N,), -float(np.log(N)), device=xd.device)
         lb = torch.full((M,), -float(np.log(M)), device=xd.device)
         eps_s = epsilon_schedule(xd, yd, p, blur, scaling)
-        eps = eps_s[0]
-        a_y, b_x = sk.softmin(yd, xd, la, eps), sk.softmin(xd, yd, lb, eps)
-        a_x, b_y = sk.softmin(xd, xd, la, eps), sk.softmin(yd, yd, lb, eps)
+        # the whole loop on the device: one fused step (four soft-mins, two
+        # launches) per eps, the potentials in one array
+        L = sk.L
+        pot = torch.empty(4 * (N + M), dtype=torch.float32, device=xd.device)
+        nb = L.pc_sinkhorn_step_work_bytes(N, M)
+        work = torch.empty(max(1, (nb + 15) // 16) * 4, dtype=torch.float32, device=xd.device)
+
+        def step(eps, mode):
+            rc = L.pc_sinkhorn_step(ctypes.c_void_p(xd.data_ptr()), N, ctypes.c_void_p(yd.data_ptr()), M,
+                                    ctypes.c_void_p(pot.data_ptr()), float(eps), int(p), mode, ctypes.c_void_p(work.data_ptr()),
+                                    _stream())
+            if rc != 0:
+                raise RuntimeError("pc_sinkhorn_step failed: %s" % L.pc_last_error().decode(errors="replace"))
+
+        step(eps_s[0], 0)
         for eps in eps_s:
-            at_y, bt_x = sk.softmin(yd, xd, la + b_x / eps, eps), sk.softmin(xd, yd, lb + a_y / eps, eps)
-            at_x, bt_y = sk.softmin(xd, xd, la + a_x / eps, eps), sk.softmin(yd, yd, lb + b_y / eps, eps)
-            a_y, b_x = 0.5 * (a_y + at_y), 0.5 * (b_x + bt_x)
-            a_x, b_y = 0.5 * (a_x + at_x), 0.5 * (b_y + bt_y)
-        prev = (a_x, b_x, a_y, b_y)
-        a_y, b_x = sk.softmin(yd, xd, la + prev[1] / eps, eps), sk.softmin(xd, yd, lb + prev[2] / eps, eps)
-        a_x, b_y = sk.softmin(xd, xd, la + prev[0] / eps, eps), sk.softmin(yd, yd, lb + prev[3] / eps, eps)
+            step(eps, 1)
+        eps = eps_s[-1]
+        step(eps, 2)
+        a_x, b_x, a_y, b_y, pa_x, pb_x, pa_y, pb_y = torch.split(pot, [N, N, M, M, N, N, M, M])
+        prev = (pa_x, pb_x, pa_y, pb_y)
         ctx.sk, ctx.eps, ctx.la, ctx.lb = sk, eps, la, lb
         ctx.pots = (a_x, b_x, a_y, b_y) + prev
         return ((b_x - a_x).mean() + (a_y - b_y).mean()).to(x.dtype)

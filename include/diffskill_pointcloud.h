@@ -58,6 +58,23 @@ int pc_softmin(const float *x, int n, const float *y, int m, const float *h2, fl
 int pc_plan_grad(const float *x, int n, const float *y, int m, const float *h2, const float *r2, float eps, int p, void *work,
                  float *g, void *stream);
 
+/* Bytes of device scratch pc_sinkhorn_step needs (-1 unless n, m > 0). */
+long long pc_sinkhorn_step_work_bytes(int n, int m);
+
+/* One annealing step of the debiased Sinkhorn loop (uniform weights) on the
+ * potentials pot = [a_x, b_x, a_y, b_y, pa_x, pb_x, pa_y, pb_y] (device
+ * floats, lengths n, n, m, m, n, n, m, m): the four soft-mins
+ *   a_y <- softmin_x(log(1/n) + b_x / eps), b_x <- softmin_y(log(1/m) + a_y / eps),
+ *   a_x <- softmin_x(log(1/n) + a_x / eps), b_y <- softmin_y(log(1/m) + b_y / eps)
+ * all computed from the previous values; mode 0 takes the log weights alone
+ * (the first eps) and stores the results, mode 1 stores (previous + result) / 2
+ * (the symmetric update of every eps), mode 2 moves the previous values to
+ * pa_x.. and stores the results (the last extrapolation).  Two launches on
+ * stream, every pair evaluated, fixed combination order, no allocation.
+ * Returns 0, or -1 with pc_last_error(). */
+int pc_sinkhorn_step(const float *x, int n, const float *y, int m, float *pot, float eps, int p, int mode, void *work,
+                     void *stream);
+
 #ifdef __cplusplus
 }
 #endif
