@@ -772,7 +772,8 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __res
     for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
         int4 it = ta.items[e];
         // the first batch's positions do not depend on the tile: their loads
-        // are issued first, so both latencies overlap
+        // are issued first, so both latencies overlap; every later batch's
+        // are issued one batch ahead
         int k = threadIdx.x;
         int p = ta.order[it.y + min(k, it.z - 1)];
         v3 x = load3(st, stride, 0, p);
@@ -782,9 +783,11 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __res
         load_tile(tv, S, bi, bj, bk, g_v);
         __syncthreads();
         for (; k < it.z; k += WG) {
-            if (k >= WG) {
-                p = ta.order[it.y + k];
-                x = load3(st, stride, 0, p);
+            int pn = p;
+            v3 xn = x;
+            if (k + WG < it.z) {
+                pn = ta.order[it.y + k + WG];
+                xn = load3(st, stride, 0, pn);
             }
             Stencil sw;
             stencil(S, x, sw);
@@ -832,9 +835,11 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __res
             }
             store3(st_out, stride, 3, p, nv);
             store9(st_out, stride, 6, p, nC);
-            v3 xn = mk(tmax(tmin(x.x + S.dt * nv.x, S.upper), S.lower), tmax(tmin(x.y + S.dt * nv.y, S.upper), S.lower),
+            v3 xo = mk(tmax(tmin(x.x + S.dt * nv.x, S.upper), S.lower), tmax(tmin(x.y + S.dt * nv.y, S.upper), S.lower),
                        tmax(tmin(x.z + S.dt * nv.z, S.upper), S.lower));
-            store3(st_out, stride, 0, p, xn);
+            store3(st_out, stride, 0, p, xo);
+            p = pn;
+            x = xn;
         }
         __syncthreads();
     }
