@@ -1187,11 +1187,15 @@ __global__ void __launch_bounds__(WG) k_p2g_bwd_gather(TileArgs ta, const float*
         load_tile(tg, S, bi, bj, bk, g_adjpm);
         __syncthreads();
         for (; k < it.z; k += WG) {
-            if (k >= WG) {
-                p = ta.order[it.y + k];
-                x = load3(st, stride, 0, p);
-                v = load3(st, stride, 3, p);
-                Am = load9(aff, stride, 0, p);
+            // the next batch's particle data, one batch ahead
+            int pn = p;
+            v3 xn = x, vn = v;
+            m3 An = Am;
+            if (k + WG < it.z) {
+                pn = ta.order[it.y + k + WG];
+                xn = load3(st, stride, 0, pn);
+                vn = load3(st, stride, 3, pn);
+                An = load9(aff, stride, 0, pn);
             }
             const float* A = Am.a;
             Stencil sw;
@@ -1270,6 +1274,10 @@ __global__ void __launch_bounds__(WG) k_p2g_bwd_gather(TileArgs ta, const float*
             store3(a0, stride, 0, p, gx0);
             store3(a0, stride, 3, p, gv);
             store9(gaff_out, stride, 0, p, gaff);
+            p = pn;
+            x = xn;
+            v = vn;
+            Am = An;
         }
         __syncthreads();
     }
