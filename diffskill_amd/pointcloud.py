@@ -162,9 +162,11 @@ class _Sinkhorn:
 
 def epsilon_schedule(x, y, p, blur, scaling):
     """geomloss's annealing: the bounding-box diagonal d of both clouds, then
-    exp(arange(p log d, p log blur, p log scaling)), then blur^p."""
+    exp(arange(p log d, p log blur, p log scaling)), then blur^p.  A diagonal
+    below blur (all points within blur of each other) is taken as blur, so the
+    schedule is just blur^p twice instead of starting at eps = 0."""
     pts = torch.cat([x, y])
-    d = float((pts.max(0).values - pts.min(0).values).norm())
+    d = max(float((pts.max(0).values - pts.min(0).values).norm()), blur)
     return [d ** p] + [float(np.exp(e)) for e in np.arange(p * np.log(d), p * np.log(blur), p * np.log(scaling))] + [blur ** p]
 
 

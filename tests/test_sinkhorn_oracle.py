@@ -75,3 +75,15 @@ def test_gradient_matches_finite_differences():
         xm[m, c] -= h
         fd = (sk.sinkhorn_divergence(xp, y, **kw) - sk.sinkhorn_divergence(xm, y, **kw)) / (2 * h)
         assert abs(fd - g[m, c]) <= 1e-5 * np.abs(g).max(), (m, c, fd, g[m, c])
+
+
+def test_device_schedule_matches_oracle_and_degenerate_clouds():
+    """The host schedule of the HIP path (diffskill_amd.pointcloud) equals the
+    oracle's; clouds within blur of one point start at blur^p, not at 0."""
+    import torch
+    from diffskill_amd.pointcloud import epsilon_schedule
+    x, y = cloud(30, 8), cloud(20, 9)
+    np.testing.assert_allclose(epsilon_schedule(torch.tensor(x), torch.tensor(y), 1, 0.001, 0.5),
+                               sk.epsilon_schedule(x, y, 1, 0.001, 0.5), rtol=1e-12)
+    one = torch.full((5, 3), 0.5, dtype=torch.float64)
+    assert epsilon_schedule(one, one, 1, 0.001, 0.5) == [0.001, 0.001]
