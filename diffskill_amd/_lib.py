@@ -46,7 +46,7 @@ EXPORTS = [
     "mpm_workspace_bytes", "mpm_create", "mpm_destroy", "mpm_frame", "mpm_stride", "mpm_particle_offset",
     "mpm_adjoint", "mpm_prim_buffers", "mpm_set_frame", "mpm_get_frame", "mpm_reset_frame0",
     "mpm_set_prim_frame", "mpm_get_prim_frame", "mpm_set_actions", "mpm_substep_fwd", "mpm_substep_bwd",
-    "mpm_grid_mass", "mpm_grid_mass_bwd", "mpm_add_obs_grad", "mpm_action_grad", "mpm_clear_grads",
+    "mpm_grid_mass", "mpm_grid_mass_bwd", "mpm_density_loss_bwd", "mpm_add_obs_grad", "mpm_action_grad", "mpm_clear_grads",
     "mpm_update_prims", "mpm_copy_frame", "mpm_set_profiling", "mpm_profile_read", "mpm_last_error",
     "mpm_set_launch", "mpm_get_launch", "mpm_min_dist_cols", "mpm_min_dist", "mpm_min_dist_bwd",
     "mpm_collision_bytes", "mpm_set_collisions", "mpm_get_collisions",
@@ -90,6 +90,7 @@ def load():
         "mpm_substep_bwd": (i32, [vp, i32, i32, vp]),
         "mpm_grid_mass": (i32, [vp, i32, vp, vp]),
         "mpm_grid_mass_bwd": (i32, [vp, i32, vp, vp]),
+        "mpm_density_loss_bwd": (i32, [vp, i32, vp, vp, ctypes.c_float, vp]),
         "mpm_add_obs_grad": (i32, [vp, i32, vp, vp, vp]),
         "mpm_action_grad": (i32, [vp, i32, i32, vp, vp]),
         "mpm_clear_grads": (i32, [vp, i32, vp]),

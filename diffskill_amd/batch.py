@@ -134,7 +134,6 @@ class BatchedTrajOpt:
             st["exp_avg_sq"] = torch.zeros_like(a)
         n = batch.consts[0]["n_grid"]
         self.mass = torch.zeros((batch.n_scenes, n, n, n), dtype=torch.float32, device="cuda")
-        self.gmass = torch.zeros_like(self.mass)
         self.table = torch.zeros((self.H, P, 7), dtype=torch.float32, device="cuda")
         self.grads = torch.zeros((self.H, P, 7), dtype=torch.float32, device="cuda")
         self.use_graph = use_graph
@@ -150,10 +149,8 @@ class BatchedTrajOpt:
         e.substep_fwd(0, self.T)
         self.mass.zero_()
         e.grid_mass(self.T, self.mass)
-        torch.sub(self.mass, self.targets, out=self.gmass)
-        self.gmass.sign_()
         e.clear_grads(self.T)
-        e.grid_mass_bwd(self.T, self.gmass)
+        e.density_loss_bwd(self.T, self.mass, self.targets)
         e.substep_bwd(0, self.T)
         for s in range(self.H):
             e.action_grad_into(s, self.S, self.grads[s])

@@ -1416,8 +1416,12 @@ __global__ void __launch_bounds__(PPB) k_grid_mass(ChunkArgs ca, const float* __
         seg++;
     }
 }
+// adjoint of compute_grid_m_kernel: gm is the adjoint of the mass grid, or,
+// with tgt given, the mass grid itself, whose adjoint under the weighted L1
+// density loss (loss.py:173-175, :213) wt * sign(m - tgt) is formed per node
+// here instead of in a grid of its own.
 __global__ void __launch_bounds__(PPB) k_grid_mass_bwd(ChunkArgs ca, const float* __restrict__ st, const float* __restrict__ gm,
-                                                       float* __restrict__ adj) {
+                                                       const float* __restrict__ tgt, float wt, float* __restrict__ adj) {
     int4 ch = ca.chunks[blockIdx.x];
     int i = threadIdx.x;
     if (i >= ch.z) return;
@@ -1427,6 +1431,7 @@ __global__ void __launch_bounds__(PPB) k_grid_mass_bwd(ChunkArgs ca, const float
     Stencil sw;
     stencil(S, x, sw);
     const float* g = gm + (size_t)S.m_off;
+    const float* t = tgt ? tgt + (size_t)S.m_off : nullptr;
     int n = S.n_grid;
     float gfx[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -1437,7 +1442,13 @@ __global__ void __launch_bounds__(PPB) k_grid_mass_bwd(ChunkArgs ca, const float
             for (int c = 0; c < 3; c++) {
                 int I0 = sw.base[0] + a, I1 = sw.base[1] + b, I2 = sw.base[2] + c;
                 if (!in_grid(S, I0, I1, I2)) continue;
-                float gg = g[((size_t)I0 * n + I1) * n + I2] * S.p_mass;
+                size_t id = ((size_t)I0 * n + I1) * n + I2;
+                float gn = g[id];
+                if (t) {
+                    float d = gn - t[id];
+                    gn = d > 0.f ? wt : (d < 0.f ? -wt : 0.f);
+                }
+                float gg = gn * S.p_mass;
                 int off[3] = {a, b, c};
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
@@ -2488,7 +2499,20 @@ int mpm_grid_mass_bwd(mpm_ctx* ctx, int f, const float* g, void* stream) {
     if (f != ctx->adj_top) return fail("mpm_grid_mass_bwd: frame is not the current adjoint frame");
     hipStream_t st = S_(stream);
     if (ensure_frame(ctx, f, st)) return -1;
-    hipLaunchKernelGGL(k_grid_mass_bwd, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), g, ctx->adj_ptr(f));
+    hipLaunchKernelGGL(k_grid_mass_bwd, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), g, (const float*)nullptr,
+                       0.f, ctx->adj_ptr(f));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mpm_density_loss_bwd(mpm_ctx* ctx, int f, const float* mass, const float* target, float weight, void* stream) {
+    if (!ctx || !mass || !target) return fail("mpm_density_loss_bwd: null argument");
+    if (ctx->adj_top < 0) ctx->adj_top = f;
+    if (f != ctx->adj_top) return fail("mpm_density_loss_bwd: frame is not the current adjoint frame");
+    hipStream_t st = S_(stream);
+    if (ensure_frame(ctx, f, st)) return -1;
+    hipLaunchKernelGGL(k_grid_mass_bwd, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), mass, target, weight,
+                       ctx->adj_ptr(f));
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -141,13 +141,35 @@ class Engine:
     def substep_bwd(self, s, count=1):
         _lib.check(self.L.mpm_substep_bwd(self.ctx, int(s), int(count), _stream()), "mpm_substep_bwd")
 
+    def _mass_grid(self, t, name, out=False):
+        """t as a contiguous fp32 device grid of every scene's n^3 nodes
+        (scene after scene), checked before the kernel indexes it."""
+        n = sum(g ** 3 for g in self.n_grids)
+        if t.numel() != n:
+            raise ValueError("%s: %d values, the scenes' mass grids hold %d" % (name, t.numel(), n))
+        if out:
+            if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+                raise ValueError("%s: needs a contiguous float32 device tensor" % name)
+            return t
+        return t.to(device="cuda", dtype=torch.float32).contiguous()
+
     def grid_mass(self, f, out):
-        assert out.is_cuda and out.dtype == torch.float32 and out.is_contiguous()
+        out = self._mass_grid(out, "grid_mass", out=True)
         _lib.check(self.L.mpm_grid_mass(self.ctx, int(f), ctypes.c_void_p(out.data_ptr()), _stream()), "mpm_grid_mass")
 
     def grid_mass_bwd(self, f, g):
-        g = g.contiguous().float()
+        g = self._mass_grid(g, "grid_mass_bwd")
         _lib.check(self.L.mpm_grid_mass_bwd(self.ctx, int(f), ctypes.c_void_p(g.data_ptr()), _stream()), "mpm_grid_mass_bwd")
+
+    def density_loss_bwd(self, f, mass, target, weight=1.0):
+        """Adjoint of weight * sum |mass - target| (the reference's density
+        loss, losses/loss.py:173-175, :213) into frame f's particles, the
+        node adjoint weight * sign(mass - target) formed in the grid-mass
+        adjoint kernel."""
+        mass = self._mass_grid(mass, "density_loss_bwd mass")
+        target = self._mass_grid(target, "density_loss_bwd target")
+        _lib.check(self.L.mpm_density_loss_bwd(self.ctx, int(f), ctypes.c_void_p(mass.data_ptr()), ctypes.c_void_p(target.data_ptr()),
+                                               float(weight), _stream()), "mpm_density_loss_bwd")
 
     def add_obs_grad(self, f, xv_slots=None, prim_grad=None):
         a = ctypes.c_void_p(xv_slots.data_ptr()) if xv_slots is not None else None

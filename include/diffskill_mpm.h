@@ -22,6 +22,9 @@
  *   mpm_substep_bwd                    MPMSimulator.substep_grad (mpm_simulator.py:325-345)
  *   mpm_grid_mass / mpm_grid_mass_bwd  compute_grid_m_kernel and its .grad
  *                                      (mpm_simulator.py:456-466, function.py:147-152)
+ *   mpm_density_loss_bwd               compute_density_loss_kernel.grad followed by
+ *                                      compute_grid_m_kernel.grad (losses/loss.py:173-175,
+ *                                      :213, :290), one launch
  *   mpm_add_obs_grad                   GradModel._set_obs_grad (function.py:130-142)
  *   mpm_action_grad                    set_velocity.grad + get_action_grad
  *                                      (function.py:180-190, primive_base.py:276-282)
@@ -144,6 +147,12 @@ int mpm_substep_bwd(mpm_ctx *ctx, int s, int count, void *stream);
 int mpm_grid_mass(mpm_ctx *ctx, int f, float *out, void *stream);
 /* Adds d(sum g*m)/dx into the adjoint of frame f. g: [n_scenes][n^3]. */
 int mpm_grid_mass_bwd(mpm_ctx *ctx, int f, const float *g, void *stream);
+/* Adjoint of the weighted L1 density loss weight * sum |m - target| at frame
+ * f: adds d/dx into the adjoint of frame f, with the node adjoint
+ * weight * sign(m - target) (0 where they are equal) formed inside the
+ * grid-mass adjoint.  mass: the mass grid of frame f (mpm_grid_mass on a
+ * zeroed grid), target: the target densities, both [n_scenes][n^3]. */
+int mpm_density_loss_bwd(mpm_ctx *ctx, int f, const float *mass, const float *target, float weight, void *stream);
 /* Adds observation adjoints at frame f: xv_grad [stride][6] (x, v of every
  * particle slot; may be NULL), prim_grad [n_prims_total][8] (may be NULL). */
 int mpm_add_obs_grad(mpm_ctx *ctx, int f, const float *xv_grad, const float *prim_grad, void *stream);

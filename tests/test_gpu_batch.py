@@ -67,6 +67,44 @@ def test_batched_grads_match_oracle_per_scene():
         o0 += npk
 
 
+def test_fused_density_loss_adjoint_matches_grid_adjoint():
+    """mpm_density_loss_bwd (node adjoint weight * sign(m - target) formed in
+    the grid-mass adjoint) against mpm_grid_mass_bwd fed that sign grid:
+    bit-exact, including nodes where mass equals target (sign 0) and a
+    weight other than 1; a grid of the wrong size is refused on the host."""
+    from diffskill_amd import workloads
+    from diffskill_amd.batch import SceneBatch
+    w = small_workload()
+    H, S = w["H"], w["substeps"]
+    T = H * S
+    b = SceneBatch(w["scenes"], T + 1)
+    e = b.engine
+    b.restart()
+    table = torch.as_tensor(workloads.action_tensor(w["actions"], H)).cuda().clamp(-1, 1)
+    for s in range(H):
+        e.set_actions(s, S, table[s])
+    e.substep_fwd(0, T)
+    n = b.consts[0]["n_grid"]
+    mass = torch.zeros((b.n_scenes, n, n, n), dtype=torch.float32, device="cuda")
+    e.grid_mass(T, mass)
+    target = targets_for(w)
+    target[0, :8] = mass[0, :8]          # equal nodes, inside and outside the clouds
+    for weight in (1.0, 0.37):
+        print("case fused density adjoint weight %g" % weight, flush=True)
+        g = torch.sign(mass - target) * weight
+        e.clear_grads(T)
+        e.grid_mass_bwd(T, g)
+        ref = e.adjoint(T).clone()
+        e.clear_grads(T)
+        e.density_loss_bwd(T, mass, target, weight)
+        got = e.adjoint(T).clone()
+        torch.cuda.synchronize()
+        assert torch.count_nonzero(ref) > 0
+        assert torch.equal(got, ref)
+    with pytest.raises(ValueError):
+        e.density_loss_bwd(T, mass[:1], target)
+
+
 @pytest.mark.parametrize("ckpt_k,keep_blocks", [(0, 0), (0, 1), (3, -1), (3, 0)])
 def test_kept_and_rebuilt_adjoint_grids_agree(ckpt_k, keep_blocks):
     """keep_blocks=-1 (default: the adjoint reads the forward's binning and
