@@ -222,3 +222,61 @@ def test_field_views_match_state_and_adjoint_accessors():
     np.testing.assert_allclose(g.get_state(0)[:3], [0.31, 0.32, 0.33], atol=1e-7)
     sim.x.grad.fill(0)
     assert np.abs(g.position.grad.to_numpy()).max() == 0
+
+
+@pytest.mark.parametrize("corner", ["low", "high"])
+def test_walls_and_clamp_parity(corner):
+    """Particles driven into the domain walls (grid_op's bound-3 wall
+    projection and ground Coulomb friction, mpm_simulator.py:241-260; g2p's
+    position clamp, :283): 20 forward substeps against the fp64 oracle, and
+    one adjoint substep taken at the walls."""
+    sim_cfg, prims, _ = capsule_scene(n=1500, seed=12)
+    if corner == "low":
+        st = particles_box(1500, [0.08, 0.08, 0.08], [0.14, 0.14, 0.14], seed=12, v_sigma=0.5, F_jitter=0.03)
+        st[:, 3:6] += -8.0
+    else:
+        st = particles_box(1500, [0.85, 0.85, 0.85], [0.9, 0.9, 0.9], seed=12, v_sigma=0.5, F_jitter=0.03)
+        st[:, 3:6] += 8.0
+    osim, c = oracle_sim(sim_cfg)
+    o = Oracle(osim, [oracle_prim(p) for p in prims])
+    dx, S, H = c["dx"], c["substeps"], 4
+    acts = np.zeros((H, 3))
+    print("case walls %s forward" % corner, flush=True)
+    sim, P = build(sim_cfg, prims, st)
+    for s in range(H):
+        P.set_action(s, S, acts[s])
+        for i in range(s * S, (s + 1) * S):
+            sim.substep(i)
+    frames, _ = OracleRollout(o, S).run(st, prim_state0(prims[0])[None], acts)
+    T = H * S
+    got, ref = flat(sim.get_state(T)), frames[T]
+    # the walls were reached: some particle's stencil touches a wall node
+    lo_n = np.floor(ref[:, 0:3] / dx - 0.5)
+    assert (lo_n.min() < 3) if corner == "low" else (lo_n.max() + 2 > c["n_grid"] - 3)
+    if corner == "high":   # and positions were clamped at 1 - 3 dx
+        assert any(np.any(fr[:, 0:3] >= 1 - 3 * dx - 1e-12) for fr in frames)
+    assert np.abs(got[:, 0:3] - ref[:, 0:3]).max() < 1e-4 * dx
+    assert np.abs(got[:, 3:6] - ref[:, 3:6]).max() < 1e-3 * np.abs(ref[:, 3:6]).max()
+    assert np.abs(got[:, 15:24] - ref[:, 15:24]).max() < 1e-4
+
+    print("case walls %s adjoint" % corner, flush=True)
+    sw = frames[T]
+    sim, P = build(sim_cfg, prims, sw)
+    P.set_action(0, S, acts[0])
+    sim.substep(0)
+    adj1 = np.random.default_rng(13).normal(size=(1500, 24))
+    sim.clear_grads()
+    sim.add_obs_grad(1, torch.as_tensor(adj1[:, 0:6], dtype=torch.float32), None)
+    a = sim.engine.adjoint(1)
+    a[6:24, sim.engine.scene_cols(0)] = sim._to_slots(torch.as_tensor(adj1[:, 6:24], dtype=torch.float32, device="cuda")).t()
+    sim.substep_grad(0)
+    gx, gv, gF, gC = [t.detach().cpu().double().numpy() for t in sim.get_adjoint(0)]
+    ps0 = prim_state0(prims[0])[None]
+    pv = o.set_velocity(0, acts[0], S)[None]
+    _, ps1 = o.substep(sw, ps0, pv)
+    ref0, _, _, _ = o.substep_grad(sw, adj1, ps0, pv, ps1, np.zeros((1, 8)))
+    got = np.concatenate([gx, gv, gC.reshape(-1, 9), gF.reshape(-1, 9)], 1)
+    for name, sl in (("x", slice(0, 3)), ("v", slice(3, 6)), ("C", slice(6, 15)), ("F", slice(15, 24))):
+        err = np.linalg.norm(got[:, sl] - ref0[:, sl]) / np.linalg.norm(ref0[:, sl])
+        print(name, err, flush=True)
+        assert err < 1e-3, name
