@@ -138,6 +138,27 @@ def test_batched_matches_per_scene_contexts():
     assert rel(gp, g[:, :gp.shape[1]]) <= 1e-4
 
 
+def test_empty_scene_in_a_batch():
+    """A scene without particles packed between two others: the others'
+    losses and action gradients are those of the batch without it (to the
+    float-atomics order noise), its own loss is its target's total mass and
+    its action gradients are 0."""
+    w = small_workload()
+    targets = targets_for(w)
+    _, g, l = run_pass(w, targets)
+    empty = dict(w["scenes"][1], x0=np.zeros((0, 3)))
+    w2 = dict(w, scenes=w["scenes"][:1] + [empty] + w["scenes"][1:], actions=w["actions"][:1] + [w["actions"][1]] + w["actions"][1:])
+    t2 = torch.cat([targets[:1], targets[1:2], targets[1:]])
+    print("case empty scene in a batch of %d" % len(w2["scenes"]), flush=True)
+    _, g2, l2 = run_pass(w2, t2)
+    p = [a.shape[1] for a in w["actions"]]
+    np.testing.assert_allclose(l2[1], float(targets[1].double().sum()), rtol=1e-6)
+    np.testing.assert_allclose(np.delete(l2, 1), l, rtol=1e-5)
+    assert np.all(g2[:, p[0]:p[0] + p[1]] == 0)
+    keep = np.r_[0:p[0], p[0] + p[1]:g2.shape[1]]
+    assert rel(g2[:, keep], g[:, :len(keep)]) <= 1e-4
+
+
 def test_graph_replays_match_eager():
     """The captured forward + adjoint pass, replayed several times with eager
     work and synchronisation in between, reproduces the eager result."""
