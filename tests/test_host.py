@@ -181,3 +181,13 @@ def test_workloads_are_seeded_and_sized():
     np.testing.assert_array_equal(w["scenes"][5]["x0"], w2["scenes"][5]["x0"])
     c2 = workloads.config2()
     assert c2["x0"].shape == (20000, 3) and c2["H"] * c2["substeps"] == 500
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """Without the built HIP library the product path raises; there is no
+    CPU fallback to route through."""
+    from diffskill_amd import _lib
+    monkeypatch.setattr(_lib, "_LIB", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libdiffskill_mpm.so"))
+    with pytest.raises(RuntimeError, match="not built"):
+        _lib.load()
