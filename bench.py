@@ -42,6 +42,7 @@ PHASE_KERNEL = {"stress": "k_stress", "p2g": "k_p2g_scatter", "p2g_grid": "k_p2g
                 "g2p_bwd": "k_g2p_bwd_tile", "p2g_bwd": "k_p2g_bwd_gather", "stress_bwd": "k_stress_bwd", "bin": "k_bin",
                 "grid_op": "k_grid_op", "grid_op_keep": "k_grid_op", "grid_op_bwd": "k_grid_op_bwd"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batched64_config4.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_issue_batched64_config4.json")
 
 
 def measured_traffic(phase):
@@ -51,6 +52,17 @@ def measured_traffic(phase):
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)[PHASE_KERNEL[phase]]
         return int(t["read_bytes"] + t["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def measured_valu(phase):
+    """(VALU wave-instructions per launch of the phase's kernel, issue peak per
+    second) from the committed rocprofv3 SQ counter summary, or None."""
+    try:
+        with open(VALU_FILE) as f:
+            t = json.load(f)
+        return t["kernels"][PHASE_KERNEL[phase]]["valu_instr_per_dispatch"], t["issue_peak_per_s"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -225,6 +237,15 @@ def main():
     roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom),
                     algo_bytes_per_launch=int(algo), avg_launch_us=kernels[dom]["avg_us"])
+    # the same kernel against the VALU issue rate (one wave64 instruction per
+    # SIMD every 4 cycles): its instruction count per launch is from the
+    # committed SQ counter run of this workload
+    valu_issue = None
+    mv = measured_valu(dom)
+    if mv is not None:
+        ach = mv[0] / avg_s
+        valu_issue = dict(kernel=dom, instr_per_launch=int(mv[0]), achieved=round(ach / 1e9, 2), peak=round(mv[1] / 1e9, 2),
+                          unit="G wave-instr/s", frac=round(ach / mv[1], 4))
     log("kernels:", json.dumps(kernels))
 
     # ---- plain per-scene GPU path, same kernels
@@ -266,7 +287,7 @@ def main():
                    "launch": launch or batch.engine.get_launch(), "autotuned": launch is not None,
                    "hip_graph": not args.no_graph},
         "plain": plain, "grad_rel_err": grad_err, "grad_rel_err_l2": grad_err_l2, "grad_checks": grad_checks,
-        "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
+        "kernels": kernels, "roofline": roofline, "valu_issue": valu_issue, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
