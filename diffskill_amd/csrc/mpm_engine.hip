@@ -529,19 +529,24 @@ DEV int tile_index(int l0, int l1, int l2, int n) { return ((l0 + n / 9) * 6 + (
 // Segmented scatter of 27 x NCOMP per-particle node values into an LDS tile
 // (component-major, TILE floats per component).  Lanes that share a cell are
 // summed through the transpose tree (at most MAXSEG distinct cells per wave),
-// the rest add directly.  VAL(lane_node, comp) gives a lane's value.  Values
-// are packed densely, v = node * NCOMP + comp, 32 per transpose pass.
-template <int NCOMP, class ValFn>
-DEV void scatter_tile(float* tile, bool valid, int key, int l0, int l1, int l2, int lane, ValFn val) {
+// the rest add directly.  CHAN(c, r) writes the lane's 27 node values of
+// component c into r[0..26] (node n = (a * 3 + b) * 3 + cc); one transpose
+// pass per component, so a caller forms a pass's 27 values incrementally
+// along the stencil.
+template <int NCOMP, class ChanFn>
+DEV void scatter_tile_ch(float* tile, bool valid, int key, int l0, int l1, int l2, int lane, ChanFn chan) {
     unsigned long long act = __ballot(valid);
     int seg = 0;
     while (act) {
         if (seg == MAXSEG) {
             if ((act >> lane) & 1ull) {
 #pragma unroll
-                for (int n = 0; n < 27; n++)
+                for (int c = 0; c < NCOMP; c++) {
+                    float r[27];
+                    chan(c, r);
 #pragma unroll
-                    for (int c = 0; c < NCOMP; c++) atomicAdd(&tile[c * TILE + tile_index(l0, l1, l2, n)], val(n, c));
+                    for (int n = 0; n < 27; n++) atomicAdd(&tile[c * TILE + tile_index(l0, l1, l2, n)], r[n]);
+                }
             }
             break;
         }
@@ -550,18 +555,15 @@ DEV void scatter_tile(float* tile, bool valid, int key, int l0, int l1, int l2, 
         unsigned long long m = __ballot(valid && key == key0);
         bool in = (m >> lane) & 1ull;
         int s0 = __shfl(l0, lead), s1 = __shfl(l1, lead), s2 = __shfl(l2, lead);
-        constexpr int NV = 27 * NCOMP;
 #pragma unroll
-        for (int pass = 0; pass < (NV + 31) / 32; pass++) {
+        for (int c = 0; c < NCOMP; c++) {
             float r[32];
+            chan(c, r);
 #pragma unroll
-            for (int j = 0; j < 32; j++) {
-                int v = pass * 32 + j;
-                r[j] = (in && v < NV) ? val(v / NCOMP, v % NCOMP) : 0.f;
-            }
+            for (int j = 0; j < 32; j++) r[j] = (in && j < 27) ? r[j] : 0.f;
             float s = tsum32(r, lane);
-            int v = pass * 32 + (lane >> 1);
-            if (!(lane & 1) && v < NV) atomicAdd(&tile[(v % NCOMP) * TILE + tile_index(s0, s1, s2, v / NCOMP)], s);
+            int n = lane >> 1;
+            if (!(lane & 1) && n < 27) atomicAdd(&tile[c * TILE + tile_index(s0, s1, s2, n)], s);
         }
         act &= ~m;
         seg++;
@@ -945,14 +947,29 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
             }
             if (valid) store3(a0, stride, 0, p, gxn + S.inv_dx * mk(gfx[0], gfx[1], gfx[2]));
             int key = (l0 << 4) | (l1 << 2) | l2;
-            auto val = [&](int n, int c) -> float {
-                int a = n / 9, b = (n / 3) % 3, cc = n % 3;
-                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[cc][2];
-                float dp0 = (float)a - sw.fx[0], dp1 = (float)b - sw.fx[1], dp2 = (float)cc - sw.fx[2];
-                float Cd = gC.a[c * 3] * dp0 + gC.a[c * 3 + 1] * dp1 + gC.a[c * 3 + 2] * dp2;
-                return wt * comp(gnv, c) + (c4 * wt) * Cd;
+            // the node's velocity-adjoint contribution is wt u (u as above),
+            // formed per component along the stencil
+            auto chan = [&](int c, float* r) {
+                float k0 = comp(K0, c), k1 = comp(K1, c), k2 = comp(K2, c);
+                float ua = comp(u0, c);
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    float ub = ua;
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        float wab = sw.w[a][0] * sw.w[b][1];
+                        float u = ub;
+#pragma unroll
+                        for (int cc = 0; cc < 3; cc++) {
+                            r[(a * 3 + b) * 3 + cc] = wab * (sw.w[cc][2] * u);
+                            u += k2;
+                        }
+                        ub += k1;
+                    }
+                    ua += k0;
+                }
             };
-            scatter_tile<3>(acc, valid && in_tile, key, l0, l1, l2, lane, val);
+            scatter_tile_ch<3>(acc, valid && in_tile, key, l0, l1, l2, lane, chan);
         }
         __syncthreads();
         for (int k = threadIdx.x; k < TILE; k += WG) tiles_v[(size_t)e * TILE + k] = make_float4(acc[k], acc[TILE + k], acc[2 * TILE + k], 0.f);
@@ -1147,14 +1164,30 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
             v3 mv = S.p_mass * v;
             const float* A = Am.a;
             float dx = S.dx, pm = S.p_mass;
-            auto val = [&](int n, int c) -> float {
-                int a = n / 9, b = (n / 3) % 3, cc = n % 3;
-                float wt = sw.w[a][0] * sw.w[b][1] * sw.w[cc][2];
-                if (c == 3) return wt * pm;
-                float d0 = ((float)a - sw.fx[0]) * dx, d1 = ((float)b - sw.fx[1]) * dx, d2 = ((float)cc - sw.fx[2]) * dx;
-                return wt * (comp(mv, c) + A[c * 3] * d0 + A[c * 3 + 1] * d1 + A[c * 3 + 2] * d2);
+            // momentum component c at node n = (a, b, cc): wt (mv + A dpos)[c]
+            // with dpos = (n - fx) dx, formed along the stencil as u0 + a k0 +
+            // b k1 + cc k2 (kj = dx A[c][j]); component 3 is the mass wt pm
+            auto chan = [&](int c, float* r) {
+                float k0 = c < 3 ? dx * A[c * 3] : 0.f, k1 = c < 3 ? dx * A[c * 3 + 1] : 0.f, k2 = c < 3 ? dx * A[c * 3 + 2] : 0.f;
+                float ua = c < 3 ? comp(mv, c) - (sw.fx[0] * k0 + sw.fx[1] * k1 + sw.fx[2] * k2) : pm;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    float ub = ua;
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        float wab = sw.w[a][0] * sw.w[b][1];
+                        float u = ub;
+#pragma unroll
+                        for (int cc = 0; cc < 3; cc++) {
+                            r[(a * 3 + b) * 3 + cc] = wab * (sw.w[cc][2] * u);
+                            u += k2;
+                        }
+                        ub += k1;
+                    }
+                    ua += k0;
+                }
             };
-            scatter_tile<4>(tile, valid && in_tile, key, l0, l1, l2, lane, val);
+            scatter_tile_ch<4>(tile, valid && in_tile, key, l0, l1, l2, lane, chan);
         }
         __syncthreads();
         for (int k = threadIdx.x; k < TILE; k += WG)
