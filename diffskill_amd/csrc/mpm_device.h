@@ -226,12 +226,30 @@ DEV m3 mcof(const m3& F) {
 // norm, descending; Givens QR of B gives U and R ~ diag(sigma).  U and V are
 // rotations, sigma_1 >= sigma_2 >= |sigma_3|, and sigma_3 carries the sign of
 // det(A) -- the convention of the reference's ti.svd (mpm_simulator.py:129).
+// 1/x and 1/sqrt(x) for the rotation angles: in fp64 the hardware estimates
+// (v_rcp_f64, v_rsq_f64) refined by two Newton steps (y <- y + y (1 - x y) and
+// y <- y + y (1/2 - x y^2 / 2); each step squares the relative error), a few
+// ulp from the correctly rounded division and square root at a fraction of
+// their instructions.  Inputs here are normal and positive (or nonzero).
+template <typename T> DEV T rot_rcp(T x) { return T(1) / x; }
+template <typename T> DEV T rot_rsq(T x) { return T(1) / sqrt(x); }
+template <> DEV double rot_rcp<double>(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = fma(y, fma(-x, y, 1.0), y);
+    return fma(y, fma(-x, y, 1.0), y);
+}
+template <> DEV double rot_rsq<double>(double x) {
+    double h = 0.5 * x, y = __builtin_amdgcn_rsq(x);
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    return fma(y, fma(-h * y, y, 0.5), y);
+}
 template <typename T>
 DEV void jacobi_rot(T& s_pp, T& s_qq, T& s_pq, T& s_rp, T& s_rq, T& vp0, T& vp1, T& vp2, T& vq0, T& vq1, T& vq2) {
     if (fabs(s_pq) < T(1e-30)) return;
-    T th = (s_qq - s_pp) / (T(2) * s_pq);
-    T t = (th >= T(0) ? T(1) : T(-1)) / (fabs(th) + sqrt(fma(th, th, T(1))));
-    T c = T(1) / sqrt(fma(t, t, T(1)));
+    T th = (s_qq - s_pp) * rot_rcp(T(2) * s_pq);
+    T h2 = fma(th, th, T(1));
+    T t = (th >= T(0) ? T(1) : T(-1)) * rot_rcp(fabs(th) + h2 * rot_rsq(h2));
+    T c = rot_rsq(fma(t, t, T(1)));
     T s = t * c;
     s_pp -= t * s_pq;
     s_qq += t * s_pq;
@@ -245,8 +263,15 @@ DEV void jacobi_rot(T& s_pp, T& s_qq, T& s_pq, T& s_rp, T& s_rq, T& vp0, T& vp1,
 }
 template <typename T>
 DEV void givens_qr(T& a, T& b, T& c, T& s) {
-    T r = sqrt(a * a + b * b);
-    if (r > T(1e-30)) { c = a / r; s = b / r; } else { c = T(1); s = T(0); }
+    T r2 = a * a + b * b;
+    if (r2 > T(1e-60)) {
+        T y = rot_rsq(r2);
+        c = a * y;
+        s = b * y;
+    } else {
+        c = T(1);
+        s = T(0);
+    }
 }
 // Jacobi on A^T A (at most 5 cyclic sweeps, fewer once the off-diagonal
 // part is below 1e-15 of the diagonal) for V, column sort, Givens QR of A V
