@@ -1083,6 +1083,13 @@ __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* 
     }
 }
 
+// 1/x from the hardware estimate (v_rcp_f32) and one Newton step
+// (y <- y + y (1 - x y)), for the adjoint's divisions by nonzero values
+DEV float frcp(float x) {
+    float y = __builtin_amdgcn_rcpf(x);
+    return fmaf(y, fmaf(-x, y, 1.f), y);
+}
+
 // backward_svd (mpm_simulator.py:136-156) with the gap clamp of :184-192
 DEV float gap_clamp(float a) { return a >= 0.f ? tmax(a, 1e-6f) : tmin(a, -1e-6f); }
 DEV m3 backward_svd(const m3& gu, const float gs[3], const m3& gv, const m3& u, const float sig[3], const m3& v) {
@@ -1091,7 +1098,7 @@ DEV m3 backward_svd(const m3& gu, const float gs[3], const m3& gv, const m3& u, 
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) Fm.a[i * 3 + j] = (i == j) ? 0.f : 1.f / gap_clamp(s[j] - s[i]);
+        for (int j = 0; j < 3; j++) Fm.a[i * 3 + j] = (i == j) ? 0.f : frcp(gap_clamp(s[j] - s[i]));
     m3 S = mzero(), GS = mzero();
 #pragma unroll
     for (int i = 0; i < 3; i++) { S.a[i * 4] = sig[i]; GS.a[i * 4] = gs[i]; }
@@ -1370,18 +1377,20 @@ __global__ void __launch_bounds__(PPB) k_stress_bwd(ChunkArgs ca, const float* _
         for (int kk = 0; kk < 9; kk++) { gU.a[kk] += aa.a[kk]; gV.a[kk] += bb.a[kk]; }
         m3 M = mmul(mmul_tn(t.U, gF), t.V);
         float gep[3], geps[3], geh[3];
-        float r = t.dg / t.ehn, gr = 0.f;
+        // ehn > 0 on this branch (it exceeds the yield offset dg > 0)
+        float inv_ehn = frcp(t.ehn);
+        float r = t.dg * inv_ehn, gr = 0.f;
 #pragma unroll
         for (int kk = 0; kk < 3; kk++) { gep[kk] = M.a[kk * 4] * t.e[kk]; geps[kk] = gep[kk]; geh[kk] = -r * gep[kk]; gr -= gep[kk] * t.eh[kk]; }
-        float gdg = gr / t.ehn;
-        float gehn = -gr * t.dg / (t.ehn * t.ehn) + gdg;
+        float gdg = gr * inv_ehn;
+        float gehn = -gr * t.dg * (inv_ehn * inv_ehn) + gdg;
 #pragma unroll
-        for (int kk = 0; kk < 3; kk++) geh[kk] += gehn * t.eh[kk] / t.ehn;
-        float mg = (geh[0] + geh[1] + geh[2]) / 3.f;
+        for (int kk = 0; kk < 3; kk++) geh[kk] += gehn * t.eh[kk] * inv_ehn;
+        float mg = (geh[0] + geh[1] + geh[2]) * (1.f / 3.f);
 #pragma unroll
         for (int kk = 0; kk < 3; kk++) {
             geps[kk] += geh[kk] - mg;
-            if (0.05f < t.sig[kk]) gS[kk] += geps[kk] / t.sc[kk];
+            if (0.05f < t.sig[kk]) gS[kk] += geps[kk] * frcp(t.sc[kk]);
         }
     } else {
         gFt = gF;
