@@ -5,11 +5,20 @@ Workload (one "step" = one trajectory-optimisation iteration): 64 scenes,
 particle counts log-uniform in [2k, 40k], each with its own 64^3 grid and 1-2
 capsule/box tools; 200 substeps forward + 200 adjoint substeps over all scenes,
 mass-grid L1 loss, action adjoints, one Adam step (lr 1e-2).  Inputs are
-resident in HBM before timing.  Reported next to it: the plain per-scene GPU
-path (one context per scene, same kernels, no packing, no graph), the
-gradient error of the batched path against the fp64 CPU oracle on one scene,
-live HIP-event kernel timings with the HBM roofline of the dominant kernel,
-and the fp32 CPU oracle timed on a bounded sample on the host cores.
+resident in HBM before timing.  Reported next to it:
+- the gradient accuracy gate: the batched path's action gradients on four
+  scenes (the two smallest, the smallest other capsule+box scene, the
+  largest) against the fp32 CPU oracle (the ti.f32 arithmetic of the
+  reference; the gate, <= 1e-3 relative) and the fp64 one (stated);
+- the plain per-scene GPU path (one context per scene, same kernels, no
+  packing, no graph);
+- forward-only throughput of the same workload, and of the plain forward
+  dough block (5k particles, 500 substeps);
+- the gather scene's device memory, store-all against checkpoints every K
+  in {10, 50, 100} frames, and the large cutting scene's fwd+adjoint rate;
+- live HIP-event kernel timings with the HBM roofline of the dominant kernel;
+- the fp32 CPU oracle timed on a bounded sample on all host cores the box
+  grants this job and on one core.
 """
 import argparse
 import json
@@ -22,6 +31,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "fwd+adjoint particle-substeps/s, batched vs plain, at grad rel err ≤1e-3"
 HBM_PEAK_GBS = 8000.0
+GRAD_GATE = 1e-3
+HOST_CORES = 16   # the host-core share of one GPU on the measurement box
 
 # algorithmic bytes per particle for each particle kernel: the particle-state
 # (and particle-adjoint) floats it must read and write once, fp32
@@ -38,9 +49,9 @@ ALGO_BYTES = {
 
 
 # engine phase -> kernel name in the rocprof summaries
-PHASE_KERNEL = {"stress": "k_stress", "p2g": "k_p2g_scatter", "p2g_grid": "k_p2g_scatter", "g2p": "k_g2p_tile",
+PHASE_KERNEL = {"stress": "k_stress", "p2g": "k_p2g_scatter", "p2g_grid": "k_p2g_scatter_rebuild", "g2p": "k_g2p_tile",
                 "g2p_bwd": "k_g2p_bwd_tile", "p2g_bwd": "k_p2g_bwd_gather", "stress_bwd": "k_stress_bwd", "bin": "k_bin",
-                "grid_op": "k_grid_op", "grid_op_keep": "k_grid_op", "grid_op_bwd": "k_grid_op_bwd"}
+                "grid_op": "k_grid_op", "grid_op_keep": "k_grid_op_rebuild", "grid_op_bwd": "k_grid_op_bwd"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batched64_config4.json")
 VALU_FILE = os.path.join(ROOT, "profiles", "valu_issue_batched64_config4.json")
 
@@ -71,7 +82,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_targets(w, batch):
+def build_targets(w):
     """Target mass grids: compute_grid_m of the slab target particles (same engine)."""
     import torch
     from diffskill_amd.batch import SceneBatch
@@ -83,34 +94,53 @@ def build_targets(w, batch):
     return out
 
 
-def oracle_scene_grads(w, k, table, target_grid, threads):
-    """fp64 oracle: loss and action adjoint of scene k under the given action table."""
+def check_scene_ids(sizes, scenes):
+    """The two smallest scenes, the smallest other scene with a box tool, the largest."""
+    import numpy as np
+    order = [int(k) for k in np.argsort(sizes)]
+    ids = order[:2]
+    box = [k for k in order if k not in ids and any(p["shape"] == "Box" for p in scenes[k]["prims"])]
+    if box:
+        ids.append(box[0])
+    if order[-1] not in ids:
+        ids.append(order[-1])
+    return ids
+
+
+def oracle_scene_grads(w, k, table, target_grid, bits, threads):
+    """CPU oracle (fp64 or fp32): loss and action adjoint of scene k under the given action table."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle, OracleRollout, prim_from_cfg, prim_state_from_cfg, reset_state, sim_from_cfg
     sc = w["scenes"][k]
     osim, S = sim_from_cfg(sc["sim"])
-    o = Oracle(osim, [prim_from_cfg(p) for p in sc["prims"]], bits=64, threads=threads)
+    o = Oracle(osim, [prim_from_cfg(p) for p in sc["prims"]], bits=bits, threads=threads)
     ro = OracleRollout(o, w["substeps"])
     dims = [p["action"]["dim"] for p in sc["prims"]]
     acts = np.concatenate([table[:, j, :d] for j, d in enumerate(dims)], axis=1)
     frames, _ = ro.run(reset_state(sc["x0"]), np.stack([prim_state_from_cfg(p) for p in sc["prims"]]), acts)
     T = len(frames) - 1
-    m = o.grid_mass(frames[T])
+    m = o.grid_mass(frames[T]).astype(np.float64)
     tg = target_grid.astype(np.float64)
     loss = np.abs(m - tg).sum()
-    ga, _, _ = ro.grad({T: o.grid_mass_grad(frames[T], np.sign(m - tg))})
-    return loss, ga, dims
+    ga, _, _ = ro.grad({T: o.grid_mass_grad(frames[T], np.sign(m - tg).astype(o.dtype))})
+    return loss, np.asarray(ga, np.float64), dims
 
 
-def cpu_baseline(w, budget_s, threads):
+def rel_errs(g, ref):
+    import numpy as np
+    return (float(np.abs(g - ref).max() / max(np.abs(ref).max(), 1e-30)),
+            float(np.linalg.norm(g - ref) / max(np.linalg.norm(ref), 1e-30)))
+
+
+def cpu_baseline(w, budget_s, threads, scenes=None):
     """fp32 oracle (C + OpenMP) on whole scenes of the same workload, fwd + adjoint
     over all 200 substeps, until ~budget_s seconds of CPU work have run."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle, OracleRollout, prim_from_cfg, prim_state_from_cfg, reset_state, sim_from_cfg
     done_ps, spent, used = 0, 0.0, []
-    order = list(range(len(w["scenes"])))
+    order = list(range(len(w["scenes"]))) if scenes is None else scenes
     for k in order:
         sc = w["scenes"][k]
         osim, _ = sim_from_cfg(sc["sim"])
@@ -130,6 +160,97 @@ def cpu_baseline(w, budget_s, threads):
     return done_ps / spent, used, spent
 
 
+def timed(fn, reps):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1000 / reps
+
+
+def config1_forward(reps=5):
+    """Plain forward dough block: 5k particles, capsule, 50 env steps x 10
+    substeps at dt 2e-4 (the largest stable step), captured as one graph."""
+    import torch
+    from diffskill_amd import workloads
+    from diffskill_amd.batch import BatchedTrajOpt, SceneBatch
+    w = workloads.config1(dt=2e-4)
+    H, S = w["H"], w["substeps"]
+    b = SceneBatch([dict(sim=w["sim"], prims=w["prims"], x0=w["x0"])], H * S + 1, keep_blocks=0)
+    n = b.consts[0]["n_grid"]
+    init = torch.zeros((H, 1, 7))
+    init[:, 0, :3] = torch.as_tensor(w["actions"], dtype=torch.float32)
+    opt = BatchedTrajOpt(b, H, S, torch.zeros((1, n, n, n), device="cuda"), lr=0.0, init_actions=init)
+    opt.forward()
+    ms = timed(opt.forward, reps)
+    return dict(workload="plain forward: 5k-particle dough block, capsule, 500 substeps at dt 2e-4, 64^3 grid",
+                particles=len(w["x0"]), substeps=H * S, ms_per_pass=round(ms, 3), value=len(w["x0"]) * H * S / (ms / 1000),
+                unit="particle-substeps/s (forward)")
+
+
+def config3_memory():
+    """Gather scene (30k particles, gripper + spatula, 1000 substeps, one-way
+    Chamfer loss) through the drop-in GradModel: peak device memory and
+    fwd+adjoint time for store-all and checkpoints every K frames."""
+    import numpy as np
+    import torch
+    from diffskill_amd import workloads
+    from diffskill_amd.engine import GradModel, TaichiEnv
+    from diffskill_amd.pointcloud import chamfer_distance
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    w = workloads.config3()
+    H = w["H"]
+    acts = np.concatenate(w["actions"], axis=1)
+    t = torch.as_tensor(w["target_points"], dtype=torch.float32, device="cuda")
+    out = {}
+    for K in (None, 10, 50, 100):
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        env = TaichiEnv(dict(SIMULATOR=w["sim"], PRIMITIVES=w["prims"], SHAPES=[]), init_particles=w["x0"], ckpt_k=K)
+        env.initialize()
+        gm = GradModel(env, softness=666.0)
+        t0 = time.perf_counter()
+        obs = gm.reset()
+        a = torch.tensor(acts, dtype=torch.float32, device="cuda", requires_grad=True)
+        for s in range(H):
+            obs = gm.forward(s, a[s], *obs)
+        loss = chamfer_distance(obs[0][:, :3], t)
+        loss.backward()
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        peak = torch.cuda.max_memory_allocated() - base
+        out["store-all" if K is None else "K=%d" % K] = dict(peak_bytes=int(peak), seconds=round(sec, 3),
+                                                            value=len(w["x0"]) * H * w["substeps"] / sec)
+        del env, gm, obs, a, loss
+    return dict(workload="gather scene: 30k particles, gripper + spatula, 1000 substeps fwd+adjoint through GradModel, "
+                         "one-way Chamfer loss", unit="particle-substeps/s (fwd+adjoint, eager, per-env-step host calls)",
+                runs=out)
+
+
+def config5_rate():
+    """The large cutting scene (1M particles, 128^3 grid, 2000 substeps, a
+    checkpoint every 50 frames): one fwd+adjoint pass, timed."""
+    import torch
+    from diffskill_amd import workloads
+    from diffskill_amd.batch import BatchedTrajOpt, SceneBatch
+    w = workloads.config5()
+    H, S = w["H"], w["substeps"]
+    n = 128
+    b = SceneBatch([dict(sim=w["sim"], prims=w["prims"], x0=w["x0"])], H * S + 1, ckpt_k=w["ckpt_k"])
+    init = torch.zeros((H, 1, 7))
+    init[:, 0, :3] = torch.as_tensor(w["actions"], dtype=torch.float32)
+    opt = BatchedTrajOpt(b, H, S, torch.zeros((1, n, n, n), device="cuda"), lr=0.0, init_actions=init, use_graph=False)
+    opt._device_pass()
+    ms = timed(opt._device_pass, 1)
+    return dict(workload="cutting scene: 1M particles, knife, 128^3 grid, 2000 substeps fwd+adjoint, checkpoint every 50",
+                particles=len(w["x0"]), substeps=H * S, ms_per_pass=round(ms, 1), value=len(w["x0"]) * H * S / (ms / 1000),
+                unit="particle-substeps/s (fwd+adjoint)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,8 +267,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other workloads (configs 1, 3, 5)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--check-scenes", type=int, default=2, help="scenes (smallest first) checked against the fp64 oracle")
+    ap.add_argument("--check-scenes", default="auto",
+                    help="scenes checked against the oracles: 'auto' (two smallest, a box scene, the largest) or a list")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
@@ -172,7 +295,7 @@ def main():
     T = H * S
     log("workload: %d scenes, %d particles, %d substeps" % (len(w["scenes"]), int(np.sum(w["sizes"])), T))
     batch = SceneBatch(w["scenes"], T + 1, ckpt_k=args.ckpt_k, keep_blocks=args.keep_blocks)
-    targets = build_targets(w, batch)
+    targets = build_targets(w)
     init = torch.as_tensor(workloads.action_tensor(w["actions"], H))
     opt = BatchedTrajOpt(batch, H, S, targets, lr=1e-2, init_actions=init, use_graph=not args.no_graph)
     n_total = batch.total_particles
@@ -184,27 +307,47 @@ def main():
         log("autotuned launch shape %s (%d candidates, %.1f-%.1f ms per eager pass)"
             % (launch, len(tune_ms), min(tune_ms.values()), max(tune_ms.values())))
 
-    # ---- gradient accuracy of the batched path vs the fp64 oracle (smallest scene)
-    grad_err, grad_err_l2, grad_checks = None, None, []
-    if not args.no_oracle and args.check_scenes > 0:
+    # ---- gradient accuracy gate: the batched path vs the fp32 oracle (the
+    # reference's ti.f32 arithmetic), with the fp64 distance stated
+    grad_checks, gate = [], None
+    if not args.no_oracle:
         opt._device_pass()
         torch.cuda.synchronize()
         g_all = opt.action_grads().detach().cpu().double().numpy()
         losses = opt.losses().detach().cpu().numpy()
-        for k in [int(i) for i in np.argsort(w["sizes"])[:args.check_scenes]]:
+        ids = check_scene_ids(w["sizes"], w["scenes"]) if args.check_scenes == "auto" else \
+            [int(v) for v in args.check_scenes.split(",") if v]
+        for k in ids:
             o0 = int(batch.prim_offsets[k])
             npk = len(w["scenes"][k]["prims"])
             table = torch.clamp(init, -1, 1).numpy()[:, o0:o0 + npk]
-            loss_ref, ga, dims = oracle_scene_grads(w, k, table, targets[k].cpu().numpy(), threads=min(16, os.cpu_count() or 1))
-            g_flat = np.concatenate([g_all[:, o0 + j, :d] for j, d in enumerate(dims)], axis=1)
-            e_max = float(np.abs(g_flat - ga).max() / max(np.abs(ga).max(), 1e-30))
-            e_l2 = float(np.linalg.norm(g_flat - ga) / max(np.linalg.norm(ga), 1e-30))
-            grad_checks.append(dict(scene=k, particles=int(w["sizes"][k]), tools=[p["shape"] for p in w["scenes"][k]["prims"]],
-                                    loss=float(losses[k]), loss_oracle=float(loss_ref), rel_err_max=e_max, rel_err_l2=e_l2))
-            log("grad check scene %d (%d particles): loss %.6g vs oracle %.6g, grad rel err max-element %.3g, l2 %.3g"
-                % (k, w["sizes"][k], losses[k], loss_ref, e_max, e_l2))
-        grad_err = max(c["rel_err_max"] for c in grad_checks)
-        grad_err_l2 = max(c["rel_err_l2"] for c in grad_checks)
+            tgt = targets[k].cpu().numpy()
+            c = dict(scene=k, particles=int(w["sizes"][k]), tools=[p["shape"] for p in w["scenes"][k]["prims"]],
+                     loss=float(losses[k]))
+            for bits in (32, 64):
+                t0 = time.perf_counter()
+                loss_ref, ga, dims = oracle_scene_grads(w, k, table, tgt, bits, HOST_CORES)
+                g_flat = np.concatenate([g_all[:, o0 + j, :d] for j, d in enumerate(dims)], axis=1)
+                e_max, e_l2 = rel_errs(g_flat, ga)
+                c["loss_fp%d" % bits] = float(loss_ref)
+                c["rel_err_max_fp%d" % bits] = e_max
+                c["rel_err_l2_fp%d" % bits] = e_l2
+                if bits == 32:
+                    g32 = ga
+                else:
+                    c["fp32_oracle_vs_fp64_max"] = rel_errs(g32, ga)[0]
+                log("grad check scene %d (%d particles, %s): loss %.6g vs fp%d oracle %.6g, grad rel err max %.3g, l2 %.3g (%.1fs)"
+                    % (k, w["sizes"][k], "+".join(c["tools"]), losses[k], bits, loss_ref, e_max, e_l2,
+                       time.perf_counter() - t0))
+            grad_checks.append(c)
+        worst = max(c["rel_err_max_fp32"] for c in grad_checks)
+        gate = dict(bound=GRAD_GATE, vs="fp32 oracle (ti.f32 arithmetic)", rel_err_max=worst,
+                    rel_err_l2=max(c["rel_err_l2_fp32"] for c in grad_checks),
+                    fp64_rel_err_max=max(c["rel_err_max_fp64"] for c in grad_checks),
+                    fp64_rel_err_l2=max(c["rel_err_l2_fp64"] for c in grad_checks),
+                    status="pass" if worst <= GRAD_GATE else "FAIL")
+        log("gradient gate (<= %.0e vs the fp32 oracle): %s, worst max-element %.3g; fp64 distance %.3g"
+            % (GRAD_GATE, gate["status"], worst, gate["fp64_rel_err_max"]))
 
     # ---- batched path: warmup (includes graph capture), then K timed iterations
     for _ in range(args.warmup):
@@ -218,6 +361,12 @@ def main():
     ms_step = (t1 - t0) * 1000 / args.steps
     value = n_total * T / (ms_step / 1000)
     log("batched: %.2f ms/iteration, %.4g particle-substeps/s (fwd+adj)" % (ms_step, value))
+
+    # ---- forward only (restart, 200 substeps, mass grids), its own graph
+    opt.forward()
+    fwd_ms = timed(opt.forward, max(1, args.steps))
+    forward = dict(ms_per_pass=round(fwd_ms, 3), value=n_total * T / (fwd_ms / 1000), unit="particle-substeps/s (forward)")
+    log("batched forward only: %.2f ms/pass, %.4g particle-substeps/s" % (fwd_ms, forward["value"]))
 
     # ---- live per-kernel timing (HIP events on the launch stream), eager pass
     e = batch.engine
@@ -254,26 +403,42 @@ def main():
         del opt
         p = PlainTrajOpt(w["scenes"], H, S, targets, lr=1e-2, init_actions=init, ckpt_k=args.ckpt_k, keep_blocks=args.keep_blocks)
         p.iteration()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        reps = 1
-        for _ in range(reps):
-            p.iteration()
-        torch.cuda.synchronize()
-        pm = (time.perf_counter() - t0) * 1000 / reps
+        pm = timed(p.iteration, 1)
         plain = dict(ms_per_step=round(pm, 3), value=n_total * T / (pm / 1000), speedup=round(pm / ms_step, 3))
         log("plain per-scene: %.2f ms/iteration, batched speedup %.2fx" % (pm, pm / ms_step))
         del p
+    launch_used = batch.engine.get_launch()
+    del batch
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+    # ---- the other workloads
+    extra = None
+    if not args.no_extra:
+        extra = {}
+        extra["config1_forward"] = config1_forward()
+        log("config 1 forward: %.2f ms/pass, %.4g particle-substeps/s" % (extra["config1_forward"]["ms_per_pass"],
+                                                                          extra["config1_forward"]["value"]))
+        extra["config3_memory"] = config3_memory()
+        for k, v in extra["config3_memory"]["runs"].items():
+            log("config 3 %s: peak %.3f GB, %.2f s fwd+adj" % (k, v["peak_bytes"] / 1e9, v["seconds"]))
+        torch.cuda.empty_cache()
+        extra["config5"] = config5_rate()
+        log("config 5: %.1f ms fwd+adj, %.4g particle-substeps/s" % (extra["config5"]["ms_per_pass"], extra["config5"]["value"]))
 
     # ---- CPU baseline: fp32 oracle on the host cores, bounded sample
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        v, used, spent = cpu_baseline(w, args.cpu_budget, threads)
-        cpu = dict(value=v, unit="particle-substeps/s (fwd+adjoint)", cores=threads, kind="port",
+        v, used, spent = cpu_baseline(w, args.cpu_budget, HOST_CORES)
+        cpu = dict(value=v, unit="particle-substeps/s (fwd+adjoint)", cores=HOST_CORES, kind="port",
                    sample="scenes %s of the workload (%d particles), all %d substeps fwd+adjoint, %.1f s"
                           % (used, int(sum(w["sizes"][i] for i in used)), T, spent))
-        log("cpu baseline: %.4g particle-substeps/s on %d threads" % (v, threads))
+        log("cpu baseline: %.4g particle-substeps/s on %d threads" % (v, HOST_CORES))
+        k0 = int(np.argsort(w["sizes"])[0])
+        v1, used1, spent1 = cpu_baseline(w, 0.0, 1, scenes=[k0])
+        cpu["one_core"] = dict(value=v1, cores=1, sample="scene %d (%d particles), all %d substeps fwd+adjoint, %.1f s"
+                                                         % (k0, int(w["sizes"][k0]), T, spent1))
+        log("cpu baseline, one core: %.4g particle-substeps/s" % v1)
 
     out = {
         "metric": METRIC, "value": value, "unit": "particle-substeps/s", "n_gpus": 1, "steps": args.steps,
@@ -283,10 +448,10 @@ def main():
                                "64^3 grid each, 1-2 capsule/box tools, %d substeps fwd+adjoint + Adam step"
                                % (len(w["scenes"]), n_total, T),
                    "scenes": len(w["scenes"]), "particles": n_total, "substeps": T, "grid": 64,
-                   "ckpt_k": args.ckpt_k or "store-all", "keep_blocks": batch.engine.keep_blocks,
-                   "launch": launch or batch.engine.get_launch(), "autotuned": launch is not None,
-                   "hip_graph": not args.no_graph},
-        "plain": plain, "grad_rel_err": grad_err, "grad_rel_err_l2": grad_err_l2, "grad_checks": grad_checks,
+                   "ckpt_k": args.ckpt_k or "store-all", "keep_blocks": args.keep_blocks,
+                   "launch": launch_used, "autotuned": launch is not None, "hip_graph": not args.no_graph},
+        "grad_gate": gate, "grad_checks": grad_checks,
+        "plain": plain, "forward": forward, "extra": extra,
         "kernels": kernels, "roofline": roofline, "valu_issue": valu_issue, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

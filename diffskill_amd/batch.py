@@ -139,7 +139,7 @@ class BatchedTrajOpt:
         self.use_graph = use_graph
         self.graph = None
 
-    def _device_pass(self):
+    def _forward_pass(self):
         e = self.b.engine
         self.b.restart()
         torch.clamp(self.actions.detach(), -1, 1, out=self.table)
@@ -149,6 +149,10 @@ class BatchedTrajOpt:
         e.substep_fwd(0, self.T)
         self.mass.zero_()
         e.grid_mass(self.T, self.mass)
+
+    def _device_pass(self):
+        e = self.b.engine
+        self._forward_pass()
         e.clear_grads(self.T)
         e.density_loss_bwd(self.T, self.mass, self.targets)
         e.substep_bwd(0, self.T)
@@ -180,6 +184,25 @@ class BatchedTrajOpt:
             self._device_pass()
             self._step()
         self.graph = g
+
+    def forward(self):
+        """Forward only: restart, the H env steps of every scene and the
+        final mass grids (the loss input), no adjoint and no update; with
+        ``use_graph`` captured once as a graph of its own and replayed."""
+        if not self.use_graph:
+            self._forward_pass()
+            return
+        if getattr(self, "fwd_graph", None) is None:
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                self._forward_pass()
+            torch.cuda.current_stream().wait_stream(s)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._forward_pass()
+            self.fwd_graph = g
+        self.fwd_graph.replay()
 
     def iteration(self):
         if self.use_graph:

@@ -247,8 +247,12 @@ template <typename T>
 DEV void jacobi_rot(T& s_pp, T& s_qq, T& s_pq, T& s_rp, T& s_rq, T& vp0, T& vp1, T& vp2, T& vq0, T& vq1, T& vq2) {
     if (fabs(s_pq) < T(1e-30)) return;
     T th = (s_qq - s_pp) * rot_rcp(T(2) * s_pq);
+    // t = sgn(th) / (|th| + sqrt(th^2 + 1)); past |th| = 1e15 that is
+    // 1 / (2 th) to the last bit in either precision, and th^2 would
+    // overflow fp32 (|th| reaches ~1e30 at |s_pq| ~ 1e-30)
     T h2 = fma(th, th, T(1));
-    T t = (th >= T(0) ? T(1) : T(-1)) * rot_rcp(fabs(th) + h2 * rot_rsq(h2));
+    T t = fabs(th) > T(1e15) ? T(0.5) * rot_rcp(th)
+                             : (th >= T(0) ? T(1) : T(-1)) * rot_rcp(fabs(th) + h2 * rot_rsq(h2));
     T c = rot_rsq(fma(t, t, T(1)));
     T s = t * c;
     s_pp -= t * s_pq;

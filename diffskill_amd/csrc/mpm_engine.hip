@@ -278,11 +278,10 @@ __global__ void __launch_bounds__(PPB) k_bin(BinArgs a, const float* __restrict_
         a.counters[CNT_ITEMS] = 0;
         a.counters[CNT_STAMP] = stamp;
     }
-    if (do_fk && blockIdx.x == 0 && (int)threadIdx.x < n_prims_total) {
-        int q = threadIdx.x;
-        prim_fk(prims[q], pstate + ((size_t)f * n_prims_total + q) * 8, pvel + ((size_t)f * n_prims_total + q) * 8,
-                pstate + ((size_t)(f + 1) * n_prims_total + q) * 8);
-    }
+    if (do_fk && blockIdx.x == 0)
+        for (int q = threadIdx.x; q < n_prims_total; q += blockDim.x)
+            prim_fk(prims[q], pstate + ((size_t)f * n_prims_total + q) * 8, pvel + ((size_t)f * n_prims_total + q) * 8,
+                    pstate + ((size_t)(f + 1) * n_prims_total + q) * 8);
     int4 ch = a.chunks[blockIdx.x];
     int i = threadIdx.x;
     int lane = i & 63;
@@ -701,8 +700,8 @@ DEV float4 gather_tiles(const BlockMeta& m, int lane, const float4* __restrict__
 // and the velocity.
 // With only_overflow set it runs only when the kept grid overflowed (the
 // adjoint then recomputes the grid instead of reading the kept one).
-__global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
-                                                 float4* __restrict__ g_v, int only_overflow) {
+DEV void grid_op_body(const GridArgs& ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm, float4* __restrict__ g_v,
+                      int only_overflow) {
     int lane = threadIdx.x & 63;
     int wave = threadIdx.x >> 6;
     int e = uniform(blockIdx.x * 4 + wave);
@@ -736,6 +735,18 @@ __global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, const float4* __re
             ga.keep_v[(size_t)e * NODES_PER_BLK + lane] = v4;
         }
     }
+}
+// the forward substep's grid operator
+__global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
+                                                 float4* __restrict__ g_v) {
+    grid_op_body(ga, tiles, g_pm, g_v, 0);
+}
+// the same operator rebuilding the grid of a substep for its adjoint (the
+// grid was not kept, or, with only_overflow, its active list outgrew the
+// kept grid): a kernel of its own so that profiles separate the two
+__global__ void __launch_bounds__(256) k_grid_op_rebuild(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
+                                                         float4* __restrict__ g_v, int only_overflow) {
+    grid_op_body(ga, tiles, g_pm, g_v, only_overflow);
 }
 
 // load the 6x6x6 node tile of a cell block from a node grid into LDS
@@ -1142,9 +1153,8 @@ __global__ void __launch_bounds__(PPB) k_stress(ChunkArgs ca, const float* __res
 // p2g scatter over work items (mpm_simulator.py:217-225): momentum and mass of
 // every particle of the item into the block's LDS tile, flushed to the item's
 // tile buffer.  Reads x, v and the stored affine matrix.
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_p2g_scatter(TileArgs ta, const float* __restrict__ st, const float* __restrict__ aff,
-                                                    float4* __restrict__ tiles, int only_overflow) {
-    __shared__ float tile[4 * TILE];
+DEV void p2g_scatter_body(float* tile, const TileArgs& ta, const float* __restrict__ st, const float* __restrict__ aff,
+                          float4* __restrict__ tiles, int only_overflow) {
     if (only_overflow && kept_fits(ta.counters, ta.keep_cap)) return;
     int n_items = ta.counters[CNT_ITEMS];
     int lane = threadIdx.x & 63;
@@ -1201,6 +1211,19 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
             tiles[(size_t)e * TILE + k] = make_float4(tile[k], tile[TILE + k], tile[2 * TILE + k], tile[3 * TILE + k]);
         __syncthreads();
     }
+}
+// the forward substep's p2g
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_p2g_scatter(TileArgs ta, const float* __restrict__ st, const float* __restrict__ aff,
+                                                    float4* __restrict__ tiles) {
+    __shared__ float tile[4 * TILE];
+    p2g_scatter_body(tile, ta, st, aff, tiles, 0);
+}
+// the same scatter rebuilding a substep's grid for its adjoint (see
+// k_grid_op_rebuild)
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_p2g_scatter_rebuild(TileArgs ta, const float* __restrict__ st, const float* __restrict__ aff,
+                                                    float4* __restrict__ tiles, int only_overflow) {
+    __shared__ float tile[4 * TILE];
+    p2g_scatter_body(tile, ta, st, aff, tiles, only_overflow);
 }
 
 // first half of the p2g adjoint over work items: gathers the node adjoints of
@@ -1333,11 +1356,11 @@ __global__ void __launch_bounds__(PPB) k_stress_bwd(ChunkArgs ca, const float* _
                                                     const PrimC* __restrict__ prims, int n_prims_total,
                                                     const float* __restrict__ pstate, const float* __restrict__ pvel,
                                                     float* __restrict__ gpstate, float* __restrict__ gpvel, int f) {
-    if (blockIdx.x == 0 && (int)threadIdx.x < n_prims_total) {
-        int q = threadIdx.x;
-        size_t o0 = ((size_t)f * n_prims_total + q) * 8, o1 = ((size_t)(f + 1) * n_prims_total + q) * 8;
-        prim_fk_rev(prims[q], pstate + o0, pvel + o0, gpstate + o1, gpstate + o0, gpvel + o0);
-    }
+    if (blockIdx.x == 0)
+        for (int q = threadIdx.x; q < n_prims_total; q += blockDim.x) {
+            size_t o0 = ((size_t)f * n_prims_total + q) * 8, o1 = ((size_t)(f + 1) * n_prims_total + q) * 8;
+            prim_fk_rev(prims[q], pstate + o0, pvel + o0, gpstate + o1, gpstate + o0, gpvel + o0);
+        }
     int4 ch = ca.chunks[blockIdx.x];
     int i = threadIdx.x;
     if (i >= ch.z) return;
@@ -2032,9 +2055,9 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
     }
     if (fork) HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
     { PhaseTimer t(c, MPM_PH_P2G, st);
-      hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm, 0); }
+      hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm); }
     { PhaseTimer t(c, MPM_PH_GRID, st);
-      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f, bs, c->keep), c->tiles_pm, c->g_pm, c->g_v, 0); }
+      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f, bs, c->keep), c->tiles_pm, c->g_pm, c->g_v); }
     { PhaseTimer t(c, MPM_PH_G2P, st);
       hipLaunchKernelGGL(k_g2p_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, out, c->g_v); }
     HIPCHK(hipGetLastError());
@@ -2065,10 +2088,10 @@ int launch_fallbacks(mpm_ctx* c, int f, hipStream_t s) {
     TileArgs ta = c->targs(bs);
     GridArgs ga = c->gargs(f, bs, false);
     { PhaseTimer t(c, MPM_PH_P2G_GRID, s);
-      hipLaunchKernelGGL(k_p2g_scatter, dim3(64), dim3(WG), 0, s, ta, (const float*)c->frame_ptr(f), (const float*)c->aff_ptr(f),
+      hipLaunchKernelGGL(k_p2g_scatter_rebuild, dim3(64), dim3(WG), 0, s, ta, (const float*)c->frame_ptr(f), (const float*)c->aff_ptr(f),
                          c->tiles_pm, 1); }
     { PhaseTimer t(c, MPM_PH_GRID_KEEP, s);
-      hipLaunchKernelGGL(k_grid_op, dim3(64), dim3(256), 0, s, ga, c->tiles_pm, c->g_pm, c->g_v, 1); }
+      hipLaunchKernelGGL(k_grid_op_rebuild, dim3(64), dim3(256), 0, s, ga, c->tiles_pm, c->g_pm, c->g_v, 1); }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -2104,9 +2127,9 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     // time than the two no-op launches themselves (80.4 vs 79.2 ms per pass).
     if (!kept) {
         { PhaseTimer t(c, MPM_PH_P2G_GRID, st);
-          hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm, 0); }
+          hipLaunchKernelGGL(k_p2g_scatter_rebuild, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm, 0); }
         { PhaseTimer t(c, MPM_PH_GRID_KEEP, st);
-          hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->tiles_pm, c->g_pm, c->g_v, 0); }
+          hipLaunchKernelGGL(k_grid_op_rebuild, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->tiles_pm, c->g_pm, c->g_v, 0); }
     } else if (c->keep_cap < c->n_blocks) {
         if (launch_fallbacks(c, f, st)) return -1;
     }
@@ -2182,7 +2205,25 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     for (int i = 0; i < n_scenes; i++) {
         if (scenes[i].n_grid < 8 || scenes[i].n_grid > 1024) return fail("mpm_create: n_grid outside [8, 1024]");
         if (scenes[i].n_particles < 0) return fail("mpm_create: negative particle count");
-        if (scenes[i].n_prims < 0 || scenes[i].n_prims > 4) return fail("mpm_create: at most 4 primitives per scene");
+        if (scenes[i].n_prims < 0) return fail("mpm_create: negative primitive count");
+    }
+    if (!prims) {
+        for (int i = 0; i < n_scenes; i++)
+            if (scenes[i].n_prims > 0) return fail("mpm_create: primitives are null");
+    }
+    // every collision unit of a scene (one per primitive, two for the
+    // two-jaw grippers) carries its pre-collide velocity through the grid
+    // adjoint in registers, MAX_UNITS of them
+    for (int i = 0, q = 0; i < n_scenes; q += scenes[i].n_prims, i++) {
+        int units = 0;
+        for (int j = 0; j < scenes[i].n_prims; j++) {
+            int t = prims[q + j].type;
+            if (t < MPM_SPHERE || t > MPM_TORUS) return fail("mpm_create: unknown primitive type " + std::to_string(t));
+            units += (t == MPM_GRIPPER || t == MPM_GRIPPER2) ? 2 : 1;
+        }
+        if (units > MAX_UNITS)
+            return fail("mpm_create: scene " + std::to_string(i) + " has " + std::to_string(units) +
+                        " collision units (grippers count two); at most " + std::to_string(MAX_UNITS));
     }
     Layout L = make_layout(scenes, n_scenes, max_frames, ckpt_k, keep_blocks);
     if (!workspace || workspace_bytes < (int64_t)L.total)
@@ -2443,6 +2484,17 @@ static void mark_set(mpm_ctx* c, int f) {
     else { c->seg_valid = f / c->K; c->seg_lo = f; c->seg_hi = f; }
 }
 
+// an entry point that adds into the adjoint of frame f: f must be a simulated
+// frame of the context and, once the adjoint chain has a top, that top
+static int adj_frame_check(mpm_ctx* c, int f, const char* who) {
+    if (f < 0 || f >= c->max_frames) return fail(std::string(who) + ": frame " + std::to_string(f) + " outside [0, max_frames)");
+    if (f > c->valid_hi) return fail(std::string(who) + ": frame " + std::to_string(f) + " has not been simulated");
+    if (c->adj_top >= 0 && f != c->adj_top)
+        return fail(std::string(who) + ": frame " + std::to_string(f) + " is not the current adjoint frame " +
+                    std::to_string(c->adj_top));
+    return 0;
+}
+
 int mpm_set_frame(mpm_ctx* ctx, int f, const float* src, void* stream) {
     if (!ctx || !src) return fail("mpm_set_frame: null argument");
     if (f < 0 || f >= ctx->max_frames) return fail("mpm_set_frame: frame out of range");
@@ -2537,8 +2589,8 @@ int mpm_grid_mass(mpm_ctx* ctx, int f, float* out, void* stream) {
 
 int mpm_grid_mass_bwd(mpm_ctx* ctx, int f, const float* g, void* stream) {
     if (!ctx || !g) return fail("mpm_grid_mass_bwd: null argument");
+    if (adj_frame_check(ctx, f, "mpm_grid_mass_bwd")) return -1;
     if (ctx->adj_top < 0) ctx->adj_top = f;
-    if (f != ctx->adj_top) return fail("mpm_grid_mass_bwd: frame is not the current adjoint frame");
     hipStream_t st = S_(stream);
     if (ensure_frame(ctx, f, st)) return -1;
     hipLaunchKernelGGL(k_grid_mass_bwd, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), g, (const float*)nullptr,
@@ -2549,8 +2601,8 @@ int mpm_grid_mass_bwd(mpm_ctx* ctx, int f, const float* g, void* stream) {
 
 int mpm_density_loss_bwd(mpm_ctx* ctx, int f, const float* mass, const float* target, float weight, void* stream) {
     if (!ctx || !mass || !target) return fail("mpm_density_loss_bwd: null argument");
+    if (adj_frame_check(ctx, f, "mpm_density_loss_bwd")) return -1;
     if (ctx->adj_top < 0) ctx->adj_top = f;
-    if (f != ctx->adj_top) return fail("mpm_density_loss_bwd: frame is not the current adjoint frame");
     hipStream_t st = S_(stream);
     if (ensure_frame(ctx, f, st)) return -1;
     hipLaunchKernelGGL(k_grid_mass_bwd, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), mass, target, weight,
@@ -2561,8 +2613,8 @@ int mpm_density_loss_bwd(mpm_ctx* ctx, int f, const float* mass, const float* ta
 
 int mpm_add_obs_grad(mpm_ctx* ctx, int f, const float* xv_grad, const float* prim_grad, void* stream) {
     if (!ctx) return fail("mpm_add_obs_grad: null context");
+    if (adj_frame_check(ctx, f, "mpm_add_obs_grad")) return -1;
     if (ctx->adj_top < 0) ctx->adj_top = f;
-    if (f != ctx->adj_top) return fail("mpm_add_obs_grad: frame is not the current adjoint frame");
     int n = std::max(ctx->stride, ctx->n_prims * 8);
     hipLaunchKernelGGL(k_obs_grad, dim3((n + 255) / 256), dim3(256), 0, S_(stream), ctx->stride, ctx->n_prims, xv_grad, prim_grad,
                        ctx->adj_ptr(f), ctx->pstate_g + (size_t)f * ctx->n_prims * 8);
@@ -2605,8 +2657,8 @@ int mpm_min_dist(mpm_ctx* ctx, int f, float* out, int n_cols, void* stream) {
 int mpm_min_dist_bwd(mpm_ctx* ctx, int f, const float* gout, int n_cols, void* stream) {
     if (!ctx || !gout) return fail("mpm_min_dist_bwd: null argument");
     if (n_cols < mpm_min_dist_cols(ctx, -1)) return fail("mpm_min_dist_bwd: adjoint has fewer columns than the scenes need");
+    if (adj_frame_check(ctx, f, "mpm_min_dist_bwd")) return -1;
     if (ctx->adj_top < 0) ctx->adj_top = f;
-    if (f != ctx->adj_top) return fail("mpm_min_dist_bwd: frame is not the current adjoint frame");
     hipStream_t st = S_(stream);
     if (ensure_frame(ctx, f, st)) return -1;
     if (ctx->n_prims == 0) return 0;
