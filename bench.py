@@ -205,7 +205,9 @@ def config3_memory():
     acts = np.concatenate(w["actions"], axis=1)
     t = torch.as_tensor(w["target_points"], dtype=torch.float32, device="cuda")
     out = {}
+    import gc
     for K in (None, 10, 50, 100):
+        gc.collect()
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
         torch.cuda.reset_peak_memory_stats()
@@ -223,7 +225,8 @@ def config3_memory():
         torch.cuda.synchronize()
         sec = time.perf_counter() - t0
         peak = torch.cuda.max_memory_allocated() - base
-        out["store-all" if K is None else "K=%d" % K] = dict(peak_bytes=int(peak), seconds=round(sec, 3),
+        ws = env.simulator.engine.ws.numel()
+        out["store-all" if K is None else "K=%d" % K] = dict(peak_bytes=int(peak), workspace_bytes=int(ws), seconds=round(sec, 3),
                                                             value=len(w["x0"]) * H * w["substeps"] / sec)
         del env, gm, obs, a, loss
     return dict(workload="gather scene: 30k particles, gripper + spatula, 1000 substeps fwd+adjoint through GradModel, "
@@ -421,7 +424,8 @@ def main():
                                                                           extra["config1_forward"]["value"]))
         extra["config3_memory"] = config3_memory()
         for k, v in extra["config3_memory"]["runs"].items():
-            log("config 3 %s: peak %.3f GB, %.2f s fwd+adj" % (k, v["peak_bytes"] / 1e9, v["seconds"]))
+            log("config 3 %s: peak %.3f GB (engine workspace %.3f GB), %.2f s fwd+adj"
+                % (k, v["peak_bytes"] / 1e9, v["workspace_bytes"] / 1e9, v["seconds"]))
         torch.cuda.empty_cache()
         extra["config5"] = config5_rate()
         log("config 5: %.1f ms fwd+adj, %.4g particle-substeps/s" % (extra["config5"]["ms_per_pass"], extra["config5"]["value"]))

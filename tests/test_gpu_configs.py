@@ -3,8 +3,9 @@ forward dough block, the single differentiable rolling-pin scene, the
 gripper + spatula gather scene with checkpointed adjoints and a Chamfer loss,
 and the large knife-cutting scene (reduced against the oracle, full scale
 through size-independent properties).  Each case runs through the drop-in
-TaichiEnv / GradModel API and is compared with the fp64 C oracle on the same
-seeded inputs."""
+TaichiEnv / GradModel API and is compared with the C oracle on the same
+seeded inputs: its fp32 build (the reference's ti.f32 arithmetic) holds the
+bars, its fp64 build's distance is stated."""
 import numpy as np
 import pytest
 import torch
@@ -56,11 +57,11 @@ def target_grid(sim_cfg, x):
 
 def test_config1_forward_parity():
     """5k-particle dough block, capsule, 50 env steps x 10 substeps at dt 2e-4
-    (the 5e-4 of the workload statement diverges in the oracle as well);
-    x within 1e-4 dx of the fp64 oracle after all 500 substeps, or within 1.5x
-    the distance of the fp32 oracle from it when that is larger (fp32 rounding
-    alone drifts x by ~1e-4 dx over 500 substeps); v within 1e-4 dx / dt (the
-    velocity that moves a particle 1e-4 dx in one substep); F within 1e-4."""
+    (the 5e-4 of the workload statement diverges in the oracle as well):
+    after all 500 substeps x within 1e-4 dx of the fp32 oracle (the
+    reference's ti.f32 arithmetic), v within 1e-4 dx / dt (the velocity that
+    moves a particle 1e-4 dx in one substep) and F within 1e-4; the distance
+    of both to the fp64 oracle is printed."""
     from diffskill_amd import workloads
     w = workloads.config1(dt=2e-4)
     env, _ = gradmodel(w, output_grid=False)
@@ -77,14 +78,14 @@ def test_config1_forward_parity():
     x, v, F, C = sim.get_state(T)[:4]
     ref = frames[T]
     dx = 1.0 / 64
-    err_x = np.abs(x - ref[:, 0:3]).max() / dx
-    err_x32 = np.abs(f32[:, 0:3] - ref[:, 0:3]).max() / dx
-    print("x err %.3g dx (fp32 oracle %.3g dx), v err %.3g (fp32 oracle %.3g), F err %.3g"
-          % (err_x, err_x32, rel(v, ref[:, 3:6]), rel(f32[:, 3:6], ref[:, 3:6]), np.abs(F.reshape(-1, 9) - ref[:, 15:24]).max()))
+    F = F.reshape(-1, 9)
+    print("vs fp32 oracle: x %.3g dx, v %.3g dx/dt, F %.3g; vs fp64: x %.3g dx (fp32 oracle %.3g dx)"
+          % (np.abs(x - f32[:, 0:3]).max() / dx, np.abs(v - f32[:, 3:6]).max() * w["sim"]["dt"] / dx,
+             np.abs(F - f32[:, 15:24]).max(), np.abs(x - ref[:, 0:3]).max() / dx, np.abs(f32[:, 0:3] - ref[:, 0:3]).max() / dx))
     assert np.isfinite(x).all()
-    assert err_x < max(1e-4, 1.5 * err_x32)
-    assert np.abs(v - ref[:, 3:6]).max() < 1e-4 * dx / w["sim"]["dt"]
-    assert np.abs(F.reshape(-1, 9) - ref[:, 15:24]).max() < 1e-4
+    assert np.abs(x - f32[:, 0:3]).max() < 1e-4 * dx
+    assert np.abs(v - f32[:, 3:6]).max() < 1e-4 * dx / w["sim"]["dt"]
+    assert np.abs(F - f32[:, 15:24]).max() < 1e-4
     np.testing.assert_allclose(P[0].get_state(T), ps[T][0, :7], atol=1e-5)
 
 
@@ -103,19 +104,18 @@ def _grid_l1_grads(w, target, ckpt_k=None, H=None):
 def _oracle_grid_l1(w, target, H=None, bits=64):
     o, ro, frames, _ = oracle_rollout(w, H=H, bits=bits)
     T = len(frames) - 1
-    m = o.grid_mass(frames[T])
+    m = o.grid_mass(frames[T]).astype(np.float64)
     tg = target.cpu().double().numpy()
-    ga, _, _ = ro.grad({T: o.grid_mass_grad(frames[T], np.sign(m - tg))})
-    return float(np.abs(m - tg).sum()), ga
+    ga, _, _ = ro.grad({T: o.grid_mass_grad(frames[T], np.sign(m - tg).astype(o.dtype))})
+    return float(np.abs(m - tg).sum()), np.asarray(ga, np.float64)
 
 
 def test_config2_action_grads_full_horizon():
     """20k-particle ellipsoid, RollingPin, 500 substeps, L1 to the slab's mass
-    grid: the action gradients through GradModel against the fp64 oracle.
-    Over this horizon fp32 rounding of the trajectory alone moves the
-    gradients by ~1.9e-3 (the fp32 oracle against the fp64 one, measured here
-    too), so the bound is 1e-3 or 1.5x that fp32 distance, whichever is
-    larger."""
+    grid: the action gradients through GradModel within 1e-3 (max element and
+    L2, relative) of the fp32 oracle, the reference's ti.f32 arithmetic.
+    Over this horizon fp32 rounding of the trajectory moves any fp32
+    evaluation ~1.9e-3 from the fp64 oracle; that distance is printed."""
     from diffskill_amd import workloads
     w = workloads.config2()
     target = target_grid(w["sim"], w["target_x"])
@@ -123,13 +123,14 @@ def test_config2_action_grads_full_horizon():
     loss, g = _grid_l1_grads(w, target)
     print("case config2 oracle", flush=True)
     loss_ref, g_ref = _oracle_grid_l1(w, target)
-    _, g32 = _oracle_grid_l1(w, target, bits=32)
-    l2 = lambda a: float(np.linalg.norm(a - g_ref) / np.linalg.norm(g_ref))  # noqa: E731
-    print("loss %.6g vs %.6g, grad rel max %.3g, l2 %.3g (fp32 oracle: max %.3g, l2 %.3g)"
-          % (loss, loss_ref, rel(g, g_ref), l2(g), rel(g32, g_ref), l2(g32)))
+    loss32, g32 = _oracle_grid_l1(w, target, bits=32)
+    l2 = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
+    print("loss %.6g vs %.6g (fp32 %.6g); grad vs fp32 oracle: max %.3g, l2 %.3g; vs fp64: max %.3g, l2 %.3g "
+          "(fp32 oracle vs fp64: max %.3g, l2 %.3g)" % (loss, loss_ref, loss32, rel(g, g32), l2(g, g32), rel(g, g_ref),
+                                                        l2(g, g_ref), rel(g32, g_ref), l2(g32, g_ref)))
     assert abs(loss - loss_ref) < 1e-4 * loss_ref
-    assert l2(g) < max(1e-3, 1.5 * l2(g32))
-    assert rel(g, g_ref) < max(1e-3, 1.5 * rel(g32, g_ref))
+    assert rel(g, g32) < 1e-3
+    assert l2(g, g32) < 1e-3
 
 
 def _config3_grads(w, ckpt_k, H):
@@ -208,11 +209,10 @@ def test_config3_oracle_parity_short_horizon():
 def test_config5_reduced_parity():
     """Knife-cutting scene on the 128^3 grid, 27k particles (the block shrunk
     at the same particle density), 2 env steps of 50 substeps with checkpoints
-    every 50 frames, mass-grid L1 loss: action gradients against the fp64
-    oracle.  The across-blade gradients of this 2 mm plate are sensitive to
-    fp32 rounding (the fp32 oracle is 3% from the fp64 one, and the GPU
-    matches the fp32 oracle to ~1e-4), so the bound is 1e-3 or 1.5x the fp32
-    oracle's distance, whichever is larger."""
+    every 50 frames, mass-grid L1 loss: action gradients within 1e-3 of the
+    fp32 oracle.  The across-blade gradients of this 2 mm plate are
+    sensitive to fp32 rounding (the fp32 oracle is 3% from the fp64 one);
+    that distance is printed."""
     from diffskill_amd import workloads
     w = workloads.config5(n=27000, n_env_steps=2)
     target = target_grid(w["sim"], w["target_x"])
@@ -220,10 +220,10 @@ def test_config5_reduced_parity():
     loss, g = _grid_l1_grads(w, target, ckpt_k=50)
     loss_ref, g_ref = _oracle_grid_l1(w, target)
     _, g32 = _oracle_grid_l1(w, target, bits=32)
-    print("loss %.6g vs %.6g, grad rel %.3g (fp32 oracle %.3g, gpu vs fp32 oracle %.3g)"
-          % (loss, loss_ref, rel(g, g_ref), rel(g32, g_ref), rel(g, g32)))
+    print("loss %.6g vs %.6g, grad rel vs fp32 oracle %.3g, vs fp64 %.3g (fp32 oracle vs fp64 %.3g)"
+          % (loss, loss_ref, rel(g, g32), rel(g, g_ref), rel(g32, g_ref)))
     assert abs(loss - loss_ref) < 1e-4 * loss_ref
-    assert rel(g, g_ref) < max(1e-3, 1.5 * rel(g32, g_ref))
+    assert rel(g, g32) < 1e-3
 
 
 def test_config5_full_scale_properties():
