@@ -49,13 +49,14 @@ EXPORTS = [
     "mpm_grid_mass", "mpm_grid_mass_bwd", "mpm_density_loss_bwd", "mpm_add_obs_grad", "mpm_action_grad", "mpm_clear_grads",
     "mpm_update_prims", "mpm_copy_frame", "mpm_set_profiling", "mpm_profile_read", "mpm_last_error",
     "mpm_set_launch", "mpm_get_launch", "mpm_min_dist_cols", "mpm_min_dist", "mpm_min_dist_bwd",
-    "mpm_collision_bytes", "mpm_set_collisions", "mpm_get_collisions",
+    "mpm_collision_bytes", "mpm_set_collisions", "mpm_get_collisions", "mpm_set_deterministic", "mpm_get_deterministic",
     "pc_nearest", "pc_last_error", "pc_sinkhorn_work_bytes", "pc_softmin", "pc_plan_grad",
     "pc_sinkhorn_step_work_bytes", "pc_sinkhorn_step",
 ]
 
 PHASES = ["bin", "compact", "blocks", "scatter", "stress", "p2g", "grid_op", "g2p", "p2g_grid", "grid_op_keep",
-          "g2p_bwd", "grid_op_bwd", "p2g_bwd", "stress_bwd", "collide", "collide_bwd"]
+          "g2p_bwd", "grid_op_bwd", "p2g_bwd", "stress_bwd", "collide", "collide_bwd",
+          "prim_reduce"]
 
 _LIB = None
 
@@ -106,6 +107,8 @@ def load():
         "mpm_set_collisions": (i32, [vp, i32, vp, i32, vp, vp, i64, vp]),
         "mpm_get_collisions": (i32, [vp, i32, vp, vp, vp, vp]),
         "mpm_profile_read": (i32, [vp, vp, vp, i32]),
+        "mpm_set_deterministic": (i32, [vp, i32]),
+        "mpm_get_deterministic": (i32, [vp]),
         "mpm_last_error": (ctypes.c_char_p, []),
         "pc_nearest": (i32, [vp, i32, vp, i32, vp, vp, vp, vp]),
         "pc_sinkhorn_work_bytes": (ctypes.c_longlong, [i32, i32]),

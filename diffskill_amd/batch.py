@@ -36,7 +36,7 @@ class SceneBatch:
     each substep's grid kept for the adjoint (-1 default capacity, 0 none).
     """
 
-    def __init__(self, scenes, max_frames, ckpt_k=0, softness=666.0, keep_blocks=-1):
+    def __init__(self, scenes, max_frames, ckpt_k=0, softness=666.0, keep_blocks=-1, deterministic=False):
         self.specs = scenes
         self.consts = [derive(s["sim"]) for s in scenes]
         self.prims = [_prim_objects(s["prims"], softness) for s in scenes]
@@ -44,6 +44,8 @@ class SceneBatch:
         pdescs = [p.desc() for ps in self.prims for p in ps]
         self.engine = Engine(descs, pdescs, max_frames, ckpt_k, keep_blocks)
         e = self.engine
+        if deterministic:
+            e.set_deterministic(True)
         self.n_scenes = len(scenes)
         self.prim_offsets = np.concatenate([[0], np.cumsum([len(p) for p in self.prims])]).astype(int)
         self.n_prims = int(self.prim_offsets[-1])
@@ -225,11 +227,13 @@ class PlainTrajOpt:
     driven with its own launches (no packing, no graph) -- the way the
     reference drives one Taichi scene per process."""
 
-    def __init__(self, scenes, H, substeps, targets, lr=1e-2, init_actions=None, ckpt_k=0, softness=666.0, keep_blocks=-1):
+    def __init__(self, scenes, H, substeps, targets, lr=1e-2, init_actions=None, ckpt_k=0, softness=666.0, keep_blocks=-1,
+                 deterministic=False):
         self.parts = []
         P0 = 0
         for k, sc in enumerate(scenes):
-            b = SceneBatch([sc], H * substeps + 1, ckpt_k=ckpt_k, softness=softness, keep_blocks=keep_blocks)
+            b = SceneBatch([sc], H * substeps + 1, ckpt_k=ckpt_k, softness=softness, keep_blocks=keep_blocks,
+                           deterministic=deterministic)
             ia = None
             if init_actions is not None:
                 ia = init_actions[:, P0:P0 + max(1, b.n_prims)]

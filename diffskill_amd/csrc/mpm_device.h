@@ -18,12 +18,13 @@ namespace dsk {
 struct SceneC {
     int n_grid, nb, p_off, p_cnt;
     int blk_off, prim_off, n_prims, gf_mode;   // gf_mode: 0 slip walls, 1 Coulomb ground, 2 sticky ground
-    int m_off, chunk_off, n_chunks, pad;
+    int m_off, chunk_off, n_chunks, bm_off;    // bm_off: first word of the scene's node-block bitmap
     long long g_off;                            // first node (block-major layout)
     float dx, inv_dx, dt, p_mass;
     float mu, lam, ys, stress_k;
     float c4, lower, upper, gf;
     float g30[4];
+    double p_mass_d;                            // p_mass before rounding to fp32
 };
 
 struct PrimC {

@@ -253,6 +253,15 @@ struct BinArgs {
     int4* items;
     int* order;
     int wi;          // particles per work item
+    // fixed-order mode: each (cell, wave) group of a pass records its count
+    // and the previous group of its cell (a list per cell whose head carries
+    // the pass stamp), so a particle's rank is its rank within its wave plus
+    // the counts of its cell's groups in lower waves -- slot order, not the
+    // order in which the groups' atomics landed
+    int det;
+    unsigned long long* cell_head;
+    int* grp_next;
+    int* grp_cnt;
 };
 
 DEV unsigned long long lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
@@ -312,7 +321,15 @@ __global__ void __launch_bounds__(PPB) k_bin(BinArgs a, const float* __restrict_
     int base = 0;
     bool fresh = false;
     if (valid && lane == leader) {
-        base = atomicAdd(&a.cell_count[q], cnt);
+        if (a.det) {
+            atomicAdd(&a.cell_count[q], cnt);
+            unsigned long long mine = ((unsigned long long)(unsigned)stamp << 32) | (unsigned)slot;
+            unsigned long long old = atomicExch(&a.cell_head[q], mine);
+            a.grp_next[slot] = ((int)(old >> 32) == stamp) ? (int)(unsigned)(old & 0xffffffffull) : -1;
+            a.grp_cnt[slot] = cnt;
+        } else {
+            base = atomicAdd(&a.cell_count[q], cnt);
+        }
         int gb = q >> 6;
         fresh = a.blk_flag[gb] != stamp && atomicExch(&a.blk_flag[gb], stamp) != stamp;
     }
@@ -459,7 +476,18 @@ __global__ void __launch_bounds__(PPB) k_scatter(BinArgs a, int* __restrict__ nr
     if (i >= ch.z) return;
     int slot = ch.y + i;
     int q = a.bin_q[slot];
-    a.order[a.cell_start[q] + a.bin_rank[slot]] = slot;
+    int r = a.bin_rank[slot];
+    if (a.det) {
+        // the cell's groups of this pass, newest first; those of lower waves
+        // (slots below this wave's) precede this particle
+        int rec = (int)(unsigned)(a.cell_head[q] & 0xffffffffull);
+        int w = slot >> 6;
+        while (rec >= 0) {
+            if ((rec >> 6) < w) r += a.grp_cnt[rec];
+            rec = a.grp_next[rec];
+        }
+    }
+    a.order[a.cell_start[q] + r] = slot;
 }
 
 // ------------------------------------------------------------------ wave transpose reduction
@@ -533,11 +561,14 @@ DEV int tile_index(int l0, int l1, int l2, int n) { return ((l0 + n / 9) * 6 + (
 // pass per component, so a caller forms a pass's 27 values incrementally
 // along the stencil.
 template <int NCOMP, class ChanFn>
-DEV void scatter_tile_ch(float* tile, bool valid, int key, int l0, int l1, int l2, int lane, ChanFn chan) {
+DEV void scatter_tile_ch(float* tile, bool valid, int key, int l0, int l1, int l2, int lane, bool det, ChanFn chan) {
     unsigned long long act = __ballot(valid);
     int seg = 0;
     while (act) {
-        if (seg == MAXSEG) {
+        // past MAXSEG distinct cells the remaining lanes add their own values
+        // (lanes of one instruction may meet at a node); the fixed-order mode
+        // takes every cell through the tree instead
+        if (seg == MAXSEG && !det) {
             if ((act >> lane) & 1ull) {
 #pragma unroll
                 for (int c = 0; c < NCOMP; c++) {
@@ -581,6 +612,9 @@ struct TileArgs {
     const float4* keep_v;
     const int* nmap;
     int keep_cap;
+    // fixed-order mode: every wave scatters into a tile of its own, and the
+    // workgroup's tiles are summed in wave order
+    int det;
 };
 
 struct GridArgs {
@@ -601,6 +635,13 @@ struct GridArgs {
     float4* keep_v;
     int keep_cap;
     int keep_write;        // grid_op stores its result into the kept grid
+    // fixed-order mode: the grid adjoint writes each node block's primitive
+    // adjoints (per collision unit, 16 values: frames f and f+1) to
+    // padj[block] and flags the block in its scene's bitmap; a reduction
+    // sums the flagged blocks in block order
+    int det;
+    float* padj;
+    unsigned* pbits;
 };
 
 // true when the active node-block list of this pass fits the kept grid
@@ -865,11 +906,14 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
                                                      const float* __restrict__ a1, float* __restrict__ a0,
                                                      const float4* __restrict__ g_v, float4* __restrict__ tiles_v, int use_kept) {
     __shared__ float4 tv[TILE];
-    __shared__ float acc[4 * TILE];
+    __shared__ float acc[(WG / 64) * 3 * TILE];
     int n_items = ta.counters[CNT_ITEMS];
     bool kept = use_kept && kept_fits(ta.counters, ta.keep_cap);
     int lane = threadIdx.x & 63;
     int stride = ta.stride;
+    bool det = ta.det != 0;
+    int n_acc = det ? (WG / 64) * 3 * TILE : 3 * TILE;
+    float* wacc = det ? acc + (threadIdx.x >> 6) * 3 * TILE : acc;
     for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
         int4 it = ta.items[e];
         // the first batch's particle data does not depend on the tile: its
@@ -885,7 +929,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
         block_origin(S, it.x, bi, bj, bk);
         if (kept) load_tile_kept(tv, S, bi, bj, bk, ta.keep_v, ta.nmap, ta.keep_cap);
         else load_tile(tv, S, bi, bj, bk, g_v);
-        for (int k = threadIdx.x; k < 4 * TILE; k += WG) acc[k] = 0.f;
+        for (int k = threadIdx.x; k < n_acc; k += WG) acc[k] = 0.f;
         __syncthreads();
         for (int k0 = 0; k0 < it.z; k0 += WG) {
             int k = k0 + threadIdx.x;
@@ -980,10 +1024,20 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
                     ua += k0;
                 }
             };
-            scatter_tile_ch<3>(acc, valid && in_tile, key, l0, l1, l2, lane, chan);
+            scatter_tile_ch<3>(wacc, valid && in_tile, key, l0, l1, l2, lane, det, chan);
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < TILE; k += WG) tiles_v[(size_t)e * TILE + k] = make_float4(acc[k], acc[TILE + k], acc[2 * TILE + k], 0.f);
+        for (int k = threadIdx.x; k < TILE; k += WG) {
+            float4 r = make_float4(acc[k], acc[TILE + k], acc[2 * TILE + k], 0.f);
+            if (det) {
+#pragma unroll
+                for (int w = 1; w < WG / 64; w++) {
+                    const float* t = acc + w * 3 * TILE;
+                    r.x += t[k]; r.y += t[TILE + k]; r.z += t[2 * TILE + k];
+                }
+            }
+            tiles_v[(size_t)e * TILE + k] = r;
+        }
         __syncthreads();
     }
 }
@@ -1061,6 +1115,7 @@ __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* 
         }
         // collision units in reverse order; each unit's primitive adjoint is wave-reduced
         v3 x = mk((float)I0 * S.dx, (float)I1 * S.dx, (float)I2 * S.dx);
+        const int blk_gb = m.gb;
         int uidx = 0;
         for (int q = 0; q < S.n_prims; q++) { Unit u[2]; uidx += prim_units(ga.prims[S.prim_off + q], u); }
         for (int q = S.n_prims - 1; q >= 0; q--) {
@@ -1077,12 +1132,23 @@ __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* 
                 bool any = false;
 #pragma unroll
                 for (int k = 0; k < 16; k++) any |= (vals[k] != 0.f);
-                if (__ballot(any) == 0ull) continue;
+                int k = lane >> 2;
+                if (__ballot(any) == 0ull) {
+                    if (ga.det && (lane & 3) == 0) ga.padj[((size_t)blk_gb * MAX_UNITS + uidx) * 16 + k] = 0.f;
+                    continue;
+                }
                 // the 16 wave sums at once; quad k adds value k
                 float s = tsum16(vals, lane);
-                int k = lane >> 2;
-                if ((lane & 3) == 0 && s != 0.f) unsafeAtomicAdd((k < 8 ? gps0 : gps1) + q * 8 + (k & 7), s);
+                if (ga.det) {
+                    if ((lane & 3) == 0) ga.padj[((size_t)blk_gb * MAX_UNITS + uidx) * 16 + k] = s;
+                } else if ((lane & 3) == 0 && s != 0.f) {
+                    unsafeAtomicAdd((k < 8 ? gps0 : gps1) + q * 8 + (k & 7), s);
+                }
             }
+        }
+        if (ga.det && lane == 0 && S.n_prims > 0) {
+            int lb = blk_gb - S.blk_off;
+            atomicOr(&ga.pbits[S.bm_off + (lb >> 5)], 1u << (lb & 31));
         }
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
         if (act) {
@@ -1159,12 +1225,15 @@ DEV void p2g_scatter_body(float* tile, const TileArgs& ta, const float* __restri
     int n_items = ta.counters[CNT_ITEMS];
     int lane = threadIdx.x & 63;
     int stride = ta.stride;
+    bool det = ta.det != 0;
+    int n_tile = det ? (WG / 64) * 4 * TILE : 4 * TILE;
+    float* wtile = det ? tile + (threadIdx.x >> 6) * 4 * TILE : tile;
     for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
         int4 it = ta.items[e];
         const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
         int bi, bj, bk;
         block_origin(S, it.x, bi, bj, bk);
-        for (int k = threadIdx.x; k < 4 * TILE; k += WG) tile[k] = 0.f;
+        for (int k = threadIdx.x; k < n_tile; k += WG) tile[k] = 0.f;
         __syncthreads();
         for (int k0 = 0; k0 < it.z; k0 += WG) {
             int k = k0 + threadIdx.x;
@@ -1204,25 +1273,34 @@ DEV void p2g_scatter_body(float* tile, const TileArgs& ta, const float* __restri
                     ua += k0;
                 }
             };
-            scatter_tile_ch<4>(tile, valid && in_tile, key, l0, l1, l2, lane, chan);
+            scatter_tile_ch<4>(wtile, valid && in_tile, key, l0, l1, l2, lane, det, chan);
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < TILE; k += WG)
-            tiles[(size_t)e * TILE + k] = make_float4(tile[k], tile[TILE + k], tile[2 * TILE + k], tile[3 * TILE + k]);
+        for (int k = threadIdx.x; k < TILE; k += WG) {
+            float4 r = make_float4(tile[k], tile[TILE + k], tile[2 * TILE + k], tile[3 * TILE + k]);
+            if (det) {
+#pragma unroll
+                for (int w = 1; w < WG / 64; w++) {
+                    const float* t = tile + w * 4 * TILE;
+                    r.x += t[k]; r.y += t[TILE + k]; r.z += t[2 * TILE + k]; r.w += t[3 * TILE + k];
+                }
+            }
+            tiles[(size_t)e * TILE + k] = r;
+        }
         __syncthreads();
     }
 }
 // the forward substep's p2g
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_p2g_scatter(TileArgs ta, const float* __restrict__ st, const float* __restrict__ aff,
                                                     float4* __restrict__ tiles) {
-    __shared__ float tile[4 * TILE];
+    __shared__ float tile[(WG / 64) * 4 * TILE];
     p2g_scatter_body(tile, ta, st, aff, tiles, 0);
 }
 // the same scatter rebuilding a substep's grid for its adjoint (see
 // k_grid_op_rebuild)
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_p2g_scatter_rebuild(TileArgs ta, const float* __restrict__ st, const float* __restrict__ aff,
                                                     float4* __restrict__ tiles, int only_overflow) {
-    __shared__ float tile[4 * TILE];
+    __shared__ float tile[(WG / 64) * 4 * TILE];
     p2g_scatter_body(tile, ta, st, aff, tiles, only_overflow);
 }
 
@@ -1431,13 +1509,100 @@ __global__ void __launch_bounds__(PPB) k_stress_bwd(ChunkArgs ca, const float* _
     store9(a0, stride, 15, p, gF0);
 }
 
-// compute_grid_m_kernel (mpm_simulator.py:456-466): dense per-scene output
+// Fixed-order mode: the primitive adjoints of substep f from the grid
+// adjoint's per-block partials, one workgroup per scene.  Thread t sums the
+// flagged blocks of its contiguous range of bitmap words in block order, the
+// threads' sums are combined by a fixed pairwise tree, then each primitive's
+// units are added in unit order into the adjoint of frames f (values 0-7)
+// and f+1 (8-15).  The bitmap words are cleared as they are read.
+constexpr int RED_WG = 256;
+__global__ void __launch_bounds__(RED_WG) k_prim_adj_reduce(const SceneC* __restrict__ scenes, const PrimC* __restrict__ prims,
+                                                            const float* __restrict__ padj, unsigned* __restrict__ pbits,
+                                                            float* __restrict__ gpstate, int n_prims_total, int f) {
+    const SceneC& S = scenes[blockIdx.x];
+    if (S.n_prims == 0) return;
+    __shared__ float red[RED_WG][17];
+    __shared__ float tot[MAX_UNITS][16];
+    int t = threadIdx.x;
+    int nbk = S.nb * S.nb * S.nb;
+    int nw = (nbk + 31) >> 5;
+    int w0 = (int)(((long long)nw * t) / RED_WG), w1 = (int)(((long long)nw * (t + 1)) / RED_WG);
+    int n_units = 0;
+    for (int q = 0; q < S.n_prims; q++) n_units += prims[S.prim_off + q].n_units;
+    // this thread's flagged blocks (at most a few words' worth)
+    unsigned words[8];
+    int nwl = min(w1 - w0, 8);
+    for (int i = 0; i < nwl; i++) words[i] = pbits[S.bm_off + w0 + i];
+    for (int u = 0; u < n_units; u++) {
+        float a[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) a[k] = 0.f;
+        for (int i = 0; i < w1 - w0; i++) {
+            unsigned wd = i < 8 ? words[i] : pbits[S.bm_off + w0 + i];
+            while (wd) {
+                int b = __ffs(wd) - 1;
+                wd &= wd - 1;
+                const float4* r = reinterpret_cast<const float4*>(padj + ((size_t)(S.blk_off + (w0 + i) * 32 + b) * MAX_UNITS + u) * 16);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    float4 v = r[j];
+                    a[4 * j] += v.x; a[4 * j + 1] += v.y; a[4 * j + 2] += v.z; a[4 * j + 3] += v.w;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 16; k++) red[t][k] = a[k];
+        __syncthreads();
+        for (int h = RED_WG / 2; h > 0; h >>= 1) {
+            if (t < h)
+#pragma unroll
+                for (int k = 0; k < 16; k++) red[t][k] += red[t + h][k];
+            __syncthreads();
+        }
+        if (t < 16) tot[u][t] = red[0][t];
+        __syncthreads();
+    }
+    for (int i = 0; i < w1 - w0; i++) pbits[S.bm_off + w0 + i] = 0u;
+    if (t < 16) {
+        int u = 0;
+        for (int q = 0; q < S.n_prims; q++) {
+            int nu = prims[S.prim_off + q].n_units;
+            float v = 0.f;
+            for (int j = 0; j < nu; j++, u++) v += tot[u][t];
+            int fr = t < 8 ? f : f + 1;
+            gpstate[((size_t)fr * n_prims_total + S.prim_off + q) * 8 + (t & 7)] += v;
+        }
+    }
+}
+
+// Fixed-order mode of compute_grid_m_kernel: the wave-reduced node masses are
+// added as integers (units of 2^-40 p_mass, exact in any order) into macc,
+// then converted and added into the output once all particles are in.
+constexpr double MASS_FIX = 1099511627776.0;   // 2^40
+__global__ void k_grid_mass_convert(const SceneC* __restrict__ scenes, unsigned long long* __restrict__ macc, float* __restrict__ out) {
+    const SceneC& S = scenes[blockIdx.y];
+    long long n3 = (long long)S.n_grid * S.n_grid * S.n_grid;
+    double scale = (double)S.p_mass_d / MASS_FIX;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long long)gridDim.x * blockDim.x) {
+        unsigned long long v = macc[S.m_off + i];
+        if (v) {
+            out[S.m_off + i] += (float)((double)(long long)v * scale);
+            macc[S.m_off + i] = 0ull;
+        }
+    }
+}
+DEV void mass_add(float* o, unsigned long long* macc, size_t id, float val, double inv_pmass, bool det) {
+    if (det) atomicAdd(macc + id, (unsigned long long)__double2ll_rn((double)val * inv_pmass * MASS_FIX));
+    else unsafeAtomicAdd(o + id, val);
+}
+
 // compute_grid_m_kernel (mpm_simulator.py:456-466) into a dense per-scene
 // grid.  Particles are stored in cell order, so the lanes of a wave mostly
 // share a stencil base: each group of equal bases is summed through the wave
 // transpose tree and adds its 27 node masses with one atomic each (lanes past
 // MAXSEG groups add their own).
-__global__ void __launch_bounds__(PPB) k_grid_mass(ChunkArgs ca, const float* __restrict__ st, float* __restrict__ out) {
+__global__ void __launch_bounds__(PPB) k_grid_mass(ChunkArgs ca, const float* __restrict__ st, float* __restrict__ out,
+                                                   unsigned long long* __restrict__ macc, int det) {
     int4 ch = ca.chunks[blockIdx.x];
     int i = threadIdx.x;
     int lane = i & 63;
@@ -1448,6 +1613,8 @@ __global__ void __launch_bounds__(PPB) k_grid_mass(ChunkArgs ca, const float* __
     Stencil sw;
     stencil(S, x, sw);
     float* o = out + (size_t)S.m_off;
+    unsigned long long* ma = macc + (size_t)S.m_off;
+    double inv_pm = 1.0 / S.p_mass_d;
     int n = S.n_grid;
     unsigned long long act = __ballot(valid);
     int seg = 0;
@@ -1458,7 +1625,8 @@ __global__ void __launch_bounds__(PPB) k_grid_mass(ChunkArgs ca, const float* __
                 for (int j = 0; j < 27; j++) {
                     int I0 = sw.base[0] + j / 9, I1 = sw.base[1] + (j / 3) % 3, I2 = sw.base[2] + j % 3;
                     if (in_grid(S, I0, I1, I2))
-                        unsafeAtomicAdd(o + ((size_t)I0 * n + I1) * n + I2, sw.w[j / 9][0] * sw.w[(j / 3) % 3][1] * sw.w[j % 3][2] * S.p_mass);
+                        mass_add(o, ma, ((size_t)I0 * n + I1) * n + I2, sw.w[j / 9][0] * sw.w[(j / 3) % 3][1] * sw.w[j % 3][2] * S.p_mass,
+                                 inv_pm, det);
                 }
             }
             break;
@@ -1475,7 +1643,7 @@ __global__ void __launch_bounds__(PPB) k_grid_mass(ChunkArgs ca, const float* __
         int j = lane >> 1;
         if (!(lane & 1) && j < 27) {
             int I0 = s0 + j / 9, I1 = s1 + (j / 3) % 3, I2 = s2 + j % 3;
-            if (in_grid(S, I0, I1, I2)) unsafeAtomicAdd(o + ((size_t)I0 * n + I1) * n + I2, sum);
+            if (in_grid(S, I0, I1, I2)) mass_add(o, ma, ((size_t)I0 * n + I1) * n + I2, sum, inv_pm, det);
         }
         act &= ~m;
         seg++;
@@ -1836,6 +2004,14 @@ struct mpm_ctx {
     int seg_hi = -1;      // highest resident ring frame of that segment
     int adj_top = -1;     // frame whose adjoint is current
     int grid_ctas = 1024; // persistent grid-phase workgroups
+    // fixed-order mode (mpm_set_deterministic): slot-ordered cell ranks,
+    // per-wave tiles, block-ordered primitive adjoints, integer mass grid
+    bool det = false;
+    unsigned long long* cell_head = nullptr;
+    int *grp_next = nullptr, *grp_cnt = nullptr;
+    float* padj = nullptr;
+    unsigned* pbits = nullptr;
+    unsigned long long* macc = nullptr;
     // the constitutive update of a forward substep does not depend on its
     // binning: it runs on a second stream (default priority) between an
     // event fork and join, so the two overlap (parallel branches in a graph)
@@ -1873,6 +2049,8 @@ struct mpm_ctx {
         a.blk_flag = blk_flag; a.node_flag = node_flag; a.counters = b.counters; a.gstamp = gstamp; a.pblocks = pblocks;
         a.blk_stamp = b.blk_stamp; a.blk_item0 = b.blk_item0; a.blk_nitems = b.blk_nitems; a.items = b.items; a.order = b.order;
         a.wi = wi;
+        a.det = det ? 1 : 0;
+        a.cell_head = cell_head; a.grp_next = grp_next; a.grp_cnt = grp_cnt;
         return a;
     }
     TileArgs targs(int bs) const {
@@ -1880,6 +2058,7 @@ struct mpm_ctx {
         TileArgs t;
         t.scenes = d_scenes; t.blk_scene = d_blk_scene; t.items = b.items; t.counters = b.counters; t.order = b.order; t.stride = stride;
         t.keep_v = b.keep_v; t.nmap = b.nmap; t.keep_cap = keep_cap;
+        t.det = det ? 1 : 0;
         return t;
     }
     // write_keep: the forward grid_op also stores the grid into the bin slot
@@ -1894,6 +2073,9 @@ struct mpm_ctx {
         g.keep_v = b.keep_v;
         g.keep_cap = b.keep_pm ? keep_cap : 0;
         g.keep_write = (write_keep && b.keep_pm) ? 1 : 0;
+        g.det = det ? 1 : 0;
+        g.padj = padj;
+        g.pbits = pbits;
         return g;
     }
 };
@@ -1919,7 +2101,12 @@ struct PhaseTimer {
 
 struct Layout {
     size_t off_scenes, off_prims, off_chunks, off_blk_scene, off_ckpt, off_ring, off_aff, off_usv, off_adj, off_grid, off_blkints, off_cells,
-        off_pints, off_bslots, off_keep, off_tiles, off_counters, off_prim_bufs, total;
+        off_pints, off_bslots, off_keep, off_tiles, off_counters, off_prim_bufs, off_det, total;
+    // fixed-order mode buffers (within off_det): cell list heads, group
+    // records, per-block primitive adjoints, block bitmaps, integer mass grid
+    size_t det_head, det_grp, det_padj, det_pbits, det_macc;
+    int n_bm_words;
+    long long n_mass;
     // one bin slot: order, items, blk_stamp, blk_item0, blk_nitems, nblocks, nmap, counters
     size_t bs_order, bs_items, bs_stamp, bs_item0, bs_nitems, bs_nblocks, bs_nmap, bs_counters, bs_nrec, bs_bytes, keep_bytes;
     int stride, n_chunks, n_blocks, n_ckpt, ring, K, max_items, n_bslots, keep_cap, rec_cap;
@@ -1933,15 +2120,17 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, int keep_blocks) {
     Layout L;
     memset(&L, 0, sizeof(L));
-    int stride = 0, n_chunks = 0, n_blocks = 0, n_prims = 0;
-    long long nodes = 0;
+    int stride = 0, n_chunks = 0, n_blocks = 0, n_prims = 0, bm_words = 0;
+    long long nodes = 0, mass = 0;
     for (int i = 0; i < n; i++) {
         int np = std::max(sc[i].n_particles, 0);
         stride += (np + 63) & ~63;
         n_chunks += (np + PPB - 1) / PPB;
         int nb = (sc[i].n_grid + 3) / 4;
         n_blocks += nb * nb * nb;
+        bm_words += (nb * nb * nb + 31) / 32;
         nodes += (long long)nb * nb * nb * NODES_PER_BLK;
+        mass += (long long)sc[i].n_grid * sc[i].n_grid * sc[i].n_grid;
         n_prims += sc[i].n_prims;
     }
     stride = std::max(64, (stride + 255) & ~255);
@@ -1999,6 +2188,16 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     L.off_tiles = o; o = align256(o + sizeof(float4) * (size_t)L.max_items * TILE * 2);
     L.off_counters = o; o = align256(o + sizeof(int) * 64);
     L.off_prim_bufs = o; o = align256(o + sizeof(float) * 8 * (size_t)max_frames * std::max(1, n_prims) * 4);
+    L.n_bm_words = std::max(1, bm_words);
+    L.n_mass = std::max(1LL, mass);
+    L.off_det = o;
+    size_t d = 0;
+    L.det_head = d; d = align256(d + sizeof(unsigned long long) * (size_t)L.n_blocks * 64);
+    L.det_grp = d; d = align256(d + sizeof(int) * (size_t)stride * 2);
+    L.det_padj = d; d = align256(d + sizeof(float) * 16 * MAX_UNITS * (size_t)L.n_blocks);
+    L.det_pbits = d; d = align256(d + sizeof(unsigned) * (size_t)L.n_bm_words);
+    L.det_macc = d; d = align256(d + sizeof(unsigned long long) * (size_t)L.n_mass);
+    o = align256(o + d);
     L.total = o;
     return L;
 }
@@ -2139,6 +2338,11 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     { PhaseTimer t(c, MPM_PH_GRID_BWD, st);
       hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->g_pm, c->tiles_v, c->g_adjpm, c->pstate_g,
                          use_kept); }
+    if (c->det && c->n_prims > 0) {
+        PhaseTimer t(c, MPM_PH_PRIM_REDUCE, st);
+        hipLaunchKernelGGL(k_prim_adj_reduce, dim3(c->n_scenes), dim3(RED_WG), 0, st, c->d_scenes, c->d_prims, (const float*)c->padj,
+                           c->pbits, c->pstate_g, c->n_prims, f);
+    }
     if (c->n_pairs > 0) {
         // between the grid adjoint (which adds into frame f+1's primitive
         // adjoint) and the kinematics adjoint (which consumes it)
@@ -2297,11 +2501,17 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->pstate_g = c->pstate + pb;
     c->pvel = c->pstate + 2 * pb;
     c->pvel_g = c->pstate + 3 * pb;
+    c->cell_head = (unsigned long long*)(w + L.off_det + L.det_head);
+    c->grp_next = (int*)(w + L.off_det + L.det_grp);
+    c->grp_cnt = c->grp_next + L.stride;
+    c->padj = (float*)(w + L.off_det + L.det_padj);
+    c->pbits = (unsigned*)(w + L.off_det + L.det_pbits);
+    c->macc = (unsigned long long*)(w + L.off_det + L.det_macc);
 
     // host tables
     std::vector<int4> chunks;
     std::vector<int> blk_scene(L.n_blocks, 0);
-    int poff = 0, boff = 0, prim_off = 0, moff = 0;
+    int poff = 0, boff = 0, prim_off = 0, moff = 0, bmoff = 0;
     long long goff = 0;
     for (int i = 0; i < n_scenes; i++) {
         const mpm_scene_desc& d = scenes[i];
@@ -2316,12 +2526,14 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
         s.n_prims = d.n_prims;
         s.gf_mode = d.ground_friction == 0 ? 0 : (d.ground_friction < 10 ? 1 : 2);
         s.m_off = moff;
+        s.bm_off = bmoff;
         s.chunk_off = (int)chunks.size();
         s.g_off = goff;
         s.dx = (float)d.dx;
         s.inv_dx = (float)d.inv_dx;
         s.dt = (float)d.dt;
         s.p_mass = (float)d.p_mass;
+        s.p_mass_d = d.p_mass;
         s.mu = (float)d.mu;
         s.lam = (float)d.lam;
         s.ys = (float)d.yield_stress;
@@ -2343,6 +2555,7 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
         c->p_off.push_back(poff);
         poff += (d.n_particles + 63) & ~63;
         boff += nbk;
+        bmoff += (nbk + 31) / 32;
         goff += (long long)nbk * NODES_PER_BLK;
         prim_off += d.n_prims;
         moff += d.n_grid * d.n_grid * d.n_grid;
@@ -2412,6 +2625,15 @@ int mpm_get_launch(mpm_ctx* ctx, int* particles_per_item, int* tile_ctas, int* g
     if (grid_ctas) *grid_ctas = ctx->grid_ctas;
     return 0;
 }
+
+int mpm_set_deterministic(mpm_ctx* ctx, int on) {
+    if (!ctx) return fail("mpm_set_deterministic: null context");
+    ctx->det = on != 0;
+    ctx->drop_kept();  // kept binnings were ranked the other way
+    return 0;
+}
+
+int mpm_get_deterministic(mpm_ctx* ctx) { return ctx ? (ctx->det ? 1 : 0) : fail("mpm_get_deterministic: null context"); }
 
 int mpm_set_profiling(mpm_ctx* ctx, int on) {
     if (!ctx) return fail("mpm_set_profiling: null context");
@@ -2582,7 +2804,10 @@ int mpm_grid_mass(mpm_ctx* ctx, int f, float* out, void* stream) {
     if (!ctx || !out) return fail("mpm_grid_mass: null argument");
     hipStream_t st = S_(stream);
     if (ensure_frame(ctx, f, st)) return -1;
-    hipLaunchKernelGGL(k_grid_mass, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), out);
+    hipLaunchKernelGGL(k_grid_mass, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), out, ctx->macc,
+                       ctx->det ? 1 : 0);
+    if (ctx->det)
+        hipLaunchKernelGGL(k_grid_mass_convert, dim3(256, ctx->n_scenes), dim3(256), 0, st, ctx->d_scenes, ctx->macc, out);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -256,6 +256,15 @@ class Engine:
         _lib.check(self.L.mpm_get_launch(self.ctx, *[ctypes.byref(x) for x in v]), "mpm_get_launch")
         return dict(particles_per_item=v[0].value, tile_ctas=v[1].value, grid_ctas=v[2].value)
 
+    def set_deterministic(self, on=True):
+        """Fixed-order execution: bitwise-identical results for identical
+        inputs and launch shape (see mpm_set_deterministic)."""
+        _lib.check(self.L.mpm_set_deterministic(self.ctx, 1 if on else 0), "mpm_set_deterministic")
+
+    @property
+    def deterministic(self):
+        return self.L.mpm_get_deterministic(self.ctx) == 1
+
     def set_profiling(self, on=True):
         _lib.check(self.L.mpm_set_profiling(self.ctx, 1 if on else 0), "mpm_set_profiling")
 

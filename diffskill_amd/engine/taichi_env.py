@@ -16,7 +16,7 @@ from .shapes import Shapes
 
 
 class TaichiEnv:
-    def __init__(self, cfg, nn=False, loss=True, return_dist=False, init_particles=None, ckpt_k=None):
+    def __init__(self, cfg, nn=False, loss=True, return_dist=False, init_particles=None, ckpt_k=None, deterministic=False):
         self.return_dist = bool(return_dist)
         full = get_cfg_defaults()
         full.merge(cfg)
@@ -33,7 +33,7 @@ class TaichiEnv:
             self.particle_colors = None
         self.init_particles = np.asarray(init_particles, np.float64)
         self.n_particles = len(self.init_particles)
-        self.simulator = MPMSimulator(full.SIMULATOR, self.primitives, ckpt_k=ckpt_k)
+        self.simulator = MPMSimulator(full.SIMULATOR, self.primitives, ckpt_k=ckpt_k, deterministic=deterministic)
         self.dim = self.simulator.dim
         self._is_copy = True
         self.device = "cuda"

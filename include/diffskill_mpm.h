@@ -219,8 +219,8 @@ enum {
     MPM_PH_BIN = 0, MPM_PH_COMPACT = 1, MPM_PH_BLOCKS = 2, MPM_PH_SCATTER = 3, MPM_PH_STRESS = 4,
     MPM_PH_P2G = 5, MPM_PH_GRID = 6, MPM_PH_G2P = 7, MPM_PH_P2G_GRID = 8, MPM_PH_GRID_KEEP = 9,
     MPM_PH_G2P_BWD = 10, MPM_PH_GRID_BWD = 11, MPM_PH_P2G_BWD = 12, MPM_PH_STRESS_BWD = 13,
-    MPM_PH_COLLIDE = 14, MPM_PH_COLLIDE_BWD = 15,
-    MPM_N_PHASES = 16
+    MPM_PH_COLLIDE = 14, MPM_PH_COLLIDE_BWD = 15, MPM_PH_PRIM_REDUCE = 16,
+    MPM_N_PHASES = 17
 };
 int mpm_set_profiling(mpm_ctx *ctx, int on);
 
@@ -235,6 +235,20 @@ int mpm_get_launch(mpm_ctx *ctx, int *particles_per_item, int *tile_ctas, int *g
 /* Accumulated milliseconds and launch counts per phase (synchronizes the
  * recorded events); reset != 0 clears the accumulators. */
 int mpm_profile_read(mpm_ctx *ctx, double *ms, long long *counts, int reset);
+
+/* Fixed-order execution (no counterpart in the reference, whose Taichi
+ * kernels accumulate with unordered atomics): with on != 0 every sum the
+ * engine forms is taken in an order fixed by the inputs alone -- particles
+ * ranked within their cell in slot order, each wave's tile contributions
+ * summed in LDS tiles of its own and the tiles in wave order, the grid
+ * adjoint's primitive adjoints per node block summed in block order, the
+ * mass grid accumulated in integers (2^-40 p_mass) -- so identical inputs
+ * and launch shape (mpm_set_launch's particles per work item) give bitwise
+ * identical frames, mass grids and adjoints, and a checkpointed adjoint
+ * equals the store-all one.  Off by default (unordered atomics, faster).
+ * Call outside graph capture. */
+int mpm_set_deterministic(mpm_ctx *ctx, int on);
+int mpm_get_deterministic(mpm_ctx *ctx);
 
 const char *mpm_last_error(void);
 
