@@ -1510,39 +1510,33 @@ __global__ void __launch_bounds__(PPB) k_stress_bwd(ChunkArgs ca, const float* _
 }
 
 // Fixed-order mode: the primitive adjoints of substep f from the grid
-// adjoint's per-block partials, one workgroup per scene.  Thread t sums the
-// flagged blocks of its contiguous range of bitmap words in block order, the
-// threads' sums are combined by a fixed pairwise tree, then each primitive's
+// adjoint's per-block partials, one wave per scene.  Lane l sums the flagged
+// blocks of its contiguous range of bitmap words in block order, the lanes'
+// sums are combined by the fixed butterfly of tsum16, then each primitive's
 // units are added in unit order into the adjoint of frames f (values 0-7)
 // and f+1 (8-15).  The bitmap words are cleared as they are read.
-constexpr int RED_WG = 256;
-__global__ void __launch_bounds__(RED_WG) k_prim_adj_reduce(const SceneC* __restrict__ scenes, const PrimC* __restrict__ prims,
-                                                            const float* __restrict__ padj, unsigned* __restrict__ pbits,
-                                                            float* __restrict__ gpstate, int n_prims_total, int f) {
+__global__ void __launch_bounds__(64) k_prim_adj_reduce(const SceneC* __restrict__ scenes, const PrimC* __restrict__ prims,
+                                                        const float* __restrict__ padj, unsigned* __restrict__ pbits,
+                                                        float* __restrict__ gpstate, int n_prims_total, int f) {
     const SceneC& S = scenes[blockIdx.x];
     if (S.n_prims == 0) return;
-    __shared__ float red[RED_WG][17];
     __shared__ float tot[MAX_UNITS][16];
-    int t = threadIdx.x;
+    int lane = threadIdx.x;
     int nbk = S.nb * S.nb * S.nb;
     int nw = (nbk + 31) >> 5;
-    int w0 = (int)(((long long)nw * t) / RED_WG), w1 = (int)(((long long)nw * (t + 1)) / RED_WG);
+    int w0 = (int)(((long long)nw * lane) >> 6), w1 = (int)(((long long)nw * (lane + 1)) >> 6);
     int n_units = 0;
     for (int q = 0; q < S.n_prims; q++) n_units += prims[S.prim_off + q].n_units;
-    // this thread's flagged blocks (at most a few words' worth)
-    unsigned words[8];
-    int nwl = min(w1 - w0, 8);
-    for (int i = 0; i < nwl; i++) words[i] = pbits[S.bm_off + w0 + i];
     for (int u = 0; u < n_units; u++) {
         float a[16];
 #pragma unroll
         for (int k = 0; k < 16; k++) a[k] = 0.f;
-        for (int i = 0; i < w1 - w0; i++) {
-            unsigned wd = i < 8 ? words[i] : pbits[S.bm_off + w0 + i];
+        for (int i = w0; i < w1; i++) {
+            unsigned wd = pbits[S.bm_off + i];
             while (wd) {
                 int b = __ffs(wd) - 1;
                 wd &= wd - 1;
-                const float4* r = reinterpret_cast<const float4*>(padj + ((size_t)(S.blk_off + (w0 + i) * 32 + b) * MAX_UNITS + u) * 16);
+                const float4* r = reinterpret_cast<const float4*>(padj + ((size_t)(S.blk_off + i * 32 + b) * MAX_UNITS + u) * 16);
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     float4 v = r[j];
@@ -1550,27 +1544,20 @@ __global__ void __launch_bounds__(RED_WG) k_prim_adj_reduce(const SceneC* __rest
                 }
             }
         }
-#pragma unroll
-        for (int k = 0; k < 16; k++) red[t][k] = a[k];
-        __syncthreads();
-        for (int h = RED_WG / 2; h > 0; h >>= 1) {
-            if (t < h)
-#pragma unroll
-                for (int k = 0; k < 16; k++) red[t][k] += red[t + h][k];
-            __syncthreads();
-        }
-        if (t < 16) tot[u][t] = red[0][t];
-        __syncthreads();
+        // quad k of the wave holds the total of value k
+        float t = tsum16(a, lane);
+        if ((lane & 3) == 0) tot[u][lane >> 2] = t;
     }
-    for (int i = 0; i < w1 - w0; i++) pbits[S.bm_off + w0 + i] = 0u;
-    if (t < 16) {
+    __syncthreads();
+    for (int i = w0; i < w1; i++) pbits[S.bm_off + i] = 0u;
+    if (lane < 16) {
         int u = 0;
         for (int q = 0; q < S.n_prims; q++) {
             int nu = prims[S.prim_off + q].n_units;
             float v = 0.f;
-            for (int j = 0; j < nu; j++, u++) v += tot[u][t];
-            int fr = t < 8 ? f : f + 1;
-            gpstate[((size_t)fr * n_prims_total + S.prim_off + q) * 8 + (t & 7)] += v;
+            for (int j = 0; j < nu; j++, u++) v += tot[u][lane];
+            int fr = lane < 8 ? f : f + 1;
+            gpstate[((size_t)fr * n_prims_total + S.prim_off + q) * 8 + (lane & 7)] += v;
         }
     }
 }
@@ -2340,7 +2327,7 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
                          use_kept); }
     if (c->det && c->n_prims > 0) {
         PhaseTimer t(c, MPM_PH_PRIM_REDUCE, st);
-        hipLaunchKernelGGL(k_prim_adj_reduce, dim3(c->n_scenes), dim3(RED_WG), 0, st, c->d_scenes, c->d_prims, (const float*)c->padj,
+        hipLaunchKernelGGL(k_prim_adj_reduce, dim3(c->n_scenes), dim3(64), 0, st, c->d_scenes, c->d_prims, (const float*)c->padj,
                            c->pbits, c->pstate_g, c->n_prims, f);
     }
     if (c->n_pairs > 0) {

@@ -280,3 +280,30 @@ def test_walls_and_clamp_parity(corner):
         err = np.linalg.norm(got[:, sl] - ref0[:, sl]) / np.linalg.norm(ref0[:, sl])
         print(name, err, flush=True)
         assert err < 1e-3, name
+
+
+def test_copy_mode_step_matches_oracle_rollout():
+    """MPMSimulator.step in copy mode (mpm_simulator.py:440-451): each env step
+    runs its substeps from frame 0 and copies the last frame (particles and
+    tools) back to frame 0, so after k steps frame 0 holds the state of frame
+    k * substeps of one continuous oracle rollout with the same actions;
+    non-copy steps advance cur and leave the frames in place."""
+    sim_cfg, prims, st = capsule_scene(n=1500, seed=8, substeps=4)
+    acts = np.random.default_rng(9).uniform(-0.8, 0.8, (3, 3))
+    osim, c = oracle_sim(sim_cfg)
+    o = Oracle(osim, [oracle_prim(p) for p in prims])
+    frames, ps = OracleRollout(o, c["substeps"]).run(st, np.stack([prim_state0(p) for p in prims]), acts)
+    S = c["substeps"]
+    dx = 1.0 / sim_cfg["n_grid"]
+    for is_copy in (True, False):
+        print("case copy-mode step is_copy=%s" % is_copy, flush=True)
+        sim, P = build(sim_cfg, prims, st)
+        for k in range(len(acts)):
+            sim.step(is_copy, acts[k])
+            f = 0 if is_copy else (k + 1) * S
+            assert sim.cur == f
+            got = flat(sim.get_state(f))
+            ref = frames[(k + 1) * S]
+            assert np.abs(got[:, 0:3] - ref[:, 0:3]).max() < 1e-4 * dx, (is_copy, k)
+            assert np.abs(got[:, 15:24] - ref[:, 15:24]).max() < 1e-4, (is_copy, k)
+            np.testing.assert_allclose(P[0].get_state(f), ps[(k + 1) * S][0, :7], atol=1e-5)
