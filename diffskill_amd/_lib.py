@@ -50,7 +50,7 @@ EXPORTS = [
     "mpm_update_prims", "mpm_copy_frame", "mpm_set_profiling", "mpm_profile_read", "mpm_last_error",
     "mpm_set_launch", "mpm_get_launch", "mpm_min_dist_cols", "mpm_min_dist", "mpm_min_dist_bwd",
     "mpm_collision_bytes", "mpm_set_collisions", "mpm_get_collisions", "mpm_set_deterministic", "mpm_get_deterministic",
-    "pc_nearest", "pc_last_error", "pc_sinkhorn_work_bytes", "pc_softmin", "pc_plan_grad",
+    "pc_nearest", "pc_nearest_noalloc", "pc_last_error", "pc_sinkhorn_work_bytes", "pc_softmin", "pc_plan_grad",
     "pc_sinkhorn_step_work_bytes", "pc_sinkhorn_step",
 ]
 
@@ -111,6 +111,7 @@ def load():
         "mpm_get_deterministic": (i32, [vp]),
         "mpm_last_error": (ctypes.c_char_p, []),
         "pc_nearest": (i32, [vp, i32, vp, i32, vp, vp, vp, vp]),
+        "pc_nearest_noalloc": (i32, [vp, i32, vp, i32, vp, vp, vp]),
         "pc_sinkhorn_work_bytes": (ctypes.c_longlong, [i32, i32]),
         "pc_softmin": (i32, [vp, i32, vp, i32, vp, ctypes.c_float, i32, vp, vp, vp]),
         "pc_plan_grad": (i32, [vp, i32, vp, i32, vp, vp, ctypes.c_float, i32, vp, vp, vp]),

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "../../include/diffskill_pointcloud.h"
@@ -30,6 +31,13 @@ int pc_fail(const std::string& m) {
     g_pc_err = m;
     return -1;
 }
+
+// device address of `scratch` on each GPU, looked up once per device (with
+// that device current) on the first pc_nearest_noalloc call there
+constexpr int PC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_dev[PC_MAX_DEVICES] = {};
+int noalloc_scratch(unsigned long long** out);
 
 __global__ void __launch_bounds__(PC_WG) k_pc_clear(unsigned long long* __restrict__ best, int n) {
     int i = blockIdx.x * PC_WG + threadIdx.x;
@@ -73,6 +81,30 @@ __global__ void __launch_bounds__(PC_WG) k_pc_nearest(const float* __restrict__ 
     }
     if (bk >= 0)
         atomicMin(&best[i], ((unsigned long long)__float_as_uint(bd) << 32) | (unsigned)(t0 + bk));
+}
+
+// Scratch of pc_nearest_noalloc: the per-query (distance, index) words that
+// pc_nearest takes from its workspace argument, as a file-scope device array
+// (one copy per GPU, held until the process ends).  Host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device.
+constexpr int PC_NOALLOC_MAX_QUERIES = 1 << 20;   // 8 MiB per GPU
+static __device__ __align__(256) unsigned long long scratch[PC_NOALLOC_MAX_QUERIES];
+
+int noalloc_scratch(unsigned long long** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return pc_fail(std::string("pc_nearest_noalloc: hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= PC_MAX_DEVICES) return pc_fail("pc_nearest_noalloc: device index outside [0, 64)");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return pc_fail(std::string("pc_nearest_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<unsigned long long*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
 }
 
 __global__ void __launch_bounds__(PC_WG) k_pc_unpack(const unsigned long long* __restrict__ best, int n, int* __restrict__ idx,
@@ -404,6 +436,19 @@ int pc_nearest(const float* q, int nq, const float* t, int nt, unsigned long lon
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return pc_fail(std::string("pc_nearest: ") + hipGetErrorString(e));
     return 0;
+}
+
+int pc_nearest_noalloc(const float* q, int nq, const float* t, int nt, int* idx, float* d2, void* stream) {
+    if (nq < 0 || nt < 0) return pc_fail("pc_nearest_noalloc: negative point count");
+    if (nq > PC_NOALLOC_MAX_QUERIES)
+        return pc_fail("pc_nearest_noalloc: " + std::to_string(nq) + " queries exceed the static scratch of " +
+                       std::to_string(PC_NOALLOC_MAX_QUERIES) + " (use pc_nearest with a workspace)");
+    if (nq == 0) return 0;
+    unsigned long long* work = nullptr;
+    if (noalloc_scratch(&work)) return -1;
+    // the scratch keeps the previous call's words; pc_nearest clears the
+    // first nq of them (a fill kernel) before the search
+    return pc_nearest(q, nq, t, nt, work, idx, d2, stream);
 }
 
 }  // extern "C"

@@ -35,6 +35,18 @@ extern "C" {
 int pc_nearest(const float *q, int nq, const float *t, int nt, unsigned long long *work, int *idx, float *d2, void *stream);
 const char *pc_last_error(void);
 
+/* pc_nearest without the workspace argument and without any allocation: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The per-query scratch is a static device array of the
+ * library, 2^20 queries (8 MiB) per GPU, held until the process ends; nq
+ * above that is refused with an error (-1) before any GPU work.  Every call
+ * on one GPU shares that GPU's array (cleared for its nq queries at the
+ * start of each call): calls that overlap on one GPU -- on different
+ * streams or threads -- race and must be serialised by the caller.  Same
+ * outputs as pc_nearest on the same inputs. */
+int pc_nearest_noalloc(const float *q, int nq, const float *t, int nt, int *idx, float *d2, void *stream);
+
 /* Bytes of device scratch that pc_softmin and pc_plan_grad need for n rows
  * and m columns (-1 for negative counts). */
 long long pc_sinkhorn_work_bytes(int n, int m);
