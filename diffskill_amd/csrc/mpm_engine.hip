@@ -227,8 +227,10 @@ constexpr int TILE = 216;    // 6x6x6 nodes
 constexpr int WG = 256;      // threads per work-item workgroup
 constexpr int MAXSEG = 4;    // distinct cells per wave reduced through the transpose tree
 
-// per bin slot counters
-enum { CNT_NBLK = 1, CNT_PART = 2, CNT_ITEMS = 3, CNT_STAMP = 4 };
+// per bin slot counters: CNT_X0 + x counts the work items of XCD x, CNT_XB0 + x
+// is the first entry of its list in xlist
+constexpr int NXCD = 8;      // MI355X: 8 XCDs, workgroup b dispatched to XCD b % 8
+enum { CNT_NBLK = 1, CNT_PART = 2, CNT_ITEMS = 3, CNT_STAMP = 4, CNT_X0 = 8, CNT_XB0 = 16 };
 // global binning state: the pass stamp base and the occupied-block count of
 // the pass in flight (both advanced/reset by the pass's last kernel)
 enum { G_STAMP = 0, G_PBLK = 1 };
@@ -262,6 +264,13 @@ struct BinArgs {
     unsigned long long* cell_head;
     int* grp_next;
     int* grp_cnt;
+    // XCD-aware work: blk_rep[block] = the slot of the particle whose group
+    // first reached the block this pass; the block's work items are tagged
+    // (item.w) with the XCD that owns that slot's eighth of the particle
+    // columns, so items whose particles share cache lines of the SoA rows run
+    // on one XCD; xlist lists the items grouped by tag
+    int* blk_rep;
+    int* xlist;
 };
 
 DEV unsigned long long lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
@@ -286,6 +295,7 @@ __global__ void __launch_bounds__(PPB) k_bin(BinArgs a, const float* __restrict_
         a.counters[CNT_PART] = 0;
         a.counters[CNT_ITEMS] = 0;
         a.counters[CNT_STAMP] = stamp;
+
     }
     if (do_fk && blockIdx.x == 0)
         for (int q = threadIdx.x; q < n_prims_total; q += blockDim.x)
@@ -332,6 +342,7 @@ __global__ void __launch_bounds__(PPB) k_bin(BinArgs a, const float* __restrict_
         }
         int gb = q >> 6;
         fresh = a.blk_flag[gb] != stamp && atomicExch(&a.blk_flag[gb], stamp) != stamp;
+        if (fresh) a.blk_rep[gb] = slot;
     }
     rank += __shfl(base, leader);
     unsigned long long nm = __ballot(fresh);
@@ -369,6 +380,7 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nbl
     __shared__ unsigned long long s_base;
     __shared__ int s_nn[4];
     __shared__ int s_nbase;
+
     int lane = threadIdx.x & 63;
     int wave = threadIdx.x >> 6;
     int n = a.gstamp[G_PBLK];
@@ -388,6 +400,7 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nbl
         }
         int total = __shfl(incl, 63);
         int nit = (total + a.wi - 1) / a.wi;
+        int xcd = has ? min(NXCD - 1, (int)(((long long)a.blk_rep[gb] * NXCD) / a.stride)) : 0;
         // node blocks of the footprint reached first by this pass
         int nb = -1, sc = 0, ni = 0, nj = 0, nk = 0;
         if (has && lane < 8) {
@@ -433,7 +446,7 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nbl
         }
         if (!has) continue;
         a.cell_start[gb * 64 + lane] = start + incl - cnt;
-        for (int it = lane; it < nit; it += 64) a.items[ib + it] = make_int4(gb, start + it * a.wi, min(a.wi, total - it * a.wi), 0);
+        for (int it = lane; it < nit; it += 64) a.items[ib + it] = make_int4(gb, start + it * a.wi, min(a.wi, total - it * a.wi), xcd);
         if (lane == 0) {
             a.blk_stamp[gb] = stamp;
             a.blk_item0[gb] = ib;
@@ -445,9 +458,65 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nbl
 // index scatter; the pass's last kernel also advances the stamp base and
 // resets the occupied-block count for the next pass
 __global__ void __launch_bounds__(PPB) k_scatter(BinArgs a, int* __restrict__ nrec, int rec_cap) {
+    __shared__ int s_cnt[PPB][NXCD];
+    __shared__ int s_wtot[PPB / 64][NXCD];
+    __shared__ int s_base[NXCD];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.gstamp[G_STAMP] += 1;
         a.gstamp[G_PBLK] = 0;
+    }
+    if (blockIdx.x == 0) {
+        // the work items grouped by XCD tag, each group in item order: thread t
+        // counts a contiguous range of items per tag, the counts are scanned
+        // over the threads (wave scans, then the waves' totals), then every
+        // item is placed
+        int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+        int n = a.counters[CNT_ITEMS];
+        int per = (n + PPB - 1) / PPB;
+        int e0 = min(n, t * per), e1 = min(n, e0 + per);
+#pragma unroll
+        for (int x = 0; x < NXCD; x++) s_cnt[t][x] = 0;
+        for (int e = e0; e < e1; e++) s_cnt[t][a.items[e].w] += 1;
+        int c[NXCD], incl[NXCD];
+#pragma unroll
+        for (int x = 0; x < NXCD; x++) { c[x] = s_cnt[t][x]; incl[x] = c[x]; }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1)
+#pragma unroll
+            for (int x = 0; x < NXCD; x++) {
+                int u = __shfl_up(incl[x], d);
+                if (lane >= d) incl[x] += u;
+            }
+        if (lane == 63)
+#pragma unroll
+            for (int x = 0; x < NXCD; x++) s_wtot[wave][x] = incl[x];
+        __syncthreads();
+        if (t == 0) {
+            int acc = 0;
+            for (int x = 0; x < NXCD; x++) {
+                int tot = 0;
+                for (int w = 0; w < PPB / 64; w++) tot += s_wtot[w][x];
+                a.counters[CNT_X0 + x] = tot;
+                a.counters[CNT_XB0 + x] = acc;
+                s_base[x] = acc;
+                acc += tot;
+            }
+        }
+        __syncthreads();
+        int off[NXCD];
+#pragma unroll
+        for (int x = 0; x < NXCD; x++) {
+            int b = s_base[x] + incl[x] - c[x];
+            for (int w = 0; w < wave; w++) b += s_wtot[w][x];
+            off[x] = b;
+        }
+#pragma unroll
+        for (int x = 0; x < NXCD; x++) s_cnt[t][x] = off[x];
+        for (int e = e0; e < e1; e++) {
+            int x = a.items[e].w;
+            a.xlist[s_cnt[t][x]] = e;
+            s_cnt[t][x] += 1;
+        }
     }
     // node-block records: covering block ids -> their item ranges this pass
     int n_rec = min(a.counters[CNT_NBLK], rec_cap);
@@ -615,7 +684,30 @@ struct TileArgs {
     // fixed-order mode: every wave scatters into a tile of its own, and the
     // workgroup's tiles are summed in wave order
     int det;
+    const int* xlist;      // work items grouped by XCD (see BinArgs)
 };
+
+// The work items a workgroup of a tile kernel takes: those of its own XCD's
+// list (workgroup b runs on XCD b % NXCD), entry j going to that XCD's
+// workgroup j mod (its workgroup count); with fewer workgroups than XCDs,
+// every item in turn.
+struct ItemIter { const int* list; int j, step, n; };
+DEV ItemIter item_iter(const TileArgs& ta) {
+    ItemIter r;
+    if ((int)gridDim.x >= NXCD) {
+        int x = blockIdx.x % NXCD;
+        r.list = ta.xlist + ta.counters[CNT_XB0 + x];
+        r.n = ta.counters[CNT_X0 + x];
+        r.j = blockIdx.x / NXCD;
+        r.step = ((int)gridDim.x - x + NXCD - 1) / NXCD;
+    } else {
+        r.list = nullptr;
+        r.n = ta.counters[CNT_ITEMS];
+        r.j = blockIdx.x;
+        r.step = gridDim.x;
+    }
+    return r;
+}
 
 struct GridArgs {
     const SceneC* scenes;
@@ -821,9 +913,10 @@ DEV void load_tile_kept(float4* tv, const SceneC& S, int bi, int bj, int bk, con
 __global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __restrict__ st, float* __restrict__ st_out,
                                                  const float4* __restrict__ g_v) {
     __shared__ float4 tv[TILE];
-    int n_items = ta.counters[CNT_ITEMS];
+    ItemIter ir = item_iter(ta);
     int stride = ta.stride;
-    for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
+    for (int j = ir.j; j < ir.n; j += ir.step) {
+        int e = ir.list ? ir.list[j] : j;
         int4 it = ta.items[e];
         // the first batch's positions do not depend on the tile: their loads
         // are issued first, so both latencies overlap; every later batch's
@@ -907,14 +1000,15 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
                                                      const float4* __restrict__ g_v, float4* __restrict__ tiles_v, int use_kept) {
     __shared__ float4 tv[TILE];
     __shared__ float acc[(WG / 64) * 3 * TILE];
-    int n_items = ta.counters[CNT_ITEMS];
+    ItemIter ir = item_iter(ta);
     bool kept = use_kept && kept_fits(ta.counters, ta.keep_cap);
     int lane = threadIdx.x & 63;
     int stride = ta.stride;
     bool det = ta.det != 0;
     int n_acc = det ? (WG / 64) * 3 * TILE : 3 * TILE;
     float* wacc = det ? acc + (threadIdx.x >> 6) * 3 * TILE : acc;
-    for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
+    for (int j = ir.j; j < ir.n; j += ir.step) {
+        int e = ir.list ? ir.list[j] : j;
         int4 it = ta.items[e];
         // the first batch's particle data does not depend on the tile: its
         // loads are issued first, so both latencies overlap
@@ -1222,13 +1316,14 @@ __global__ void __launch_bounds__(PPB) k_stress(ChunkArgs ca, const float* __res
 DEV void p2g_scatter_body(float* tile, const TileArgs& ta, const float* __restrict__ st, const float* __restrict__ aff,
                           float4* __restrict__ tiles, int only_overflow) {
     if (only_overflow && kept_fits(ta.counters, ta.keep_cap)) return;
-    int n_items = ta.counters[CNT_ITEMS];
+    ItemIter ir = item_iter(ta);
     int lane = threadIdx.x & 63;
     int stride = ta.stride;
     bool det = ta.det != 0;
     int n_tile = det ? (WG / 64) * 4 * TILE : 4 * TILE;
     float* wtile = det ? tile + (threadIdx.x >> 6) * 4 * TILE : tile;
-    for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
+    for (int j = ir.j; j < ir.n; j += ir.step) {
+        int e = ir.list ? ir.list[j] : j;
         int4 it = ta.items[e];
         const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
         int bi, bj, bk;
@@ -1311,9 +1406,10 @@ __global__ void __launch_bounds__(WG) k_p2g_bwd_gather(TileArgs ta, const float*
                                                        float* __restrict__ a0, float* __restrict__ gaff_out,
                                                        const float4* __restrict__ g_adjpm) {
     __shared__ float4 tg[TILE];
-    int n_items = ta.counters[CNT_ITEMS];
+    ItemIter ir = item_iter(ta);
     int stride = ta.stride;
-    for (int e = blockIdx.x; e < n_items; e += gridDim.x) {
+    for (int j = ir.j; j < ir.n; j += ir.step) {
+        int e = ir.list ? ir.list[j] : j;
         int4 it = ta.items[e];
         // the first batch's particle data does not depend on the tile: its
         // loads are issued first, so both latencies overlap
@@ -1707,10 +1803,12 @@ __global__ void __launch_bounds__(PPB) k_min_dist(ChunkArgs ca, const float* __r
 
 // compute_min_dist.grad (function.py:154-160): gout[col][stride] adjoints of
 // the distances; adds d/dx into the particle adjoint (x rows) and the
-// primitive-state adjoint of frame f (wave-reduced, one atomic per wave)
+// primitive-state adjoint of frame f (wave-reduced, one atomic per wave; in
+// the fixed-order mode each wave's sums go to pmd[wave][prim] instead and
+// k_min_dist_reduce adds them in wave order)
 __global__ void __launch_bounds__(PPB) k_min_dist_bwd(ChunkArgs ca, const float* __restrict__ st, const PrimC* __restrict__ prims,
                                                       const float* __restrict__ ps, const float* __restrict__ gout,
-                                                      float* __restrict__ adj, float* __restrict__ gps) {
+                                                      float* __restrict__ adj, float* __restrict__ gps, float* __restrict__ pmd) {
     int4 ch = ca.chunks[blockIdx.x];
     int i = threadIdx.x;
     int lane = i & 63;
@@ -1734,16 +1832,51 @@ __global__ void __launch_bounds__(PPB) k_min_dist_bwd(ChunkArgs ca, const float*
         bool any = false;
 #pragma unroll
         for (int k = 0; k < 8; k++) any |= (vals[k] != 0.f);
-        if (__ballot(any) == 0ull) continue;
+        float* wp = pmd ? pmd + ((size_t)((ch.y + (i & ~63)) >> 6) * MAX_UNITS + q) * 8 : nullptr;
+        if (__ballot(any) == 0ull) {
+            if (wp && lane < 8) wp[lane] = 0.f;
+            continue;
+        }
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             float v = vals[k];
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-            if (lane == 0 && v != 0.f) unsafeAtomicAdd(gps + (size_t)(S.prim_off + q) * 8 + k, v);
+            if (wp) {
+                if (lane == 0) wp[k] = v;
+            } else if (lane == 0 && v != 0.f) {
+                unsafeAtomicAdd(gps + (size_t)(S.prim_off + q) * 8 + k, v);
+            }
         }
     }
     if (valid) store3(adj, ca.stride, 0, slot, load3(adj, ca.stride, 0, slot) + gx);
+}
+
+// Fixed-order mode of the distance adjoint: one wave per scene adds its
+// particle waves' per-primitive sums in wave order (lane l takes waves l,
+// l + 64, ..., then a fixed butterfly) into the primitive-state adjoint.
+__global__ void __launch_bounds__(64) k_min_dist_reduce(const SceneC* __restrict__ scenes, const float* __restrict__ pmd,
+                                                        float* __restrict__ gps) {
+    const SceneC& S = scenes[blockIdx.x];
+    int lane = threadIdx.x;
+    int w0 = S.p_off >> 6, nw = (S.p_cnt + 63) >> 6;
+    for (int q = 0; q < S.n_prims; q++) {
+        float a[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = 0.f;
+        for (int w = lane; w < nw; w += 64) {
+            const float* r = pmd + ((size_t)(w0 + w) * MAX_UNITS + q) * 8;
+#pragma unroll
+            for (int k = 0; k < 8; k++) a[k] += r[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) a[k] += __shfl_xor(a[k], o);
+        if (lane == 0)
+#pragma unroll
+            for (int k = 0; k < 8; k++) gps[(size_t)(S.prim_off + q) * 8 + k] += a[k];
+    }
 }
 
 // ------------------------------------------------------------------ tool-tool collision
@@ -1964,7 +2097,7 @@ struct mpm_ctx {
     float* adj[2] = {nullptr, nullptr};
     float4 *g_pm = nullptr, *g_v = nullptr, *g_adjpm = nullptr;
     int *blk_flag = nullptr, *node_flag = nullptr, *pblocks = nullptr, *gstamp = nullptr;
-    int *cell_count = nullptr, *cell_start = nullptr, *bin_q = nullptr, *bin_rank = nullptr;
+    int *cell_count = nullptr, *cell_start = nullptr, *bin_q = nullptr, *bin_rank = nullptr, *blk_rep = nullptr;
     // Bin slots: the binning of a substep (index list, work items, per-block
     // item ranges and stamps, active node-block list and map, counters) and,
     // when the grid is kept, that substep's grid (momentum/mass and velocity of
@@ -1972,7 +2105,7 @@ struct mpm_ctx {
     // slot, so the adjoint of a substep the forward ran reuses both instead of
     // binning and rebuilding the grid again; otherwise one shared slot.
     struct BinSlot {
-        int* order; int4* items; int* blk_stamp; int* blk_item0; int* blk_nitems; int* nblocks; int* nmap; int* counters; int* nrec;
+        int* order; int4* items; int* blk_stamp; int* blk_item0; int* blk_nitems; int* nblocks; int* nmap; int* counters; int* nrec; int* xlist;
         float4* keep_pm; float4* keep_v;
     };
     std::vector<BinSlot> bslots;
@@ -1999,6 +2132,7 @@ struct mpm_ctx {
     float* padj = nullptr;
     unsigned* pbits = nullptr;
     unsigned long long* macc = nullptr;
+    float* pmd = nullptr;
     // the constitutive update of a forward substep does not depend on its
     // binning: it runs on a second stream (default priority) between an
     // event fork and join, so the two overlap (parallel branches in a graph)
@@ -2038,6 +2172,7 @@ struct mpm_ctx {
         a.wi = wi;
         a.det = det ? 1 : 0;
         a.cell_head = cell_head; a.grp_next = grp_next; a.grp_cnt = grp_cnt;
+        a.blk_rep = blk_rep; a.xlist = b.xlist;
         return a;
     }
     TileArgs targs(int bs) const {
@@ -2046,6 +2181,7 @@ struct mpm_ctx {
         t.scenes = d_scenes; t.blk_scene = d_blk_scene; t.items = b.items; t.counters = b.counters; t.order = b.order; t.stride = stride;
         t.keep_v = b.keep_v; t.nmap = b.nmap; t.keep_cap = keep_cap;
         t.det = det ? 1 : 0;
+        t.xlist = b.xlist;
         return t;
     }
     // write_keep: the forward grid_op also stores the grid into the bin slot
@@ -2063,6 +2199,7 @@ struct mpm_ctx {
         g.det = det ? 1 : 0;
         g.padj = padj;
         g.pbits = pbits;
+
         return g;
     }
 };
@@ -2091,11 +2228,11 @@ struct Layout {
         off_pints, off_bslots, off_keep, off_tiles, off_counters, off_prim_bufs, off_det, total;
     // fixed-order mode buffers (within off_det): cell list heads, group
     // records, per-block primitive adjoints, block bitmaps, integer mass grid
-    size_t det_head, det_grp, det_padj, det_pbits, det_macc;
+    size_t det_head, det_grp, det_padj, det_pbits, det_macc, det_pmd;
     int n_bm_words;
     long long n_mass;
     // one bin slot: order, items, blk_stamp, blk_item0, blk_nitems, nblocks, nmap, counters
-    size_t bs_order, bs_items, bs_stamp, bs_item0, bs_nitems, bs_nblocks, bs_nmap, bs_counters, bs_nrec, bs_bytes, keep_bytes;
+    size_t bs_order, bs_items, bs_stamp, bs_item0, bs_nitems, bs_nblocks, bs_nmap, bs_counters, bs_nrec, bs_xlist, bs_bytes, keep_bytes;
     int stride, n_chunks, n_blocks, n_ckpt, ring, K, max_items, n_bslots, keep_cap, rec_cap;
     long long n_nodes;
     int n_prims;
@@ -2142,7 +2279,7 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     L.off_usv = o; o = align256(o + sizeof(float) * USV_ROWS * (size_t)stride * (L.n_ckpt + L.ring));
     L.off_adj = o; o = align256(o + ff * 2);
     L.off_grid = o; o = align256(o + sizeof(float4) * (size_t)L.n_nodes * 3);
-    L.off_blkints = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 3);   // block flags, node flags, occupied list
+    L.off_blkints = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 4);   // block flags, node flags, occupied list, representative slots
     L.off_cells = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 64 * 2); // cell counts, cell starts
     L.off_pints = o; o = align256(o + sizeof(int) * (size_t)stride * 2);         // bin key, bin rank
     // work items: at most one partial item per occupied cell block plus
@@ -2168,6 +2305,7 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     L.bs_nmap = b; b = align256(b + sizeof(int) * (size_t)L.n_blocks);
     L.bs_counters = b; b = align256(b + sizeof(int) * 64);
     L.bs_nrec = b; b = align256(b + sizeof(int) * NREC * (size_t)L.rec_cap);
+    L.bs_xlist = b; b = align256(b + sizeof(int) * (size_t)L.max_items);
     L.bs_bytes = b;
     L.keep_bytes = (size_t)L.keep_cap * NODES_PER_BLK * sizeof(float4) * 2;
     L.off_bslots = o; o = align256(o + L.bs_bytes * L.n_bslots);
@@ -2184,6 +2322,9 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     L.det_padj = d; d = align256(d + sizeof(float) * 16 * MAX_UNITS * (size_t)L.n_blocks);
     L.det_pbits = d; d = align256(d + sizeof(unsigned) * (size_t)L.n_bm_words);
     L.det_macc = d; d = align256(d + sizeof(unsigned long long) * (size_t)L.n_mass);
+    // one record per particle wave of any chunk: a chunk's last waves may
+    // lie past the stride (its first slot plus 256)
+    L.det_pmd = d; d = align256(d + sizeof(float) * 8 * MAX_UNITS * (size_t)(stride / 64 + PPB / 64));
     o = align256(o + d);
     L.total = o;
     return L;
@@ -2452,6 +2593,7 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->blk_flag = bi;
     c->node_flag = bi + (size_t)L.n_blocks;
     c->pblocks = bi + (size_t)L.n_blocks * 2;
+    c->blk_rep = bi + (size_t)L.n_blocks * 3;
     c->cell_count = (int*)(w + L.off_cells);
     c->cell_start = c->cell_count + (size_t)L.n_blocks * 64;
     c->bin_q = (int*)(w + L.off_pints);
@@ -2471,6 +2613,7 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
         S.nmap = (int*)(b + L.bs_nmap);
         S.counters = (int*)(b + L.bs_counters);
         S.nrec = (int*)(b + L.bs_nrec);
+        S.xlist = (int*)(b + L.bs_xlist);
         S.keep_pm = S.keep_v = nullptr;
         if (c->keep) {
             S.keep_pm = (float4*)(w + L.off_keep + (size_t)k * L.keep_bytes);
@@ -2494,6 +2637,7 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->padj = (float*)(w + L.off_det + L.det_padj);
     c->pbits = (unsigned*)(w + L.off_det + L.det_pbits);
     c->macc = (unsigned long long*)(w + L.off_det + L.det_macc);
+    c->pmd = (float*)(w + L.off_det + L.det_pmd);
 
     // host tables
     std::vector<int4> chunks;
@@ -2874,9 +3018,10 @@ int mpm_min_dist_bwd(mpm_ctx* ctx, int f, const float* gout, int n_cols, void* s
     hipStream_t st = S_(stream);
     if (ensure_frame(ctx, f, st)) return -1;
     if (ctx->n_prims == 0) return 0;
+    float* gps = ctx->pstate_g + (size_t)f * ctx->n_prims * 8;
     hipLaunchKernelGGL(k_min_dist_bwd, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), ctx->d_prims,
-                       ctx->pstate + (size_t)f * ctx->n_prims * 8, gout, ctx->adj_ptr(f),
-                       ctx->pstate_g + (size_t)f * ctx->n_prims * 8);
+                       ctx->pstate + (size_t)f * ctx->n_prims * 8, gout, ctx->adj_ptr(f), gps, ctx->det ? ctx->pmd : (float*)nullptr);
+    if (ctx->det) hipLaunchKernelGGL(k_min_dist_reduce, dim3(ctx->n_scenes), dim3(64), 0, st, ctx->d_scenes, (const float*)ctx->pmd, gps);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -10,7 +10,7 @@ import torch
 
 from oracle import Oracle, OracleRollout, prim_from_cfg, prim_state_from_cfg, reset_state, sim_from_cfg
 from scenes import prim_state0, state_list
-from test_gpu_collision import SIM, dough, gather_like, oracle_for
+from test_gpu_collision import SIM, dough, gather_like
 
 pytestmark = pytest.mark.gpu
 
@@ -146,4 +146,40 @@ def test_tool_collisions_and_gradmodel_are_bitwise_reproducible():
     assert l0 == l1 == l2
     assert np.array_equal(g0, g1) and np.array_equal(g0, g2)
     lf, gf = _gather_grads(False)
+    assert np.abs(g0 - gf).max() <= 1e-4 * np.abs(gf).max()
+
+
+def _return_dist_grads(det):
+    from diffskill_amd.engine import GradModel, TaichiEnv
+    from test_gpu_prims import scene
+    sim_cfg, prims, st = scene("Gripper")
+    acts = np.random.default_rng(3).uniform(-0.5, 0.5, (2, prims[0]["action"]["dim"]))
+    env = TaichiEnv(dict(SIMULATOR=sim_cfg, PRIMITIVES=prims, SHAPES=[]), init_particles=st[:, :3], deterministic=det)
+    env.initialize()
+    env.simulator.set_state(0, state_list(st, [prim_state0(p) for p in prims]))
+    env.primitives.set_softness(666.0)
+    gm = GradModel(env, softness=666.0, output_grid=(1,), return_dist=True)
+    obs = gm.reset()
+    a = torch.tensor(acts, dtype=torch.float32, device="cuda", requires_grad=True)
+    loss = 0
+    for s in range(2):
+        obs = gm.forward(s, a[s], *obs)
+        wgt = torch.as_tensor(np.random.default_rng(10 + s).normal(size=tuple(obs[0][:, 6:].shape)), dtype=torch.float32,
+                              device="cuda")
+        loss = loss + (wgt * obs[0][:, 6:]).sum()
+    loss = loss + 1e-3 * obs[2].sum()
+    loss.backward()
+    return float(loss.detach()), a.grad.detach().cpu().numpy().copy()
+
+
+def test_distance_observation_adjoint_is_bitwise_reproducible():
+    """GradModel(return_dist=True) with a two-jaw gripper: the distance
+    observations' adjoint into the tool state (per-wave sums added in wave
+    order in the fixed-order mode) gives bitwise-identical gradients run to
+    run, within 1e-4 of the default mode."""
+    print("case determinism: return_dist", flush=True)
+    l0, g0 = _return_dist_grads(True)
+    l1, g1 = _return_dist_grads(True)
+    assert l0 == l1 and np.array_equal(g0, g1)
+    lf, gf = _return_dist_grads(False)
     assert np.abs(g0 - gf).max() <= 1e-4 * np.abs(gf).max()
