@@ -264,7 +264,8 @@ def main():
     ap.add_argument("--keep-blocks", type=int, default=-1,
                     help="node blocks of each substep's grid kept for the adjoint (-1 default, 0 rebuild every grid)")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--deterministic", action="store_true", help="the engine's fixed-order mode (bitwise-reproducible)")
+    ap.add_argument("--unordered", action="store_true",
+                    help="unordered float atomics instead of the engine's default fixed-order (bitwise-reproducible) mode")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--launch", default=None,
                     help="PPI,TILE,GRID: particles per work item and workgroup counts (skips the autotuner)")
@@ -298,7 +299,7 @@ def main():
     H, S = w["H"], w["substeps"]
     T = H * S
     log("workload: %d scenes, %d particles, %d substeps" % (len(w["scenes"]), int(np.sum(w["sizes"])), T))
-    batch = SceneBatch(w["scenes"], T + 1, ckpt_k=args.ckpt_k, keep_blocks=args.keep_blocks, deterministic=args.deterministic)
+    batch = SceneBatch(w["scenes"], T + 1, ckpt_k=args.ckpt_k, keep_blocks=args.keep_blocks, deterministic=not args.unordered)
     targets = build_targets(w)
     init = torch.as_tensor(workloads.action_tensor(w["actions"], H))
     opt = BatchedTrajOpt(batch, H, S, targets, lr=1e-2, init_actions=init, use_graph=not args.no_graph)
@@ -406,7 +407,7 @@ def main():
     if not args.no_plain:
         del opt
         p = PlainTrajOpt(w["scenes"], H, S, targets, lr=1e-2, init_actions=init, ckpt_k=args.ckpt_k, keep_blocks=args.keep_blocks,
-                         deterministic=args.deterministic)
+                         deterministic=not args.unordered)
         p.iteration()
         pm = timed(p.iteration, 1)
         plain = dict(ms_per_step=round(pm, 3), value=n_total * T / (pm / 1000), speedup=round(pm / ms_step, 3))
@@ -456,7 +457,7 @@ def main():
                    "scenes": len(w["scenes"]), "particles": n_total, "substeps": T, "grid": 64,
                    "ckpt_k": args.ckpt_k or "store-all", "keep_blocks": args.keep_blocks,
                    "launch": launch_used, "autotuned": launch is not None, "hip_graph": not args.no_graph,
-                   "fixed_order": bool(args.deterministic)},
+                   "fixed_order": not args.unordered},
         "grad_gate": gate, "grad_checks": grad_checks,
         "plain": plain, "forward": forward, "extra": extra,
         "kernels": kernels, "roofline": roofline, "valu_issue": valu_issue, "cpu_baseline": cpu,

@@ -36,7 +36,7 @@ class SceneBatch:
     each substep's grid kept for the adjoint (-1 default capacity, 0 none).
     """
 
-    def __init__(self, scenes, max_frames, ckpt_k=0, softness=666.0, keep_blocks=-1, deterministic=False):
+    def __init__(self, scenes, max_frames, ckpt_k=0, softness=666.0, keep_blocks=-1, deterministic=True):
         self.specs = scenes
         self.consts = [derive(s["sim"]) for s in scenes]
         self.prims = [_prim_objects(s["prims"], softness) for s in scenes]
@@ -44,8 +44,7 @@ class SceneBatch:
         pdescs = [p.desc() for ps in self.prims for p in ps]
         self.engine = Engine(descs, pdescs, max_frames, ckpt_k, keep_blocks)
         e = self.engine
-        if deterministic:
-            e.set_deterministic(True)
+        e.set_deterministic(bool(deterministic))
         self.n_scenes = len(scenes)
         self.prim_offsets = np.concatenate([[0], np.cumsum([len(p) for p in self.prims])]).astype(int)
         self.n_prims = int(self.prim_offsets[-1])
@@ -228,7 +227,7 @@ class PlainTrajOpt:
     reference drives one Taichi scene per process."""
 
     def __init__(self, scenes, H, substeps, targets, lr=1e-2, init_actions=None, ckpt_k=0, softness=666.0, keep_blocks=-1,
-                 deterministic=False):
+                 deterministic=True):
         self.parts = []
         P0 = 0
         for k, sc in enumerate(scenes):

@@ -225,11 +225,12 @@ constexpr int WI_MIN = 256;  // the work-item capacity of a context is sized for
 constexpr int WI_MAX = 8192;
 constexpr int TILE = 216;    // 6x6x6 nodes
 constexpr int WG = 256;      // threads per work-item workgroup
-constexpr int MAXSEG = 4;    // distinct cells per wave reduced through the transpose tree
+constexpr int MAXSEG = 4;    // distinct cells per wave the mass grid reduces through the transpose tree
 
 // per bin slot counters: CNT_X0 + x counts the work items of XCD x, CNT_XB0 + x
 // is the first entry of its list in xlist
-constexpr int NXCD = 8;      // MI355X: 8 XCDs, workgroup b dispatched to XCD b % 8
+constexpr int NXCD = 8;
+constexpr int DET_GMAX = 16;   // record-array entries per cell (fixed-order ranks)      // MI355X: 8 XCDs, workgroup b dispatched to XCD b % 8
 enum { CNT_NBLK = 1, CNT_PART = 2, CNT_ITEMS = 3, CNT_STAMP = 4, CNT_X0 = 8, CNT_XB0 = 16 };
 // global binning state: the pass stamp base and the occupied-block count of
 // the pass in flight (both advanced/reset by the pass's last kernel)
@@ -255,12 +256,19 @@ struct BinArgs {
     int4* items;
     int* order;
     int wi;          // particles per work item
-    // fixed-order mode: each (cell, wave) group of a pass records its count
-    // and the previous group of its cell (a list per cell whose head carries
-    // the pass stamp), so a particle's rank is its rank within its wave plus
-    // the counts of its cell's groups in lower waves -- slot order, not the
-    // order in which the groups' atomics landed
+    // fixed-order mode: each (cell, wave) group of a pass records its wave and
+    // count, so a particle's rank is its rank within its wave plus the counts
+    // of its cell's groups in lower waves -- slot order, not the order in which
+    // the groups' atomics landed.  A group claims its index among its cell's
+    // groups of the pass with an atomic on cell_gcnt (zero between passes:
+    // k_blocks moves each occupied cell's total to cell_ng and clears it); the
+    // first DET_GMAX groups take the entries of the cell's record array
+    // (cell_grp: (wave << 7) | count), the rest are listed (cell_head: pass
+    // stamp and newest group, grp_next / grp_cnt per group leader's slot)
     int det;
+    int* cell_gcnt;
+    int* cell_ng;
+    unsigned* cell_grp;
     unsigned long long* cell_head;
     int* grp_next;
     int* grp_cnt;
@@ -333,10 +341,15 @@ __global__ void __launch_bounds__(PPB) k_bin(BinArgs a, const float* __restrict_
     if (valid && lane == leader) {
         if (a.det) {
             atomicAdd(&a.cell_count[q], cnt);
-            unsigned long long mine = ((unsigned long long)(unsigned)stamp << 32) | (unsigned)slot;
-            unsigned long long old = atomicExch(&a.cell_head[q], mine);
-            a.grp_next[slot] = ((int)(old >> 32) == stamp) ? (int)(unsigned)(old & 0xffffffffull) : -1;
-            a.grp_cnt[slot] = cnt;
+            int gi = atomicAdd(&a.cell_gcnt[q], 1);   // this group's index among the cell's groups of the pass
+            if (gi < DET_GMAX) {
+                a.cell_grp[(size_t)q * DET_GMAX + gi] = ((unsigned)(slot >> 6) << 7) | (unsigned)cnt;
+            } else {
+                unsigned long long mine = ((unsigned long long)(unsigned)stamp << 32) | (unsigned)slot;
+                unsigned long long old = atomicExch(&a.cell_head[q], mine);
+                a.grp_next[slot] = ((int)(old >> 32) == stamp) ? (int)(unsigned)(old & 0xffffffffull) : -1;
+                a.grp_cnt[slot] = cnt;
+            }
         } else {
             base = atomicAdd(&a.cell_count[q], cnt);
         }
@@ -392,6 +405,10 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nbl
         int gb = has ? a.pblocks[e] : 0;
         int cnt = has ? a.cell_count[gb * 64 + lane] : 0;
         if (has) a.cell_count[gb * 64 + lane] = 0;
+        if (has && a.det) {
+            a.cell_ng[gb * 64 + lane] = a.cell_gcnt[gb * 64 + lane];
+            a.cell_gcnt[gb * 64 + lane] = 0;
+        }
         int incl = cnt;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -547,13 +564,28 @@ __global__ void __launch_bounds__(PPB) k_scatter(BinArgs a, int* __restrict__ nr
     int q = a.bin_q[slot];
     int r = a.bin_rank[slot];
     if (a.det) {
-        // the cell's groups of this pass, newest first; those of lower waves
-        // (slots below this wave's) precede this particle
-        int rec = (int)(unsigned)(a.cell_head[q] & 0xffffffffull);
-        int w = slot >> 6;
-        while (rec >= 0) {
-            if ((rec >> 6) < w) r += a.grp_cnt[rec];
-            rec = a.grp_next[rec];
+        // the cell's groups of this pass in lower waves (slots below this
+        // wave's) precede this particle: the record array, then the list of
+        // the groups past DET_GMAX, newest first
+        int ng = a.cell_ng[q];
+        unsigned w = (unsigned)(slot >> 6);
+        const uint4* ra = reinterpret_cast<const uint4*>(a.cell_grp + (size_t)q * DET_GMAX);
+#pragma unroll
+        for (int k = 0; k < DET_GMAX / 4; k++) {
+            if (4 * k < ng) {
+                uint4 v = ra[k];
+                unsigned e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (4 * k + j < ng && (e[j] >> 7) < w) r += (int)(e[j] & 127u);
+            }
+        }
+        if (ng > DET_GMAX) {
+            int rec = (int)(unsigned)(a.cell_head[q] & 0xffffffffull);
+            while (rec >= 0) {
+                if ((unsigned)(rec >> 6) < w) r += a.grp_cnt[rec];
+                rec = a.grp_next[rec];
+            }
         }
     }
     a.order[a.cell_start[q] + r] = slot;
@@ -568,6 +600,12 @@ __global__ void __launch_bounds__(PPB) k_scatter(BinArgs a, int* __restrict__ nr
 // of that half; afterwards lanes 2j and 2j+1 hold the total of value j.
 DEV float f_of(unsigned u) { return __uint_as_float(u); }
 DEV unsigned u_of(float f) { return __float_as_uint(f); }
+// a lane's value read from the lane that the DPP control selects (every source
+// lane valid for the controls used here); written as update_dpp with a zero
+// old value and bound_ctrl so that the compiler folds it into the consuming
+// add (v_add_f32_dpp) instead of issuing a separate move
+template <int DPP>
+DEV float dpp_get(unsigned u) { return f_of(__builtin_amdgcn_update_dpp(0u, u, DPP, 0xf, 0xf, true)); }
 template <int H>
 DEV void tstep_swap32(float* r) {
 #pragma unroll
@@ -592,7 +630,7 @@ DEV void tstep_dpp(float* r, int lane) {
         float lo = r[j], hi = r[j + H];
         float send = up ? lo : hi;
         float keep = up ? hi : lo;
-        r[j] = keep + f_of(__builtin_amdgcn_mov_dpp(u_of(send), DPP, 0xf, 0xf, false));
+        r[j] = keep + dpp_get<DPP>(u_of(send));
     }
 }
 // The mirrors pair lane i with i ^ 15 (resp. i ^ 7): the partner has the opposite
@@ -608,7 +646,7 @@ DEV float tsum32(float* r, int lane) {
     tstep_dpp<4, 8, DPP_ROW_MIRROR>(r, lane);
     tstep_dpp<2, 4, DPP_ROW_HALF_MIRROR>(r, lane);
     tstep_dpp<1, 2, DPP_QUAD_XOR2>(r, lane);
-    return r[0] + f_of(__builtin_amdgcn_mov_dpp(u_of(r[0]), DPP_QUAD_XOR1, 0xf, 0xf, false));
+    return r[0] + dpp_get<DPP_QUAD_XOR1>(u_of(r[0]));
 }
 // 16 values: four halving steps (bits 5..2), then the quad sum; afterwards
 // every lane of quad j holds the total of value j
@@ -617,38 +655,25 @@ DEV float tsum16(float* r, int lane) {
     tstep_swap16<4>(r);
     tstep_dpp<2, 8, DPP_ROW_MIRROR>(r, lane);
     tstep_dpp<1, 4, DPP_ROW_HALF_MIRROR>(r, lane);
-    float s = r[0] + f_of(__builtin_amdgcn_mov_dpp(u_of(r[0]), DPP_QUAD_XOR2, 0xf, 0xf, false));
-    return s + f_of(__builtin_amdgcn_mov_dpp(u_of(s), DPP_QUAD_XOR1, 0xf, 0xf, false));
+    float s = r[0] + dpp_get<DPP_QUAD_XOR2>(u_of(r[0]));
+    return s + dpp_get<DPP_QUAD_XOR1>(u_of(s));
 }
 DEV int tile_index(int l0, int l1, int l2, int n) { return ((l0 + n / 9) * 6 + (l1 + (n / 3) % 3)) * 6 + (l2 + n % 3); }
 
-// Segmented scatter of 27 x NCOMP per-particle node values into an LDS tile
-// (component-major, TILE floats per component).  Lanes that share a cell are
-// summed through the transpose tree (at most MAXSEG distinct cells per wave),
-// the rest add directly.  CHAN(c, r) writes the lane's 27 node values of
+// Segmented scatter of 27 x NCOMP per-particle node values into the calling
+// wave's own LDS tile (component-major, TILE floats per component): the lanes
+// of each distinct cell of the wave, in order of their first lane, are summed
+// through the transpose tree and the 27 node sums added into the tile.  Every
+// cell takes the tree, however many a wave holds (adding the lanes' own values
+// with LDS atomics past a few cells measured slower, and its order within an
+// instruction is not fixed).  CHAN(c, r) writes the lane's 27 node values of
 // component c into r[0..26] (node n = (a * 3 + b) * 3 + cc); one transpose
 // pass per component, so a caller forms a pass's 27 values incrementally
 // along the stencil.
 template <int NCOMP, class ChanFn>
-DEV void scatter_tile_ch(float* tile, bool valid, int key, int l0, int l1, int l2, int lane, bool det, ChanFn chan) {
+DEV void scatter_tile_ch(float* tile, bool valid, int key, int l0, int l1, int l2, int lane, ChanFn chan) {
     unsigned long long act = __ballot(valid);
-    int seg = 0;
     while (act) {
-        // past MAXSEG distinct cells the remaining lanes add their own values
-        // (lanes of one instruction may meet at a node); the fixed-order mode
-        // takes every cell through the tree instead
-        if (seg == MAXSEG && !det) {
-            if ((act >> lane) & 1ull) {
-#pragma unroll
-                for (int c = 0; c < NCOMP; c++) {
-                    float r[27];
-                    chan(c, r);
-#pragma unroll
-                    for (int n = 0; n < 27; n++) atomicAdd(&tile[c * TILE + tile_index(l0, l1, l2, n)], r[n]);
-                }
-            }
-            break;
-        }
         int lead = __ffsll((long long)act) - 1;
         int key0 = __shfl(key, lead);
         unsigned long long m = __ballot(valid && key == key0);
@@ -665,7 +690,6 @@ DEV void scatter_tile_ch(float* tile, bool valid, int key, int l0, int l1, int l
             if (!(lane & 1) && n < 27) atomicAdd(&tile[c * TILE + tile_index(s0, s1, s2, n)], s);
         }
         act &= ~m;
-        seg++;
     }
 }
 
@@ -681,30 +705,29 @@ struct TileArgs {
     const float4* keep_v;
     const int* nmap;
     int keep_cap;
-    // fixed-order mode: every wave scatters into a tile of its own, and the
-    // workgroup's tiles are summed in wave order
-    int det;
     const int* xlist;      // work items grouped by XCD (see BinArgs)
 };
 
 // The work items a workgroup of a tile kernel takes: those of its own XCD's
 // list (workgroup b runs on XCD b % NXCD), entry j going to that XCD's
 // workgroup j mod (its workgroup count); with fewer workgroups than XCDs,
-// every item in turn.
+// every item in turn.  The first `lead` workgroups of the launch (a multiple
+// of NXCD) do other work and take no items.
 struct ItemIter { const int* list; int j, step, n; };
-DEV ItemIter item_iter(const TileArgs& ta) {
+DEV ItemIter item_iter(const TileArgs& ta, int lead = 0) {
     ItemIter r;
-    if ((int)gridDim.x >= NXCD) {
-        int x = blockIdx.x % NXCD;
+    int b = (int)blockIdx.x - lead, nb = (int)gridDim.x - lead;
+    if (nb >= NXCD) {
+        int x = b % NXCD;
         r.list = ta.xlist + ta.counters[CNT_XB0 + x];
         r.n = ta.counters[CNT_X0 + x];
-        r.j = blockIdx.x / NXCD;
-        r.step = ((int)gridDim.x - x + NXCD - 1) / NXCD;
+        r.j = b / NXCD;
+        r.step = (nb - x + NXCD - 1) / NXCD;
     } else {
         r.list = nullptr;
         r.n = ta.counters[CNT_ITEMS];
-        r.j = blockIdx.x;
-        r.step = gridDim.x;
+        r.j = b;
+        r.step = nb;
     }
     return r;
 }
@@ -1004,9 +1027,10 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
     bool kept = use_kept && kept_fits(ta.counters, ta.keep_cap);
     int lane = threadIdx.x & 63;
     int stride = ta.stride;
-    bool det = ta.det != 0;
-    int n_acc = det ? (WG / 64) * 3 * TILE : 3 * TILE;
-    float* wacc = det ? acc + (threadIdx.x >> 6) * 3 * TILE : acc;
+    // every wave scatters into a tile of its own (no LDS atomics between
+    // waves), the workgroup's tiles are summed in wave order at the flush
+    int n_acc = (WG / 64) * 3 * TILE;
+    float* wacc = acc + (threadIdx.x >> 6) * 3 * TILE;
     for (int j = ir.j; j < ir.n; j += ir.step) {
         int e = ir.list ? ir.list[j] : j;
         int4 it = ta.items[e];
@@ -1118,17 +1142,15 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
                     ua += k0;
                 }
             };
-            scatter_tile_ch<3>(wacc, valid && in_tile, key, l0, l1, l2, lane, det, chan);
+            scatter_tile_ch<3>(wacc, valid && in_tile, key, l0, l1, l2, lane, chan);
         }
         __syncthreads();
         for (int k = threadIdx.x; k < TILE; k += WG) {
             float4 r = make_float4(acc[k], acc[TILE + k], acc[2 * TILE + k], 0.f);
-            if (det) {
 #pragma unroll
-                for (int w = 1; w < WG / 64; w++) {
-                    const float* t = acc + w * 3 * TILE;
-                    r.x += t[k]; r.y += t[TILE + k]; r.z += t[2 * TILE + k];
-                }
+            for (int w = 1; w < WG / 64; w++) {
+                const float* t = acc + w * 3 * TILE;
+                r.x += t[k]; r.y += t[TILE + k]; r.z += t[2 * TILE + k];
             }
             tiles_v[(size_t)e * TILE + k] = r;
         }
@@ -1210,6 +1232,7 @@ __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* 
         // collision units in reverse order; each unit's primitive adjoint is wave-reduced
         v3 x = mk((float)I0 * S.dx, (float)I1 * S.dx, (float)I2 * S.dx);
         const int blk_gb = m.gb;
+        bool blk_any = false;
         int uidx = 0;
         for (int q = 0; q < S.n_prims; q++) { Unit u[2]; uidx += prim_units(ga.prims[S.prim_off + q], u); }
         for (int q = S.n_prims - 1; q >= 0; q--) {
@@ -1231,6 +1254,7 @@ __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* 
                     if (ga.det && (lane & 3) == 0) ga.padj[((size_t)blk_gb * MAX_UNITS + uidx) * 16 + k] = 0.f;
                     continue;
                 }
+                blk_any = true;
                 // the 16 wave sums at once; quad k adds value k
                 float s = tsum16(vals, lane);
                 if (ga.det) {
@@ -1240,7 +1264,8 @@ __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* 
                 }
             }
         }
-        if (ga.det && lane == 0 && S.n_prims > 0) {
+        // (blocks whose units all have zero adjoint add nothing and are not flagged)
+        if (ga.det && lane == 0 && blk_any) {
             int lb = blk_gb - S.blk_off;
             atomicOr(&ga.pbits[S.bm_off + (lb >> 5)], 1u << (lb & 31));
         }
@@ -1319,9 +1344,10 @@ DEV void p2g_scatter_body(float* tile, const TileArgs& ta, const float* __restri
     ItemIter ir = item_iter(ta);
     int lane = threadIdx.x & 63;
     int stride = ta.stride;
-    bool det = ta.det != 0;
-    int n_tile = det ? (WG / 64) * 4 * TILE : 4 * TILE;
-    float* wtile = det ? tile + (threadIdx.x >> 6) * 4 * TILE : tile;
+    // every wave scatters into a tile of its own (no LDS atomics between
+    // waves), the workgroup's tiles are summed in wave order at the flush
+    int n_tile = (WG / 64) * 4 * TILE;
+    float* wtile = tile + (threadIdx.x >> 6) * 4 * TILE;
     for (int j = ir.j; j < ir.n; j += ir.step) {
         int e = ir.list ? ir.list[j] : j;
         int4 it = ta.items[e];
@@ -1368,17 +1394,15 @@ DEV void p2g_scatter_body(float* tile, const TileArgs& ta, const float* __restri
                     ua += k0;
                 }
             };
-            scatter_tile_ch<4>(wtile, valid && in_tile, key, l0, l1, l2, lane, det, chan);
+            scatter_tile_ch<4>(wtile, valid && in_tile, key, l0, l1, l2, lane, chan);
         }
         __syncthreads();
         for (int k = threadIdx.x; k < TILE; k += WG) {
             float4 r = make_float4(tile[k], tile[TILE + k], tile[2 * TILE + k], tile[3 * TILE + k]);
-            if (det) {
 #pragma unroll
-                for (int w = 1; w < WG / 64; w++) {
-                    const float* t = tile + w * 4 * TILE;
-                    r.x += t[k]; r.y += t[TILE + k]; r.z += t[2 * TILE + k]; r.w += t[3 * TILE + k];
-                }
+            for (int w = 1; w < WG / 64; w++) {
+                const float* t = tile + w * 4 * TILE;
+                r.x += t[k]; r.y += t[TILE + k]; r.z += t[2 * TILE + k]; r.w += t[3 * TILE + k];
             }
             tiles[(size_t)e * TILE + k] = r;
         }
@@ -1399,14 +1423,99 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
     p2g_scatter_body(tile, ta, st, aff, tiles, only_overflow);
 }
 
+// Fixed-order mode: the primitive adjoints of substep f from the grid
+// adjoint's per-block partials (written for the node blocks whose bit the grid
+// adjoint set: those where some collision unit's adjoint is nonzero).
+struct PrimRedArgs {
+    const SceneC* scenes;
+    const PrimC* prims;
+    const float* padj;
+    unsigned* pbits;
+    float* gpstate;
+    int n_prims_total, f, n_scenes;
+};
+constexpr int PR_LIST = 64 * 32;   // flagged node blocks listed per round (64 bitmap words)
+
+// Scene sc, by the first wave of the calling workgroup (every thread of the
+// workgroup calls this).  In rounds of 64 bitmap words, the flagged blocks
+// are listed in block order (the words' popcounts scanned over the wave),
+// lane l sums the l-th 64th of the list in list order, the lanes' sums are
+// combined by the fixed butterfly of tsum16 and added to the running totals
+// in round order; then each primitive's units are added in unit order into the
+// adjoint of frames f (values 0-7) and f+1 (8-15).  The order depends only on
+// which blocks are flagged, never on timing or launch shape.  The bitmap
+// words are cleared as they are read.
+DEV void prim_adj_reduce_scene(const PrimRedArgs& pr, int sc, int* lst, float (*tot)[16]) {
+    const SceneC& S = pr.scenes[sc];
+    int lane = threadIdx.x & 63;
+    bool wk = threadIdx.x < 64;
+    int nbk = S.nb * S.nb * S.nb;
+    int nw = (nbk + 31) >> 5;
+    int n_units = 0;
+    for (int q = 0; q < S.n_prims; q++) n_units += pr.prims[S.prim_off + q].n_units;
+    for (int w0 = 0; w0 < nw; w0 += 64) {
+        int wi = w0 + lane;
+        unsigned wd = (wk && wi < nw) ? pr.pbits[S.bm_off + wi] : 0u;
+        int c = __popc(wd), incl = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            int u = __shfl_up(incl, d);
+            if (lane >= d) incl += u;
+        }
+        int n = __shfl(incl, 63);
+        if (wd) {
+            int o = incl - c;
+            for (unsigned w = wd; w; w &= w - 1) lst[o++] = wi * 32 + __ffs(w) - 1;
+            pr.pbits[S.bm_off + wi] = 0u;
+        }
+        __syncthreads();
+        int i0 = (n * lane) >> 6, i1 = wk ? (n * (lane + 1)) >> 6 : i0;
+        for (int u = 0; u < n_units; u++) {
+            float a[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) a[k] = 0.f;
+            for (int i = i0; i < i1; i++) {
+                const float4* r = reinterpret_cast<const float4*>(pr.padj + ((size_t)(S.blk_off + lst[i]) * MAX_UNITS + u) * 16);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    float4 v = r[j];
+                    a[4 * j] += v.x; a[4 * j + 1] += v.y; a[4 * j + 2] += v.z; a[4 * j + 3] += v.w;
+                }
+            }
+            // quad k of the wave holds the total of value k
+            float t = tsum16(a, lane);
+            if (wk && (lane & 3) == 0) tot[u][lane >> 2] = (w0 == 0 ? 0.f : tot[u][lane >> 2]) + t;
+        }
+        __syncthreads();
+    }
+    if (wk && lane < 16) {
+        int u = 0;
+        for (int q = 0; q < S.n_prims; q++) {
+            int nu = pr.prims[S.prim_off + q].n_units;
+            float v = 0.f;
+            for (int j = 0; j < nu; j++, u++) v += tot[u][lane];
+            int fr = lane < 8 ? pr.f : pr.f + 1;
+            pr.gpstate[((size_t)fr * pr.n_prims_total + S.prim_off + q) * 8 + (lane & 7)] += v;
+        }
+    }
+}
+
 // first half of the p2g adjoint over work items: gathers the node adjoints of
 // (momentum, mass) from the block's LDS tile and produces the adjoints of v,
 // of the affine matrix (to gaff) and the weight/position part of x.
 __global__ void __launch_bounds__(WG) k_p2g_bwd_gather(TileArgs ta, const float* __restrict__ st, const float* __restrict__ aff,
                                                        float* __restrict__ a0, float* __restrict__ gaff_out,
-                                                       const float4* __restrict__ g_adjpm) {
+                                                       const float4* __restrict__ g_adjpm, PrimRedArgs pr, int lead) {
     __shared__ float4 tg[TILE];
-    ItemIter ir = item_iter(ta);
+    __shared__ int lst[PR_LIST];
+    __shared__ float tot[MAX_UNITS][16];
+    // the fixed-order mode's primitive-adjoint reduction of this substep, one
+    // leading workgroup per scene, beside the work items
+    if ((int)blockIdx.x < lead) {
+        if ((int)blockIdx.x < pr.n_scenes && pr.scenes[blockIdx.x].n_prims > 0) prim_adj_reduce_scene(pr, blockIdx.x, lst, tot);
+        return;
+    }
+    ItemIter ir = item_iter(ta, lead);
     int stride = ta.stride;
     for (int j = ir.j; j < ir.n; j += ir.step) {
         int e = ir.list ? ir.list[j] : j;
@@ -1605,57 +1714,14 @@ __global__ void __launch_bounds__(PPB) k_stress_bwd(ChunkArgs ca, const float* _
     store9(a0, stride, 15, p, gF0);
 }
 
-// Fixed-order mode: the primitive adjoints of substep f from the grid
-// adjoint's per-block partials, one wave per scene.  Lane l sums the flagged
-// blocks of its contiguous range of bitmap words in block order, the lanes'
-// sums are combined by the fixed butterfly of tsum16, then each primitive's
-// units are added in unit order into the adjoint of frames f (values 0-7)
-// and f+1 (8-15).  The bitmap words are cleared as they are read.
-__global__ void __launch_bounds__(64) k_prim_adj_reduce(const SceneC* __restrict__ scenes, const PrimC* __restrict__ prims,
-                                                        const float* __restrict__ padj, unsigned* __restrict__ pbits,
-                                                        float* __restrict__ gpstate, int n_prims_total, int f) {
-    const SceneC& S = scenes[blockIdx.x];
-    if (S.n_prims == 0) return;
+// one workgroup per scene; used where the reduction must finish before the
+// tool-tool collision adjoint (otherwise the p2g adjoint's leading workgroups
+// run it beside their own work)
+__global__ void __launch_bounds__(64) k_prim_adj_reduce(PrimRedArgs pr) {
+    __shared__ int lst[PR_LIST];
     __shared__ float tot[MAX_UNITS][16];
-    int lane = threadIdx.x;
-    int nbk = S.nb * S.nb * S.nb;
-    int nw = (nbk + 31) >> 5;
-    int w0 = (int)(((long long)nw * lane) >> 6), w1 = (int)(((long long)nw * (lane + 1)) >> 6);
-    int n_units = 0;
-    for (int q = 0; q < S.n_prims; q++) n_units += prims[S.prim_off + q].n_units;
-    for (int u = 0; u < n_units; u++) {
-        float a[16];
-#pragma unroll
-        for (int k = 0; k < 16; k++) a[k] = 0.f;
-        for (int i = w0; i < w1; i++) {
-            unsigned wd = pbits[S.bm_off + i];
-            while (wd) {
-                int b = __ffs(wd) - 1;
-                wd &= wd - 1;
-                const float4* r = reinterpret_cast<const float4*>(padj + ((size_t)(S.blk_off + i * 32 + b) * MAX_UNITS + u) * 16);
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    float4 v = r[j];
-                    a[4 * j] += v.x; a[4 * j + 1] += v.y; a[4 * j + 2] += v.z; a[4 * j + 3] += v.w;
-                }
-            }
-        }
-        // quad k of the wave holds the total of value k
-        float t = tsum16(a, lane);
-        if ((lane & 3) == 0) tot[u][lane >> 2] = t;
-    }
-    __syncthreads();
-    for (int i = w0; i < w1; i++) pbits[S.bm_off + i] = 0u;
-    if (lane < 16) {
-        int u = 0;
-        for (int q = 0; q < S.n_prims; q++) {
-            int nu = prims[S.prim_off + q].n_units;
-            float v = 0.f;
-            for (int j = 0; j < nu; j++, u++) v += tot[u][lane];
-            int fr = lane < 8 ? f : f + 1;
-            gpstate[((size_t)fr * n_prims_total + S.prim_off + q) * 8 + (lane & 7)] += v;
-        }
-    }
+    if (pr.scenes[blockIdx.x].n_prims == 0) return;
+    prim_adj_reduce_scene(pr, blockIdx.x, lst, tot);
 }
 
 // Fixed-order mode of compute_grid_m_kernel: the wave-reduced node masses are
@@ -2124,9 +2190,12 @@ struct mpm_ctx {
     int seg_hi = -1;      // highest resident ring frame of that segment
     int adj_top = -1;     // frame whose adjoint is current
     int grid_ctas = 1024; // persistent grid-phase workgroups
-    // fixed-order mode (mpm_set_deterministic): slot-ordered cell ranks,
-    // per-wave tiles, block-ordered primitive adjoints, integer mass grid
-    bool det = false;
+    // fixed-order mode (mpm_set_deterministic, on by default): slot-ordered
+    // cell ranks, block-ordered primitive adjoints, wave-ordered distance
+    // adjoints, integer mass grid
+    bool det = true;
+    int *cell_gcnt = nullptr, *cell_ng = nullptr;
+    unsigned* cell_grp = nullptr;
     unsigned long long* cell_head = nullptr;
     int *grp_next = nullptr, *grp_cnt = nullptr;
     float* padj = nullptr;
@@ -2171,6 +2240,7 @@ struct mpm_ctx {
         a.blk_stamp = b.blk_stamp; a.blk_item0 = b.blk_item0; a.blk_nitems = b.blk_nitems; a.items = b.items; a.order = b.order;
         a.wi = wi;
         a.det = det ? 1 : 0;
+        a.cell_gcnt = cell_gcnt; a.cell_ng = cell_ng; a.cell_grp = cell_grp;
         a.cell_head = cell_head; a.grp_next = grp_next; a.grp_cnt = grp_cnt;
         a.blk_rep = blk_rep; a.xlist = b.xlist;
         return a;
@@ -2180,7 +2250,6 @@ struct mpm_ctx {
         TileArgs t;
         t.scenes = d_scenes; t.blk_scene = d_blk_scene; t.items = b.items; t.counters = b.counters; t.order = b.order; t.stride = stride;
         t.keep_v = b.keep_v; t.nmap = b.nmap; t.keep_cap = keep_cap;
-        t.det = det ? 1 : 0;
         t.xlist = b.xlist;
         return t;
     }
@@ -2228,7 +2297,7 @@ struct Layout {
         off_pints, off_bslots, off_keep, off_tiles, off_counters, off_prim_bufs, off_det, total;
     // fixed-order mode buffers (within off_det): cell list heads, group
     // records, per-block primitive adjoints, block bitmaps, integer mass grid
-    size_t det_head, det_grp, det_padj, det_pbits, det_macc, det_pmd;
+    size_t det_gn, det_garr, det_head, det_grp, det_padj, det_pbits, det_macc, det_pmd;
     int n_bm_words;
     long long n_mass;
     // one bin slot: order, items, blk_stamp, blk_item0, blk_nitems, nblocks, nmap, counters
@@ -2317,6 +2386,8 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     L.n_mass = std::max(1LL, mass);
     L.off_det = o;
     size_t d = 0;
+    L.det_gn = d; d = align256(d + sizeof(int) * 2 * (size_t)L.n_blocks * 64);
+    L.det_garr = d; d = align256(d + sizeof(unsigned) * DET_GMAX * (size_t)L.n_blocks * 64);
     L.det_head = d; d = align256(d + sizeof(unsigned long long) * (size_t)L.n_blocks * 64);
     L.det_grp = d; d = align256(d + sizeof(int) * (size_t)stride * 2);
     L.det_padj = d; d = align256(d + sizeof(float) * 16 * MAX_UNITS * (size_t)L.n_blocks);
@@ -2466,10 +2537,18 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     { PhaseTimer t(c, MPM_PH_GRID_BWD, st);
       hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->g_pm, c->tiles_v, c->g_adjpm, c->pstate_g,
                          use_kept); }
+    // fixed-order mode: the primitive adjoints are reduced by their own launch
+    // when the collision adjoint needs them, else by the p2g adjoint's leading
+    // workgroups (a multiple of NXCD, so the item workgroups keep their XCDs)
+    PrimRedArgs pr{c->d_scenes, c->d_prims, (const float*)c->padj, c->pbits, c->pstate_g, c->n_prims, f, c->n_scenes};
+    int pr_lead = 0;
     if (c->det && c->n_prims > 0) {
-        PhaseTimer t(c, MPM_PH_PRIM_REDUCE, st);
-        hipLaunchKernelGGL(k_prim_adj_reduce, dim3(c->n_scenes), dim3(64), 0, st, c->d_scenes, c->d_prims, (const float*)c->padj,
-                           c->pbits, c->pstate_g, c->n_prims, f);
+        if (c->n_pairs > 0) {
+            PhaseTimer t(c, MPM_PH_PRIM_REDUCE, st);
+            hipLaunchKernelGGL(k_prim_adj_reduce, dim3(c->n_scenes), dim3(64), 0, st, pr);
+        } else {
+            pr_lead = (c->n_scenes + NXCD - 1) / NXCD * NXCD;
+        }
     }
     if (c->n_pairs > 0) {
         // between the grid adjoint (which adds into frame f+1's primitive
@@ -2479,7 +2558,8 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
                            c->n_prims, f + 1);
     }
     { PhaseTimer t(c, MPM_PH_P2G_BWD, st);
-      hipLaunchKernelGGL(k_p2g_bwd_gather, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, a0, c->gaff, c->g_adjpm); }
+      hipLaunchKernelGGL(k_p2g_bwd_gather, dim3(c->tile_ctas + pr_lead), dim3(WG), 0, st, ta, in, (const float*)aff, a0, c->gaff,
+                         c->g_adjpm, pr, pr_lead); }
     { PhaseTimer t(c, MPM_PH_STRESS_BWD, st);
       hipLaunchKernelGGL(k_stress_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, (const float*)c->gaff, a1, a0,
                          (const float*)c->usv_ptr(f), c->d_prims, c->n_prims, c->pstate, c->pvel, c->pstate_g, c->pvel_g, f); }
@@ -2631,6 +2711,9 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->pstate_g = c->pstate + pb;
     c->pvel = c->pstate + 2 * pb;
     c->pvel_g = c->pstate + 3 * pb;
+    c->cell_gcnt = (int*)(w + L.off_det + L.det_gn);
+    c->cell_ng = c->cell_gcnt + (size_t)L.n_blocks * 64;
+    c->cell_grp = (unsigned*)(w + L.off_det + L.det_garr);
     c->cell_head = (unsigned long long*)(w + L.off_det + L.det_head);
     c->grp_next = (int*)(w + L.off_det + L.det_grp);
     c->grp_cnt = c->grp_next + L.stride;

@@ -138,7 +138,7 @@ class _GridGrad:
 
 
 class MPMSimulator:
-    def __init__(self, cfg, primitives=(), ckpt_k=None, keep_blocks=-1, deterministic=False):
+    def __init__(self, cfg, primitives=(), ckpt_k=None, keep_blocks=-1, deterministic=True):
         c = derive(cfg)
         self.cfg = cfg
         self.consts = c
@@ -165,7 +165,8 @@ class MPMSimulator:
             ckpt_k = int(cfg.get("ckpt_k", 0) or 0) if hasattr(cfg, "get") else 0
         self.ckpt_k = ckpt_k
         self.keep_blocks = keep_blocks
-        # fixed-order execution of the engine (bitwise-reproducible results)
+        # fixed-order execution of the engine (bitwise-reproducible results;
+        # False: unordered float atomics)
         self.deterministic = bool(deterministic)
         self.grid_m = _GridField(self)
         self.x = _ParticleField(self, "x")
@@ -213,8 +214,7 @@ class MPMSimulator:
         self.engine = Engine([scene_desc(self.consts, n, self.n_primitive)], self._prim_descs(), self.max_steps, self.ckpt_k,
                              self.keep_blocks)
         self._ctx = self.engine.ctx
-        if self.deterministic:
-            self.engine.set_deterministic(True)
+        self.engine.set_deterministic(self.deterministic)
         if keep is not None:
             self.engine.prim_state[0, :self.n_primitive] = keep
         for j, p in enumerate(self.primitives):

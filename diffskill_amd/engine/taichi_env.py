@@ -16,7 +16,7 @@ from .shapes import Shapes
 
 
 class TaichiEnv:
-    def __init__(self, cfg, nn=False, loss=True, return_dist=False, init_particles=None, ckpt_k=None, deterministic=False):
+    def __init__(self, cfg, nn=False, loss=True, return_dist=False, init_particles=None, ckpt_k=None, deterministic=True):
         self.return_dist = bool(return_dist)
         full = get_cfg_defaults()
         full.merge(cfg)
