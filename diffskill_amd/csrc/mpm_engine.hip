@@ -230,7 +230,8 @@ constexpr int MAXSEG = 4;    // distinct cells per wave the mass grid reduces th
 // per bin slot counters: CNT_X0 + x counts the work items of XCD x, CNT_XB0 + x
 // is the first entry of its list in xlist
 constexpr int NXCD = 8;
-constexpr int DET_GMAX = 16;   // record-array entries per cell (fixed-order ranks)      // MI355X: 8 XCDs, workgroup b dispatched to XCD b % 8
+constexpr int DET_GMAX = 16;
+constexpr int FB_LEAD = 64;    // leading workgroups that carry an overflow fallback   // record-array entries per cell (fixed-order ranks)      // MI355X: 8 XCDs, workgroup b dispatched to XCD b % 8
 enum { CNT_NBLK = 1, CNT_PART = 2, CNT_ITEMS = 3, CNT_STAMP = 4, CNT_X0 = 8, CNT_XB0 = 16 };
 // global binning state: the pass stamp base and the occupied-block count of
 // the pass in flight (both advanced/reset by the pass's last kernel)
@@ -711,12 +712,13 @@ struct TileArgs {
 // The work items a workgroup of a tile kernel takes: those of its own XCD's
 // list (workgroup b runs on XCD b % NXCD), entry j going to that XCD's
 // workgroup j mod (its workgroup count); with fewer workgroups than XCDs,
-// every item in turn.  The first `lead` workgroups of the launch (a multiple
-// of NXCD) do other work and take no items.
+// every item in turn.  The item workgroups are the launch's workgroups
+// [lead, lead + nwg) (lead a multiple of NXCD; nwg <= 0: the rest of the
+// launch); the others do other work.
 struct ItemIter { const int* list; int j, step, n; };
-DEV ItemIter item_iter(const TileArgs& ta, int lead = 0) {
+DEV ItemIter item_iter(const TileArgs& ta, int lead = 0, int nwg = 0) {
     ItemIter r;
-    int b = (int)blockIdx.x - lead, nb = (int)gridDim.x - lead;
+    int b = (int)blockIdx.x - lead, nb = nwg > 0 ? nwg : (int)gridDim.x - lead;
     if (nb >= NXCD) {
         int x = b % NXCD;
         r.list = ta.xlist + ta.counters[CNT_XB0 + x];
@@ -857,10 +859,11 @@ DEV float4 gather_tiles(const BlockMeta& m, int lane, const float4* __restrict__
 // With only_overflow set it runs only when the kept grid overflowed (the
 // adjoint then recomputes the grid instead of reading the kept one).
 DEV void grid_op_body(const GridArgs& ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm, float4* __restrict__ g_v,
-                      int only_overflow) {
+                      int only_overflow, int lead = 0, int nwg = 0) {
     int lane = threadIdx.x & 63;
     int wave = threadIdx.x >> 6;
-    int e = uniform(blockIdx.x * 4 + wave);
+    int nb = nwg > 0 ? nwg : (int)gridDim.x - lead;
+    int e = uniform(((int)blockIdx.x - lead) * 4 + wave);
     // the first block's record is read together with the counters (a record
     // beyond the active count is stale and left unused)
     BlockMeta m;
@@ -869,7 +872,7 @@ DEV void grid_op_body(const GridArgs& ga, const float4* __restrict__ tiles, floa
     if (only_overflow && kept_fits(ga.counters, ga.keep_cap)) return;
     int n_active = ga.counters[CNT_NBLK];
     int stamp = ga.counters[CNT_STAMP];
-    for (; e < n_active; e += gridDim.x * 4) {
+    for (; e < n_active; e += nb * 4) {
         if (!have) block_meta(ga, e, stamp, m);
         have = false;
         const SceneC& S = ga.scenes[m.sc];
@@ -1015,15 +1018,33 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __res
     }
 }
 
+// The kept-grid overflow fallback of the next adjoint substep (the grid of
+// substep f rebuilt when its active list outgrew the kept grid), run by the
+// leading workgroups of two launches of substep f+1's adjoint instead of two
+// launches of its own that exit at once on every substep whose grid fits:
+// the p2g rebuild beside the g2p adjoint, the grid rebuild beside the grid
+// adjoint (lead 0: not here).
+struct P2GRebuild { TileArgs ta; const float* st; const float* aff; float4* tiles; int lead; };
+DEV void p2g_scatter_body(float* tile, const TileArgs& ta, const float* __restrict__ st, const float* __restrict__ aff,
+                          float4* __restrict__ tiles, int only_overflow, int lead = 0, int nwg = 0);
+struct GridRebuild { GridArgs ga; const float4* tiles; float4* g_pm; float4* g_v; int lead; };
+
 // reverse of g2p over work items: scatter the adjoint of grid_v_out into the
 // item's tile buffer, write the first part of the adjoint of x (frame f).
 // a1 = adjoint of frame f+1, st1 = frame f+1.
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_g2p_bwd_tile(TileArgs ta, const float* __restrict__ st, const float* __restrict__ st1,
                                                      const float* __restrict__ a1, float* __restrict__ a0,
-                                                     const float4* __restrict__ g_v, float4* __restrict__ tiles_v, int use_kept) {
-    __shared__ float4 tv[TILE];
-    __shared__ float acc[(WG / 64) * 3 * TILE];
-    ItemIter ir = item_iter(ta);
+                                                     const float4* __restrict__ g_v, float4* __restrict__ tiles_v, int use_kept,
+                                                     P2GRebuild rb) {
+    // the velocity tile and the wave tiles, or the p2g tiles of a rebuild
+    __shared__ float4 smem[4 * TILE];
+    float4* tv = smem;
+    float* acc = reinterpret_cast<float*>(smem + TILE);
+    if ((int)blockIdx.x < rb.lead) {
+        p2g_scatter_body(reinterpret_cast<float*>(smem), rb.ta, rb.st, rb.aff, rb.tiles, 1, 0, rb.lead);
+        return;
+    }
+    ItemIter ir = item_iter(ta, rb.lead);
     bool kept = use_kept && kept_fits(ta.counters, ta.keep_cap);
     int lane = threadIdx.x & 63;
     int stride = ta.stride;
@@ -1161,10 +1182,16 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
 // reverse of grid_op: adjoint of (momentum, mass) per node, plus the adjoint
 // of the primitive states of frames f and f+1 (wave-reduced, then atomics).
 __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* __restrict__ g_pm, const float4* __restrict__ tiles_v,
-                                                     float4* __restrict__ g_adjpm, float* __restrict__ gpstate, int use_kept) {
+                                                     float4* __restrict__ g_adjpm, float* __restrict__ gpstate, int use_kept,
+                                                     GridRebuild rb) {
+    if ((int)blockIdx.x < rb.lead) {
+        grid_op_body(rb.ga, rb.tiles, rb.g_pm, rb.g_v, 1, 0, rb.lead);
+        return;
+    }
     int lane = threadIdx.x & 63;
     int wave = threadIdx.x >> 6;
-    int e = uniform(blockIdx.x * 4 + wave);
+    int nb = (int)gridDim.x - rb.lead;
+    int e = uniform(((int)blockIdx.x - rb.lead) * 4 + wave);
     // the first block's record is read together with the counters (a record
     // beyond the active count is stale and left unused)
     BlockMeta m;
@@ -1173,7 +1200,7 @@ __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* 
     int n_active = ga.counters[CNT_NBLK];
     bool kept = use_kept && kept_fits(ga.counters, ga.keep_cap);
     int stamp = ga.counters[CNT_STAMP];
-    for (; e < n_active; e += gridDim.x * 4) {
+    for (; e < n_active; e += nb * 4) {
         float4 pm = kept ? ga.keep_pm[(size_t)e * NODES_PER_BLK + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
         if (!have) block_meta(ga, e, stamp, m);
         have = false;
@@ -1339,9 +1366,9 @@ __global__ void __launch_bounds__(PPB) k_stress(ChunkArgs ca, const float* __res
 // every particle of the item into the block's LDS tile, flushed to the item's
 // tile buffer.  Reads x, v and the stored affine matrix.
 DEV void p2g_scatter_body(float* tile, const TileArgs& ta, const float* __restrict__ st, const float* __restrict__ aff,
-                          float4* __restrict__ tiles, int only_overflow) {
+                          float4* __restrict__ tiles, int only_overflow, int lead, int nwg) {
     if (only_overflow && kept_fits(ta.counters, ta.keep_cap)) return;
-    ItemIter ir = item_iter(ta);
+    ItemIter ir = item_iter(ta, lead, nwg);
     int lane = threadIdx.x & 63;
     int stride = ta.stride;
     // every wave scatters into a tile of its own (no LDS atomics between
@@ -2161,7 +2188,15 @@ struct mpm_ctx {
     float* gaff = nullptr;  // adjoint of the affine matrix (9 rows), one frame
     std::vector<int> aff_of; // frame whose affine matrix each slot holds (-1: none)
     float* adj[2] = {nullptr, nullptr};
-    float4 *g_pm = nullptr, *g_v = nullptr, *g_adjpm = nullptr;
+    // dense grids (momentum/mass, velocity) of substep f at gpm_[f & 1],
+    // gv_[f & 1], so that a rebuild of substep f can run while substep f+1's
+    // grid adjoint still reads its own; one grid adjoint
+    float4 *gpm_[2] = {nullptr, nullptr}, *gv_[2] = {nullptr, nullptr}, *g_adjpm = nullptr;
+    float4* g_pm(int f) const { return gpm_[f & 1]; }
+    float4* g_v(int f) const { return gv_[f & 1]; }
+    // substep whose overflow fallback ran beside the adjoint of substep f+1
+    // (-1: none pending)
+    int fb_done = -1;
     int *blk_flag = nullptr, *node_flag = nullptr, *pblocks = nullptr, *gstamp = nullptr;
     int *cell_count = nullptr, *cell_start = nullptr, *bin_q = nullptr, *bin_rank = nullptr, *blk_rep = nullptr;
     // Bin slots: the binning of a substep (index list, work items, per-block
@@ -2229,7 +2264,14 @@ struct mpm_ctx {
     float* aff_ptr(int f) const { return aff + (size_t)slot_of(f) * 9 * stride; }
     float* usv_ptr(int f) const { return usv + (size_t)slot_of(f) * USV_ROWS * stride; }
     int bslot_of(int f) const { return keep ? slot_of(f) : 0; }
-    void drop_kept() { for (auto& k : kept_of) k = -1; }
+    void drop_kept() { for (auto& k : kept_of) k = -1; fb_done = -1; }
+    // frame f's state is in memory: a checkpoint, or a ring frame of the
+    // resident segment
+    bool frame_resident(int f) const {
+        if (f < 0 || f > valid_hi) return false;
+        if (f % K == 0) return true;
+        return seg_valid == f / K && seg_lo <= f && f <= seg_hi;
+    }
     ChunkArgs cargs() const { ChunkArgs a; a.scenes = d_scenes; a.chunks = d_chunks; a.n_chunks = n_chunks; a.stride = stride; return a; }
     BinArgs bargs(int bs) const {
         const BinSlot& b = bslots[bs];
@@ -2347,7 +2389,7 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     L.off_aff = o; o = align256(o + sizeof(float) * 9 * (size_t)stride * (L.n_ckpt + L.ring + 1));
     L.off_usv = o; o = align256(o + sizeof(float) * USV_ROWS * (size_t)stride * (L.n_ckpt + L.ring));
     L.off_adj = o; o = align256(o + ff * 2);
-    L.off_grid = o; o = align256(o + sizeof(float4) * (size_t)L.n_nodes * 3);
+    L.off_grid = o; o = align256(o + sizeof(float4) * (size_t)L.n_nodes * 5);
     L.off_blkints = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 4);   // block flags, node flags, occupied list, representative slots
     L.off_cells = o; o = align256(o + sizeof(int) * (size_t)L.n_blocks * 64 * 2); // cell counts, cell starts
     L.off_pints = o; o = align256(o + sizeof(int) * (size_t)stride * 2);         // bin key, bin rank
@@ -2455,11 +2497,13 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
     { PhaseTimer t(c, MPM_PH_P2G, st);
       hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm); }
     { PhaseTimer t(c, MPM_PH_GRID, st);
-      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f, bs, c->keep), c->tiles_pm, c->g_pm, c->g_v); }
+      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f, bs, c->keep), c->tiles_pm, c->g_pm(f),
+                         c->g_v(f)); }
     { PhaseTimer t(c, MPM_PH_G2P, st);
-      hipLaunchKernelGGL(k_g2p_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, out, c->g_v); }
+      hipLaunchKernelGGL(k_g2p_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, out, c->g_v(f)); }
     HIPCHK(hipGetLastError());
     c->kept_of[bs] = c->keep ? f : -1;
+    c->fb_done = -1;   // the forward reuses the tile buffer and the dense grids
     return 0;
 }
 
@@ -2489,7 +2533,7 @@ int launch_fallbacks(mpm_ctx* c, int f, hipStream_t s) {
       hipLaunchKernelGGL(k_p2g_scatter_rebuild, dim3(64), dim3(WG), 0, s, ta, (const float*)c->frame_ptr(f), (const float*)c->aff_ptr(f),
                          c->tiles_pm, 1); }
     { PhaseTimer t(c, MPM_PH_GRID_KEEP, s);
-      hipLaunchKernelGGL(k_grid_op_rebuild, dim3(64), dim3(256), 0, s, ga, c->tiles_pm, c->g_pm, c->g_v, 1); }
+      hipLaunchKernelGGL(k_grid_op_rebuild, dim3(64), dim3(256), 0, s, ga, c->tiles_pm, c->g_pm(f), c->g_v(f), 1); }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -2527,16 +2571,31 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
         { PhaseTimer t(c, MPM_PH_P2G_GRID, st);
           hipLaunchKernelGGL(k_p2g_scatter_rebuild, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm, 0); }
         { PhaseTimer t(c, MPM_PH_GRID_KEEP, st);
-          hipLaunchKernelGGL(k_grid_op_rebuild, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->tiles_pm, c->g_pm, c->g_v, 0); }
-    } else if (c->keep_cap < c->n_blocks) {
+          hipLaunchKernelGGL(k_grid_op_rebuild, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->tiles_pm, c->g_pm(f), c->g_v(f), 0); }
+    } else if (c->keep_cap < c->n_blocks && c->fb_done != f) {
         if (launch_fallbacks(c, f, st)) return -1;
+    }
+    c->fb_done = -1;
+    // the next adjoint substep's overflow fallback rides on this one's g2p and
+    // grid adjoint launches when its binning, grid and frame are already here
+    // (and not under profiling, which times every launch on its own)
+    int g = f - 1;
+    bool fuse = g >= 0 && !c->prof && c->keep && c->keep_cap < c->n_blocks && c->kept_of[c->bslot_of(g)] == g &&
+                c->frame_resident(g) && c->aff_of[c->slot_of(g)] == g;
+    P2GRebuild prb{};
+    GridRebuild grb{};
+    if (fuse) {
+        int bg = c->bslot_of(g);
+        prb = P2GRebuild{c->targs(bg), c->frame_ptr(g), c->aff_ptr(g), c->tiles_pm, FB_LEAD};
+        grb = GridRebuild{c->gargs(g, bg, false), c->tiles_pm, c->g_pm(g), c->g_v(g), FB_LEAD};
     }
     int use_kept = kept ? 1 : 0;
     { PhaseTimer t(c, MPM_PH_G2P_BWD, st);
-      hipLaunchKernelGGL(k_g2p_bwd_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, in1, a1, a0, c->g_v, c->tiles_v, use_kept); }
+      hipLaunchKernelGGL(k_g2p_bwd_tile, dim3(c->tile_ctas + prb.lead), dim3(WG), 0, st, ta, in, in1, a1, a0, c->g_v(f), c->tiles_v,
+                         use_kept, prb); }
     { PhaseTimer t(c, MPM_PH_GRID_BWD, st);
-      hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->g_pm, c->tiles_v, c->g_adjpm, c->pstate_g,
-                         use_kept); }
+      hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas + grb.lead), dim3(256), 0, st, ga, c->g_pm(f), c->tiles_v, c->g_adjpm,
+                         c->pstate_g, use_kept, grb); }
     // fixed-order mode: the primitive adjoints are reduced by their own launch
     // when the collision adjoint needs them, else by the p2g adjoint's leading
     // workgroups (a multiple of NXCD, so the item workgroups keep their XCDs)
@@ -2564,6 +2623,7 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
       hipLaunchKernelGGL(k_stress_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, (const float*)c->gaff, a1, a0,
                          (const float*)c->usv_ptr(f), c->d_prims, c->n_prims, c->pstate, c->pvel, c->pstate_g, c->pvel_g, f); }
     HIPCHK(hipGetLastError());
+    if (fuse) c->fb_done = g;
     return 0;
 }
 
@@ -2666,9 +2726,11 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->adj[0] = (float*)(w + L.off_adj);
     c->adj[1] = c->adj[0] + c->frame_floats();
     float4* g = (float4*)(w + L.off_grid);
-    c->g_pm = g;
-    c->g_v = g + L.n_nodes;
+    c->gpm_[0] = g;
+    c->gv_[0] = g + L.n_nodes;
     c->g_adjpm = g + 2 * L.n_nodes;
+    c->gpm_[1] = g + 3 * L.n_nodes;
+    c->gv_[1] = g + 4 * L.n_nodes;
     int* bi = (int*)(w + L.off_blkints);
     c->blk_flag = bi;
     c->node_flag = bi + (size_t)L.n_blocks;

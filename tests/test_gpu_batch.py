@@ -120,6 +120,30 @@ def test_kept_and_rebuilt_adjoint_grids_agree(ckpt_k, keep_blocks):
     assert rel(g, g_ref) <= 1e-4
 
 
+def test_overflow_fallback_under_graph_replay():
+    """keep_blocks=1: every substep's kept grid overflows, so each substep's
+    grid is rebuilt for the adjoint by workgroups riding on the previous
+    adjoint substep's g2p and grid-adjoint launches.  Three replays of the
+    captured iteration (eager work between them) each give the gradients of
+    the eager kept-grid pass."""
+    from diffskill_amd import workloads
+    from diffskill_amd.batch import BatchedTrajOpt, SceneBatch
+    print("case overflow fallback under graph replay", flush=True)
+    w = small_workload()
+    targets = targets_for(w)
+    _, g_ref, _ = run_pass(w, targets)
+    H, S = w["H"], w["substeps"]
+    b = SceneBatch(w["scenes"], H * S + 1, keep_blocks=1)
+    init = torch.as_tensor(workloads.action_tensor(w["actions"], H))
+    opt = BatchedTrajOpt(b, H, S, targets, lr=0.0, init_actions=init, use_graph=True)
+    for _ in range(3):
+        opt.iteration()
+        torch.cuda.synchronize()
+        g = opt.action_grads().detach().cpu().double().numpy()
+        assert rel(g, g_ref) <= 1e-4
+        _ = torch.ones(1 << 20, device="cuda").sum().item()
+
+
 def test_batched_matches_per_scene_contexts():
     from diffskill_amd import workloads
     from diffskill_amd.batch import PlainTrajOpt
