@@ -1773,19 +1773,14 @@ DEV void mass_add(float* o, unsigned long long* macc, size_t id, float val, doub
 }
 
 // compute_grid_m_kernel (mpm_simulator.py:456-466) into a dense per-scene
-// grid.  Particles are stored in cell order, so the lanes of a wave mostly
-// share a stencil base: each group of equal bases is summed through the wave
-// transpose tree and adds its 27 node masses with one atomic each (lanes past
-// MAXSEG groups add their own).
-__global__ void __launch_bounds__(PPB) k_grid_mass(ChunkArgs ca, const float* __restrict__ st, float* __restrict__ out,
-                                                   unsigned long long* __restrict__ macc, int det) {
-    int4 ch = ca.chunks[blockIdx.x];
-    int i = threadIdx.x;
-    int lane = i & 63;
-    bool valid = i < ch.z;
-    const SceneC& S = ca.scenes[ch.x];
-    int slot = ch.y + (valid ? i : 0);
-    v3 x = load3(st, ca.stride, 0, slot);
+// grid.  The lanes of a wave that share a stencil base are summed through the
+// wave transpose tree and add their 27 node masses with one atomic each (lanes
+// past MAXSEG groups add their own), so the kernel wants the lanes of a wave
+// to hold particles of few cells: particle p of scene S at lane `valid`.
+DEV void grid_mass_wave(const SceneC& S, const float* __restrict__ st, int stride, int p, bool valid, float* __restrict__ out,
+                        unsigned long long* __restrict__ macc, int det) {
+    int lane = threadIdx.x & 63;
+    v3 x = load3(st, stride, 0, p);
     Stencil sw;
     stencil(S, x, sw);
     float* o = out + (size_t)S.m_off;
@@ -1823,6 +1818,31 @@ __global__ void __launch_bounds__(PPB) k_grid_mass(ChunkArgs ca, const float* __
         }
         act &= ~m;
         seg++;
+    }
+}
+// particles in slot order, one chunk per workgroup (stored in cell order at
+// set_state, they drift from it as the material deforms)
+__global__ void __launch_bounds__(PPB) k_grid_mass(ChunkArgs ca, const float* __restrict__ st, float* __restrict__ out,
+                                                   unsigned long long* __restrict__ macc, int det) {
+    int4 ch = ca.chunks[blockIdx.x];
+    bool valid = (int)threadIdx.x < ch.z;
+    grid_mass_wave(ca.scenes[ch.x], st, ca.stride, ch.y + (valid ? (int)threadIdx.x : 0), valid, out, macc, det);
+}
+// particles in the cell order of a kept binning of this frame or the one
+// before (its work items), so a wave's lanes share one or two stencil bases
+__global__ void __launch_bounds__(WG) k_grid_mass_items(TileArgs ta, const float* __restrict__ st, float* __restrict__ out,
+                                                        unsigned long long* __restrict__ macc, int det) {
+    ItemIter ir = item_iter(ta);
+    for (int j = ir.j; j < ir.n; j += ir.step) {
+        int e = ir.list ? ir.list[j] : j;
+        int4 it = ta.items[e];
+        const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
+        for (int k0 = 0; k0 < it.z; k0 += WG) {
+            int k = k0 + (int)threadIdx.x;
+            bool valid = k < it.z;
+            int p = ta.order[it.y + (valid ? k : 0)];
+            grid_mass_wave(S, st, ta.stride, p, valid, out, macc, det);
+        }
     }
 }
 // adjoint of compute_grid_m_kernel: gm is the adjoint of the mass grid, or,
@@ -3080,8 +3100,17 @@ int mpm_grid_mass(mpm_ctx* ctx, int f, float* out, void* stream) {
     if (!ctx || !out) return fail("mpm_grid_mass: null argument");
     hipStream_t st = S_(stream);
     if (ensure_frame(ctx, f, st)) return -1;
-    hipLaunchKernelGGL(k_grid_mass, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), out, ctx->macc,
-                       ctx->det ? 1 : 0);
+    // visit the particles in the cell order of frame f's binning, or of frame
+    // f-1's (one substep stale, nearly the same cells), when one is kept
+    int g = -1;
+    for (int h = f; h >= std::max(0, f - 1); h--)
+        if (g < 0 && ctx->keep && ctx->kept_of[ctx->bslot_of(h)] == h) g = h;
+    if (g >= 0)
+        hipLaunchKernelGGL(k_grid_mass_items, dim3(ctx->tile_ctas), dim3(WG), 0, st, ctx->targs(ctx->bslot_of(g)), ctx->frame_ptr(f), out,
+                           ctx->macc, ctx->det ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_grid_mass, dim3(ctx->n_chunks), dim3(PPB), 0, st, ctx->cargs(), ctx->frame_ptr(f), out, ctx->macc,
+                           ctx->det ? 1 : 0);
     if (ctx->det)
         hipLaunchKernelGGL(k_grid_mass_convert, dim3(256, ctx->n_scenes), dim3(256), 0, st, ctx->d_scenes, ctx->macc, out);
     HIPCHK(hipGetLastError());
