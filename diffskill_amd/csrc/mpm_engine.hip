@@ -229,9 +229,9 @@ constexpr int MAXSEG = 4;    // distinct cells per wave the mass grid reduces th
 
 // per bin slot counters: CNT_X0 + x counts the work items of XCD x, CNT_XB0 + x
 // is the first entry of its list in xlist
-constexpr int NXCD = 8;
-constexpr int DET_GMAX = 16;
-constexpr int FB_LEAD = 64;    // leading workgroups that carry an overflow fallback   // record-array entries per cell (fixed-order ranks)      // MI355X: 8 XCDs, workgroup b dispatched to XCD b % 8
+constexpr int NXCD = 8;        // MI355X: 8 XCDs, workgroup b dispatched to XCD b % 8
+constexpr int DET_GMAX = 16;   // record-array entries per cell (fixed-order ranks)
+constexpr int FB_LEAD = 64;    // leading workgroups that carry an overflow fallback
 enum { CNT_NBLK = 1, CNT_PART = 2, CNT_ITEMS = 3, CNT_STAMP = 4, CNT_X0 = 8, CNT_XB0 = 16 };
 // global binning state: the pass stamp base and the occupied-block count of
 // the pass in flight (both advanced/reset by the pass's last kernel)
