@@ -659,7 +659,19 @@ DEV float tsum16(float* r, int lane) {
     float s = r[0] + dpp_get<DPP_QUAD_XOR2>(u_of(r[0]));
     return s + dpp_get<DPP_QUAD_XOR1>(u_of(s));
 }
+// 4 values: two halving steps (bits 5, 4), then the sum over each 16-lane
+// group; afterwards every lane of group g = lane >> 4 holds the total of value g
+DEV float tsum4(float* r) {
+    tstep_swap32<2>(r);
+    tstep_swap16<1>(r);
+    float s = r[0] + dpp_get<DPP_ROW_MIRROR>(u_of(r[0]));
+    s += dpp_get<DPP_ROW_HALF_MIRROR>(u_of(s));
+    s += dpp_get<DPP_QUAD_XOR2>(u_of(s));
+    return s + dpp_get<DPP_QUAD_XOR1>(u_of(s));
+}
 DEV int tile_index(int l0, int l1, int l2, int n) { return ((l0 + n / 9) * 6 + (l1 + (n / 3) % 3)) * 6 + (l2 + n % 3); }
+// node v = (a * 3 + b) * 4 + z of the 3x3x4 nodes two z-adjacent cells cover
+DEV int tile_index4(int l0, int l1, int l2, int v) { return ((l0 + v / 12) * 6 + (l1 + (v / 4) % 3)) * 6 + (l2 + v % 4); }
 
 // Segmented scatter of 27 x NCOMP per-particle node values into the calling
 // wave's own LDS tile (component-major, TILE floats per component): the lanes
@@ -667,10 +679,14 @@ DEV int tile_index(int l0, int l1, int l2, int n) { return ((l0 + n / 9) * 6 + (
 // through the transpose tree and the 27 node sums added into the tile.  Every
 // cell takes the tree, however many a wave holds (adding the lanes' own values
 // with LDS atomics past a few cells measured slower, and its order within an
-// instruction is not fixed).  CHAN(c, r) writes the lane's 27 node values of
-// component c into r[0..26] (node n = (a * 3 + b) * 3 + cc); one transpose
-// pass per component, so a caller forms a pass's 27 values incrementally
-// along the stencil.
+// instruction is not fixed).  When the rest of the wave is one more cell, the
+// z-neighbour of the first (the common case in cell order: a wave of 64
+// sorted particles spans one or two cells), both go through one tree pass
+// over the 3x3x4 nodes they cover together (36 values: the 32-value tree and
+// a 4-value one) instead of two 27-value passes.  CHAN(c, r) writes the
+// lane's 27 node values of component c into r[0..26] (node n = (a * 3 + b) *
+// 3 + cc); one transpose pass per component, so a caller forms a pass's 27
+// values incrementally along the stencil.
 template <int NCOMP, class ChanFn>
 DEV void scatter_tile_ch(float* tile, bool valid, int key, int l0, int l1, int l2, int lane, ChanFn chan) {
     unsigned long long act = __ballot(valid);
@@ -680,6 +696,29 @@ DEV void scatter_tile_ch(float* tile, bool valid, int key, int l0, int l1, int l
         unsigned long long m = __ballot(valid && key == key0);
         bool in = (m >> lane) & 1ull;
         int s0 = __shfl(l0, lead), s1 = __shfl(l1, lead), s2 = __shfl(l2, lead);
+        unsigned long long mz = (key0 & 3) != 3 ? __ballot(valid && key == key0 + 1) : 0ull;
+        if (mz != 0ull && mz == (act & ~m)) {
+            bool inz = (mz >> lane) & 1ull;
+#pragma unroll
+            for (int c = 0; c < NCOMP; c++) {
+                float r[32], u[32], u4[4];
+                chan(c, r);
+#pragma unroll
+                for (int v = 0; v < 36; v++) {
+                    int ab = v / 4, z = v % 4;
+                    float fa = z < 3 ? r[ab * 3 + z] : 0.f;
+                    float fb = z > 0 ? r[ab * 3 + z - 1] : 0.f;
+                    float val = in ? fa : (inz ? fb : 0.f);
+                    if (v < 32) u[v] = val;
+                    else u4[v - 32] = val;
+                }
+                float s = tsum32(u, lane);
+                float t = tsum4(u4);
+                if (!(lane & 1)) atomicAdd(&tile[c * TILE + tile_index4(s0, s1, s2, lane >> 1)], s);
+                if (!(lane & 15)) atomicAdd(&tile[c * TILE + tile_index4(s0, s1, s2, 32 + (lane >> 4))], t);
+            }
+            break;
+        }
 #pragma unroll
         for (int c = 0; c < NCOMP; c++) {
             float r[32];
