@@ -111,14 +111,19 @@ class BatchedTrajOpt:
     -> action adjoints -> Adam step.  With ``use_graph`` the whole iteration,
     the Adam step included (``capturable`` Adam, its state created up front),
     is captured once as a HIP graph and replayed, so no host work sits
-    between iterations.
+    between iterations.  With ``resort`` (the default) the engine re-sorts
+    the particles' slots into cell order at every env step boundary
+    (mpm_set_resort, when ``substeps`` is at least 8): the losses and action
+    gradients do not depend on slot order, and the tile kernels keep reading
+    and writing whole cache lines as the dough deforms.
     """
 
-    def __init__(self, batch, H, substeps, targets, lr=1e-2, init_actions=None, use_graph=True):
+    def __init__(self, batch, H, substeps, targets, lr=1e-2, init_actions=None, use_graph=True, resort=True):
         self.b = batch
         self.H = int(H)
         self.S = int(substeps)
         self.T = self.H * self.S
+        batch.engine.set_resort(self.S if resort and self.S >= 8 else 0)
         self.targets = targets.contiguous()
         P = max(1, batch.n_prims)
         a = torch.zeros((self.H, P, 7), dtype=torch.float32, device="cuda") if init_actions is None else \
@@ -227,7 +232,7 @@ class PlainTrajOpt:
     reference drives one Taichi scene per process."""
 
     def __init__(self, scenes, H, substeps, targets, lr=1e-2, init_actions=None, ckpt_k=0, softness=666.0, keep_blocks=-1,
-                 deterministic=True):
+                 deterministic=True, resort=True):
         self.parts = []
         P0 = 0
         for k, sc in enumerate(scenes):
@@ -237,7 +242,8 @@ class PlainTrajOpt:
             if init_actions is not None:
                 ia = init_actions[:, P0:P0 + max(1, b.n_prims)]
             P0 += b.n_prims
-            self.parts.append(BatchedTrajOpt(b, H, substeps, targets[k:k + 1], lr=lr, init_actions=ia, use_graph=False))
+            self.parts.append(BatchedTrajOpt(b, H, substeps, targets[k:k + 1], lr=lr, init_actions=ia, use_graph=False,
+                                             resort=resort))
 
     @property
     def total_particles(self):

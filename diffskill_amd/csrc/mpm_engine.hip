@@ -232,6 +232,7 @@ constexpr int MAXSEG = 4;    // distinct cells per wave the mass grid reduces th
 constexpr int NXCD = 8;        // MI355X: 8 XCDs, workgroup b dispatched to XCD b % 8
 constexpr int DET_GMAX = 16;   // record-array entries per cell (fixed-order ranks)
 constexpr int FB_LEAD = 64;    // leading workgroups that carry an overflow fallback
+constexpr int RESORT_MIN = 8;  // shortest slot re-sort interval (substeps) the workspace holds permutations for
 enum { CNT_NBLK = 1, CNT_PART = 2, CNT_ITEMS = 3, CNT_STAMP = 4, CNT_X0 = 8, CNT_XB0 = 16 };
 // global binning state: the pass stamp base and the occupied-block count of
 // the pass in flight (both advanced/reset by the pass's last kernel)
@@ -280,6 +281,7 @@ struct BinArgs {
     // on one XCD; xlist lists the items grouped by tag
     int* blk_rep;
     int* xlist;
+    int* blk_tot;    // particles of each occupied block this pass
 };
 
 DEV unsigned long long lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
@@ -469,6 +471,7 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nbl
             a.blk_stamp[gb] = stamp;
             a.blk_item0[gb] = ib;
             a.blk_nitems[gb] = nit;
+            a.blk_tot[gb] = total;
         }
     }
 }
@@ -974,9 +977,13 @@ DEV void load_tile_kept(float4* tv, const SceneC& S, int bi, int bj, int bk, con
 }
 
 // g2p over work items: 27-node gather from the LDS tile, APIC C, advection
-// with clamp (mpm_simulator.py:264-283)
+// with clamp (mpm_simulator.py:264-283).  With perm given (a substep that ends
+// at a re-sort boundary) particle p's outputs go to slot perm[p] of frame f+1,
+// and its F of frame f+1, which the constitutive kernel left at slot p of
+// fstage, moves there too.
 __global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __restrict__ st, float* __restrict__ st_out,
-                                                 const float4* __restrict__ g_v) {
+                                                 const float4* __restrict__ g_v, const int* __restrict__ perm,
+                                                 const float* __restrict__ fstage) {
     __shared__ float4 tv[TILE];
     ItemIter ir = item_iter(ta);
     int stride = ta.stride;
@@ -1045,11 +1052,16 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __res
                 nC.a[3] = c4 * Gx.y, nC.a[4] = c4 * Gy.y, nC.a[5] = c4 * Gz.y;
                 nC.a[6] = c4 * Gx.z, nC.a[7] = c4 * Gy.z, nC.a[8] = c4 * Gz.z;
             }
-            store3(st_out, stride, 3, p, nv);
-            store9(st_out, stride, 6, p, nC);
+            int o = p;
+            if (perm) {
+                o = perm[p];
+                store9(st_out, stride, 15, o, load9(fstage, stride, 15, p));
+            }
+            store3(st_out, stride, 3, o, nv);
+            store9(st_out, stride, 6, o, nC);
             v3 xo = mk(tmax(tmin(x.x + S.dt * nv.x, S.upper), S.lower), tmax(tmin(x.y + S.dt * nv.y, S.upper), S.lower),
                        tmax(tmin(x.z + S.dt * nv.z, S.upper), S.lower));
-            store3(st_out, stride, 0, p, xo);
+            store3(st_out, stride, 0, o, xo);
             p = pn;
             x = xn;
         }
@@ -1074,7 +1086,7 @@ struct GridRebuild { GridArgs ga; const float4* tiles; float4* g_pm; float4* g_v
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_g2p_bwd_tile(TileArgs ta, const float* __restrict__ st, const float* __restrict__ st1,
                                                      const float* __restrict__ a1, float* __restrict__ a0,
                                                      const float4* __restrict__ g_v, float4* __restrict__ tiles_v, int use_kept,
-                                                     P2GRebuild rb) {
+                                                     P2GRebuild rb, const int* __restrict__ perm1) {
     // the velocity tile and the wave tiles, or the p2g tiles of a rebuild
     __shared__ float4 smem[4 * TILE];
     float4* tv = smem;
@@ -1098,7 +1110,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
         // loads are issued first, so both latencies overlap
         int p = ta.order[it.y + min((int)threadIdx.x, it.z - 1)];
         v3 x = load3(st, stride, 0, p);
-        v3 x1 = load3(st1, stride, 0, p);
+        // frame f+1 holds particle p at slot perm1[p] when it was re-sorted
+        v3 x1 = load3(st1, stride, 0, perm1 ? perm1[p] : p);
         v3 gx1 = load3(a1, stride, 0, p);
         v3 gv1 = load3(a1, stride, 3, p);
         m3 gC = load9(a1, stride, 6, p);
@@ -1115,7 +1128,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
             if (k0 > 0) {
                 p = valid ? ta.order[it.y + k] : ta.order[it.y];
                 x = load3(st, stride, 0, p);
-                x1 = load3(st1, stride, 0, p);
+                x1 = load3(st1, stride, 0, perm1 ? perm1[p] : p);
                 gx1 = load3(a1, stride, 0, p);
                 gv1 = load3(a1, stride, 3, p);
                 gC = load9(a1, stride, 6, p);
@@ -2228,6 +2241,79 @@ __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
 
 }  // namespace
 
+// ------------------------------------------------------------------ slot re-sort
+// Every R substeps the particles' slots are re-sorted into the cell order of
+// the binning of the substep that ends at the boundary frame (a relabelling
+// of slots within each scene: gathers and scatters of the tile kernels then
+// touch whole cache lines again as the material deforms).  The new slot of a
+// particle is its scene's first column + the particles of the scene's
+// occupied blocks below its block (block-index order) + its position within
+// its block's range of the sorted list: a function of the binning alone, so
+// the fixed-order mode's binning gives a fixed-order re-sort.
+// Pass 1, one workgroup per scene: exclusive sums of the occupied blocks'
+// particle counts in block-index order.
+__global__ void __launch_bounds__(256) k_resort_blocks(BinArgs a, int* __restrict__ blk_pre) {
+    __shared__ int s_w[4];
+    const SceneC& S = a.scenes[blockIdx.x];
+    int nbk = S.nb * S.nb * S.nb;
+    int stamp = a.counters[CNT_STAMP];
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int run = 0;
+    for (int b0 = 0; b0 < nbk; b0 += 256) {
+        int b = b0 + (int)threadIdx.x;
+        int gb = S.blk_off + b;
+        int c = (b < nbk && a.blk_stamp[gb] == stamp) ? a.blk_tot[gb] : 0;
+        int incl = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            int t = __shfl_up(incl, d);
+            if (lane >= d) incl += t;
+        }
+        if (lane == 63) s_w[wave] = incl;
+        __syncthreads();
+        int off = run;
+        for (int w = 0; w < wave; w++) off += s_w[w];
+        if (b < nbk) blk_pre[gb] = off + incl - c;
+        run += s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        __syncthreads();
+    }
+}
+// Pass 2 over the work items: perm[slot] = new slot of the particle.
+__global__ void __launch_bounds__(WG) k_resort_define(TileArgs ta, const int* __restrict__ cell_start, const int* __restrict__ blk_pre,
+                                                      int* __restrict__ perm) {
+    ItemIter ir = item_iter(ta);
+    for (int j = ir.j; j < ir.n; j += ir.step) {
+        int e = ir.list ? ir.list[j] : j;
+        int4 it = ta.items[e];
+        const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
+        int base = S.p_off + blk_pre[it.x] - cell_start[it.x * 64];
+        for (int k = threadIdx.x; k < it.z; k += WG) {
+            int i = it.y + k;
+            perm[ta.order[i]] = base + i;
+        }
+    }
+}
+// The adjoint of a boundary frame, in its (re-sorted) slot order, gathered
+// back into the order of the frame before: dst[row][p] = src[row][perm[p]].
+__global__ void __launch_bounds__(PPB) k_resort_adjoint(ChunkArgs ca, const float* __restrict__ src, float* __restrict__ dst,
+                                                       const int* __restrict__ perm) {
+    int4 ch = ca.chunks[blockIdx.x];
+    if ((int)threadIdx.x >= ch.z) return;
+    int p = ch.y + threadIdx.x;
+    int q = perm[p];
+    size_t stride = (size_t)ca.stride;
+#pragma unroll
+    for (int r = 0; r < NST; r++) dst[r * stride + p] = src[r * stride + q];
+}
+// caller's view: map[c] (slot of frame 0's particle c) through perm
+__global__ void k_resort_compose(int* __restrict__ map, const int* __restrict__ perm, const SceneC* __restrict__ scenes) {
+    const SceneC& S = scenes[blockIdx.y];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < S.p_cnt; i += gridDim.x * blockDim.x) {
+        int c = S.p_off + i;
+        map[c] = perm[map[c]];
+    }
+}
+
 // ==================================================================== host side
 struct mpm_ctx {
     int n_scenes = 0, n_prims = 0, max_frames = 0, K = 0, n_ckpt = 0, ring = 0;
@@ -2256,6 +2342,17 @@ struct mpm_ctx {
     // substep whose overflow fallback ran beside the adjoint of substep f+1
     // (-1: none pending)
     int fb_done = -1;
+    // slot re-sort every `resort` substeps (0: off): frame s with s % resort
+    // == 0 (0 < s < max_frames - 1) is written in the cell order of substep
+    // s-1's binning; perm_of(s)[slot in frame s-1] = slot in frame s
+    int resort = 0;
+    int* rs_perm = nullptr;
+    float* rs_fstage = nullptr;
+    float* rs_adj = nullptr;
+    int *blk_tot = nullptr, *blk_pre = nullptr;
+    bool is_bnd(int s) const { return resort > 0 && s > 0 && s % resort == 0 && s + 1 < max_frames; }
+    int* perm_of(int s) const { return rs_perm + (size_t)(s / resort - 1) * stride; }
+    int epoch_of(int f) const { return resort > 0 ? std::max(0, std::min(f, max_frames - 2) / resort) : 0; }
     int *blk_flag = nullptr, *node_flag = nullptr, *pblocks = nullptr, *gstamp = nullptr;
     int *cell_count = nullptr, *cell_start = nullptr, *bin_q = nullptr, *bin_rank = nullptr, *blk_rep = nullptr;
     // Bin slots: the binning of a substep (index list, work items, per-block
@@ -2343,7 +2440,7 @@ struct mpm_ctx {
         a.det = det ? 1 : 0;
         a.cell_gcnt = cell_gcnt; a.cell_ng = cell_ng; a.cell_grp = cell_grp;
         a.cell_head = cell_head; a.grp_next = grp_next; a.grp_cnt = grp_cnt;
-        a.blk_rep = blk_rep; a.xlist = b.xlist;
+        a.blk_rep = blk_rep; a.xlist = b.xlist; a.blk_tot = blk_tot;
         return a;
     }
     TileArgs targs(int bs) const {
@@ -2399,6 +2496,9 @@ struct Layout {
     // fixed-order mode buffers (within off_det): cell list heads, group
     // records, per-block primitive adjoints, block bitmaps, integer mass grid
     size_t det_gn, det_garr, det_head, det_grp, det_padj, det_pbits, det_macc, det_pmd;
+    // slot re-sort: permutations of the boundary frames, F of a boundary frame
+    // in the old order, one adjoint frame, per-block particle counts and sums
+    size_t off_resort, rs_perm, rs_fstage, rs_adj, rs_blk, n_bnd;
     int n_bm_words;
     long long n_mass;
     // one bin slot: order, items, blk_stamp, blk_item0, blk_nitems, nblocks, nmap, counters
@@ -2498,6 +2598,14 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     // lie past the stride (its first slot plus 256)
     L.det_pmd = d; d = align256(d + sizeof(float) * 8 * MAX_UNITS * (size_t)(stride / 64 + PPB / 64));
     o = align256(o + d);
+    L.off_resort = o;
+    L.n_bnd = (size_t)std::max(1, (max_frames - 2) / RESORT_MIN);
+    size_t r = 0;
+    L.rs_perm = r; r = align256(r + sizeof(int) * L.n_bnd * (size_t)stride);
+    L.rs_fstage = r; r = align256(r + ff);
+    L.rs_adj = r; r = align256(r + ff);
+    L.rs_blk = r; r = align256(r + sizeof(int) * 2 * (size_t)L.n_blocks);
+    o = align256(o + r);
     L.total = o;
     return L;
 }
@@ -2528,10 +2636,14 @@ int run_bin(mpm_ctx* c, int f, int do_fk, hipStream_t st) {
     return 0;
 }
 
-// forward substep f -> f+1 (writes frame f+1 wherever its slot is)
-int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
+// forward substep f -> f+1 (writes frame f+1 wherever its slot is).  When
+// frame f+1 is a re-sort boundary, its slot order is defined from this
+// substep's binning, or, recomputing (a checkpoint segment), the order the
+// first pass defined is applied again.
+int run_fwd(mpm_ctx* c, int f, hipStream_t st, bool recompute = false) {
     const float* in = c->frame_ptr(f);
     float* out = c->frame_ptr(f + 1);
+    bool bnd = c->is_bnd(f + 1);
     int bs = c->bslot_of(f);
     TileArgs ta = c->targs(bs);
     float* aff = c->aff_ptr(f);
@@ -2544,10 +2656,15 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
         HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     }
     { PhaseTimer t(c, MPM_PH_STRESS, sst);
-      hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, sst, c->cargs(), in, out, aff, c->usv_ptr(f)); }
+      hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, sst, c->cargs(), in, bnd ? c->rs_fstage : out, aff, c->usv_ptr(f)); }
     if (fork) HIPCHK(hipEventRecord(c->ev_join, c->side));
     c->aff_of[c->slot_of(f)] = f;
     if (run_bin(c, f, 1, st)) return -1;
+    if (bnd && !recompute) {
+        hipLaunchKernelGGL(k_resort_blocks, dim3(c->n_scenes), dim3(256), 0, st, c->bargs(bs), c->blk_pre);
+        hipLaunchKernelGGL(k_resort_define, dim3(c->tile_ctas), dim3(WG), 0, st, ta, (const int*)c->cell_start, (const int*)c->blk_pre,
+                           c->perm_of(f + 1));
+    }
     if (c->n_pairs > 0) {
         PhaseTimer t(c, MPM_PH_COLLIDE, st);
         hipLaunchKernelGGL(k_tool_collide, dim3(c->n_cgroups), dim3(256), 0, st, c->coll, c->d_prims, c->pstate, c->n_prims, f + 1);
@@ -2559,7 +2676,8 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st) {
       hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f, bs, c->keep), c->tiles_pm, c->g_pm(f),
                          c->g_v(f)); }
     { PhaseTimer t(c, MPM_PH_G2P, st);
-      hipLaunchKernelGGL(k_g2p_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, out, c->g_v(f)); }
+      hipLaunchKernelGGL(k_g2p_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, out, c->g_v(f),
+                         bnd ? (const int*)c->perm_of(f + 1) : (const int*)nullptr, (const float*)c->rs_fstage); }
     HIPCHK(hipGetLastError());
     c->kept_of[bs] = c->keep ? f : -1;
     c->fb_done = -1;   // the forward reuses the tile buffer and the dense grids
@@ -2575,7 +2693,7 @@ int ensure_frame(mpm_ctx* c, int f, hipStream_t st) {
     // recompute the segment from its checkpoint (never rewrites a checkpoint)
     int start = a * c->K;
     for (int s = start; s < f; s++)
-        if (run_fwd(c, s, st)) return -1;
+        if (run_fwd(c, s, st, true)) return -1;
     c->seg_valid = a;
     c->seg_lo = start + 1;
     c->seg_hi = f;
@@ -2603,6 +2721,14 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     const float* in = c->frame_ptr(f);
     const float* in1 = c->frame_ptr(f + 1);
     const float* a1 = c->adj_ptr(f + 1);
+    // frame f+1 re-sorted: its adjoint gathered back into frame f's order,
+    // its positions read through the permutation
+    const int* perm1 = nullptr;
+    if (c->is_bnd(f + 1)) {
+        perm1 = c->perm_of(f + 1);
+        hipLaunchKernelGGL(k_resort_adjoint, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), a1, c->rs_adj, perm1);
+        a1 = c->rs_adj;
+    }
     float* a0 = c->adj_ptr(f);
     int bs = c->bslot_of(f);
     // bins and grid of substep f kept from its forward run, or rebuilt here
@@ -2651,7 +2777,7 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     int use_kept = kept ? 1 : 0;
     { PhaseTimer t(c, MPM_PH_G2P_BWD, st);
       hipLaunchKernelGGL(k_g2p_bwd_tile, dim3(c->tile_ctas + prb.lead), dim3(WG), 0, st, ta, in, in1, a1, a0, c->g_v(f), c->tiles_v,
-                         use_kept, prb); }
+                         use_kept, prb, perm1); }
     { PhaseTimer t(c, MPM_PH_GRID_BWD, st);
       hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas + grb.lead), dim3(256), 0, st, ga, c->g_pm(f), c->tiles_v, c->g_adjpm,
                          c->pstate_g, use_kept, grb); }
@@ -2842,6 +2968,11 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->pbits = (unsigned*)(w + L.off_det + L.det_pbits);
     c->macc = (unsigned long long*)(w + L.off_det + L.det_macc);
     c->pmd = (float*)(w + L.off_det + L.det_pmd);
+    c->rs_perm = (int*)(w + L.off_resort + L.rs_perm);
+    c->rs_fstage = (float*)(w + L.off_resort + L.rs_fstage);
+    c->rs_adj = (float*)(w + L.off_resort + L.rs_adj);
+    c->blk_tot = (int*)(w + L.off_resort + L.rs_blk);
+    c->blk_pre = c->blk_tot + L.n_blocks;
 
     // host tables
     std::vector<int4> chunks;
@@ -2969,6 +3100,38 @@ int mpm_set_deterministic(mpm_ctx* ctx, int on) {
 }
 
 int mpm_get_deterministic(mpm_ctx* ctx) { return ctx ? (ctx->det ? 1 : 0) : fail("mpm_get_deterministic: null context"); }
+
+int mpm_set_resort(mpm_ctx* ctx, int substeps) {
+    if (!ctx) return fail("mpm_set_resort: null context");
+    if (substeps != 0 && substeps < RESORT_MIN)
+        return fail("mpm_set_resort: interval must be 0 (off) or at least " + std::to_string(RESORT_MIN) + " substeps");
+    if (substeps == ctx->resort) return 0;
+    ctx->resort = substeps;
+    // frames after 0 were written in the previous slot orders
+    ctx->valid_hi = std::min(ctx->valid_hi, 0);
+    ctx->seg_valid = -1;
+    ctx->adj_top = -1;
+    ctx->drop_kept();
+    return 0;
+}
+
+int mpm_get_resort(mpm_ctx* ctx) { return ctx ? ctx->resort : fail("mpm_get_resort: null context"); }
+
+int mpm_slot_order(mpm_ctx* ctx, int f, int* map, void* stream) {
+    if (!ctx || !map) return fail("mpm_slot_order: null argument");
+    if (f < 0 || f >= ctx->max_frames) return fail("mpm_slot_order: frame " + std::to_string(f) + " outside [0, max_frames)");
+    if (f > ctx->valid_hi) return fail("mpm_slot_order: frame " + std::to_string(f) + " has not been simulated");
+    hipStream_t st = S_(stream);
+    std::vector<int> id(ctx->stride);
+    for (int i = 0; i < ctx->stride; i++) id[i] = i;
+    HIPCHK(hipMemcpyAsync(map, id.data(), sizeof(int) * (size_t)ctx->stride, hipMemcpyHostToDevice, st));
+    for (int s = 1; s <= f; s++)
+        if (ctx->is_bnd(s))
+            hipLaunchKernelGGL(k_resort_compose, dim3(64, ctx->n_scenes), dim3(256), 0, st, map, (const int*)ctx->perm_of(s), ctx->d_scenes);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
 
 int mpm_set_profiling(mpm_ctx* ctx, int on) {
     if (!ctx) return fail("mpm_set_profiling: null context");
@@ -3143,7 +3306,7 @@ int mpm_grid_mass(mpm_ctx* ctx, int f, float* out, void* stream) {
     // f-1's (one substep stale, nearly the same cells), when one is kept
     int g = -1;
     for (int h = f; h >= std::max(0, f - 1); h--)
-        if (g < 0 && ctx->keep && ctx->kept_of[ctx->bslot_of(h)] == h) g = h;
+        if (g < 0 && ctx->keep && ctx->kept_of[ctx->bslot_of(h)] == h && ctx->epoch_of(h) == ctx->epoch_of(f)) g = h;
     if (g >= 0)
         hipLaunchKernelGGL(k_grid_mass_items, dim3(ctx->tile_ctas), dim3(WG), 0, st, ctx->targs(ctx->bslot_of(g)), ctx->frame_ptr(f), out,
                            ctx->macc, ctx->det ? 1 : 0);
@@ -3373,6 +3536,9 @@ int mpm_copy_frame(mpm_ctx* ctx, int src, int dst, void* stream) {
     if (!ctx) return fail("mpm_copy_frame: null context");
     if (dst < 0 || dst >= ctx->max_frames) return fail("mpm_copy_frame: frame out of range");
     hipStream_t st = S_(stream);
+    if (ctx->epoch_of(src) != ctx->epoch_of(dst))
+        return fail("mpm_copy_frame: frames " + std::to_string(src) + " and " + std::to_string(dst) +
+                    " have different slot orders (slot re-sort is on: mpm_set_resort)");
     if (ensure_frame(ctx, src, st)) return -1;
     if (src != dst) {
         HIPCHK(hipMemcpyAsync(ctx->frame_ptr(dst), ctx->frame_ptr(src), ctx->frame_floats() * sizeof(float), hipMemcpyDeviceToDevice, st));

@@ -265,6 +265,23 @@ class Engine:
     def deterministic(self):
         return self.L.mpm_get_deterministic(self.ctx) == 1
 
+    def set_resort(self, substeps):
+        """Re-sort the particles' slots into cell order every ``substeps``
+        substeps (0: off); invalidates every frame after 0 (see
+        mpm_set_resort)."""
+        _lib.check(self.L.mpm_set_resort(self.ctx, int(substeps)), "mpm_set_resort")
+
+    @property
+    def resort(self):
+        return self.L.mpm_get_resort(self.ctx)
+
+    def slot_order(self, f):
+        """int64 tensor (stride,): the slot in frame f of the particle at each
+        slot of frame 0."""
+        out = torch.empty(self.stride, dtype=torch.int32, device="cuda")
+        _lib.check(self.L.mpm_slot_order(self.ctx, f, out.data_ptr(), _stream()), "mpm_slot_order")
+        return out.long()
+
     def set_profiling(self, on=True):
         _lib.check(self.L.mpm_set_profiling(self.ctx, 1 if on else 0), "mpm_set_profiling")
 

@@ -253,6 +253,24 @@ int mpm_profile_read(mpm_ctx *ctx, double *ms, long long *counts, int reset);
 int mpm_set_deterministic(mpm_ctx *ctx, int on);
 int mpm_get_deterministic(mpm_ctx *ctx);
 
+/* Slot re-sort (no counterpart in the reference, whose particle fields have
+ * one fixed order): with substeps = R >= 8, every frame s with s % R == 0
+ * (0 < s < max_frames - 1) is written with its particles' slots re-sorted
+ * into the cell order of substep s-1's binning, within each scene's columns,
+ * so the tile kernels' gathers and scatters keep touching whole cache lines
+ * as the material deforms; the adjoint maps back across each boundary, a
+ * checkpointed recompute applies the orders the first pass defined, and the
+ * results are those of the unsorted run up to float summation order (bitwise
+ * reproducible in the fixed-order mode).  0 (the default) turns it off.
+ * Setting it invalidates every frame after 0.  mpm_frame and the other
+ * per-particle accessors of frame f then see frame f's own slot order;
+ * mpm_slot_order writes, for every slot c of frame 0, the slot its particle
+ * holds in frame f (stride ints, device memory; synchronises the stream).
+ * mpm_copy_frame refuses frames of different slot orders. */
+int mpm_set_resort(mpm_ctx *ctx, int substeps);
+int mpm_get_resort(mpm_ctx *ctx);
+int mpm_slot_order(mpm_ctx *ctx, int f, int *map, void *stream);
+
 const char *mpm_last_error(void);
 
 #ifdef __cplusplus
