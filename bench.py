@@ -264,6 +264,8 @@ def main():
     ap.add_argument("--keep-blocks", type=int, default=-1,
                     help="node blocks of each substep's grid kept for the adjoint (-1 default, 0 rebuild every grid)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--resort", type=int, default=-1,
+                    help="slot re-sort interval in substeps (-1: every env step, 0: off)")
     ap.add_argument("--unordered", action="store_true",
                     help="unordered float atomics instead of the engine's default fixed-order (bitwise-reproducible) mode")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
@@ -302,7 +304,9 @@ def main():
     batch = SceneBatch(w["scenes"], T + 1, ckpt_k=args.ckpt_k, keep_blocks=args.keep_blocks, deterministic=not args.unordered)
     targets = build_targets(w)
     init = torch.as_tensor(workloads.action_tensor(w["actions"], H))
-    opt = BatchedTrajOpt(batch, H, S, targets, lr=1e-2, init_actions=init, use_graph=not args.no_graph)
+    resort = True if args.resort < 0 else args.resort
+    opt = BatchedTrajOpt(batch, H, S, targets, lr=1e-2, init_actions=init, use_graph=not args.no_graph, resort=resort)
+    resort_used = batch.engine.resort
     n_total = batch.total_particles
     launch = None
     if args.launch:
@@ -407,7 +411,7 @@ def main():
     if not args.no_plain:
         del opt
         p = PlainTrajOpt(w["scenes"], H, S, targets, lr=1e-2, init_actions=init, ckpt_k=args.ckpt_k, keep_blocks=args.keep_blocks,
-                         deterministic=not args.unordered)
+                         deterministic=not args.unordered, resort=resort)
         p.iteration()
         pm = timed(p.iteration, 1)
         plain = dict(ms_per_step=round(pm, 3), value=n_total * T / (pm / 1000), speedup=round(pm / ms_step, 3))
@@ -457,7 +461,7 @@ def main():
                    "scenes": len(w["scenes"]), "particles": n_total, "substeps": T, "grid": 64,
                    "ckpt_k": args.ckpt_k or "store-all", "keep_blocks": args.keep_blocks,
                    "launch": launch_used, "autotuned": launch is not None, "hip_graph": not args.no_graph,
-                   "fixed_order": not args.unordered},
+                   "fixed_order": not args.unordered, "resort_substeps": resort_used},
         "grad_gate": gate, "grad_checks": grad_checks,
         "plain": plain, "forward": forward, "extra": extra,
         "kernels": kernels, "roofline": roofline, "valu_issue": valu_issue, "cpu_baseline": cpu,

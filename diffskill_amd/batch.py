@@ -113,9 +113,10 @@ class BatchedTrajOpt:
     is captured once as a HIP graph and replayed, so no host work sits
     between iterations.  With ``resort`` (the default) the engine re-sorts
     the particles' slots into cell order at every env step boundary
-    (mpm_set_resort, when ``substeps`` is at least 8): the losses and action
-    gradients do not depend on slot order, and the tile kernels keep reading
-    and writing whole cache lines as the dough deforms.
+    (mpm_set_resort, when ``substeps`` is at least 8), or every ``resort``
+    substeps when an int is given: the losses and action gradients do not
+    depend on slot order, and the tile kernels keep reading and writing
+    whole cache lines as the dough deforms.
     """
 
     def __init__(self, batch, H, substeps, targets, lr=1e-2, init_actions=None, use_graph=True, resort=True):
@@ -123,7 +124,8 @@ class BatchedTrajOpt:
         self.H = int(H)
         self.S = int(substeps)
         self.T = self.H * self.S
-        batch.engine.set_resort(self.S if resort and self.S >= 8 else 0)
+        r = self.S if resort is True else int(resort or 0)
+        batch.engine.set_resort(r if r >= 8 else 0)
         self.targets = targets.contiguous()
         P = max(1, batch.n_prims)
         a = torch.zeros((self.H, P, 7), dtype=torch.float32, device="cuda") if init_actions is None else \

@@ -1110,11 +1110,13 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
         // loads are issued first, so both latencies overlap
         int p = ta.order[it.y + min((int)threadIdx.x, it.z - 1)];
         v3 x = load3(st, stride, 0, p);
-        // frame f+1 holds particle p at slot perm1[p] when it was re-sorted
-        v3 x1 = load3(st1, stride, 0, perm1 ? perm1[p] : p);
-        v3 gx1 = load3(a1, stride, 0, p);
-        v3 gv1 = load3(a1, stride, 3, p);
-        m3 gC = load9(a1, stride, 6, p);
+        // frame f+1 and its adjoint hold particle p at slot perm1[p] when the
+        // frame was re-sorted
+        int p1 = perm1 ? perm1[p] : p;
+        v3 x1 = load3(st1, stride, 0, p1);
+        v3 gx1 = load3(a1, stride, 0, p1);
+        v3 gv1 = load3(a1, stride, 3, p1);
+        m3 gC = load9(a1, stride, 6, p1);
         const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
         int bi, bj, bk;
         block_origin(S, it.x, bi, bj, bk);
@@ -1128,10 +1130,11 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
             if (k0 > 0) {
                 p = valid ? ta.order[it.y + k] : ta.order[it.y];
                 x = load3(st, stride, 0, p);
-                x1 = load3(st1, stride, 0, perm1 ? perm1[p] : p);
-                gx1 = load3(a1, stride, 0, p);
-                gv1 = load3(a1, stride, 3, p);
-                gC = load9(a1, stride, 6, p);
+                p1 = perm1 ? perm1[p] : p;
+                x1 = load3(st1, stride, 0, p1);
+                gx1 = load3(a1, stride, 0, p1);
+                gv1 = load3(a1, stride, 3, p1);
+                gC = load9(a1, stride, 6, p1);
             }
             // clamp mask from the stored output position: the adjoint passes iff lower < x' < upper
             v3 gxn = mk((S.lower < x1.x && x1.x < S.upper) ? gx1.x : 0.f, (S.lower < x1.y && x1.y < S.upper) ? gx1.y : 0.f,
@@ -1717,7 +1720,8 @@ __global__ void __launch_bounds__(PPB) k_stress_bwd(ChunkArgs ca, const float* _
                                                     const float* __restrict__ usv,
                                                     const PrimC* __restrict__ prims, int n_prims_total,
                                                     const float* __restrict__ pstate, const float* __restrict__ pvel,
-                                                    float* __restrict__ gpstate, float* __restrict__ gpvel, int f) {
+                                                    float* __restrict__ gpstate, float* __restrict__ gpvel, int f,
+                                                    const int* __restrict__ perm1) {
     if (blockIdx.x == 0)
         for (int q = threadIdx.x; q < n_prims_total; q += blockDim.x) {
             size_t o0 = ((size_t)f * n_prims_total + q) * 8, o1 = ((size_t)(f + 1) * n_prims_total + q) * 8;
@@ -1743,7 +1747,7 @@ __global__ void __launch_bounds__(PPB) k_stress_bwd(ChunkArgs ca, const float* _
 #pragma unroll
     for (int kk = 0; kk < 9; kk++) FmR.a[kk] = t.Fn.a[kk] - t.R.a[kk];
     m3 sF = mmul(gst, t.Fn), sT = mmul_tn(gst, FmR);
-    m3 gF1 = load9(a1, stride, 15, p);
+    m3 gF1 = load9(a1, stride, 15, perm1 ? perm1[p] : p);   // frame f+1 re-sorted: slot perm1[p]
     m3 gF, gR;
 #pragma unroll
     for (int kk = 0; kk < 9; kk++) { gF.a[kk] = 2.f * S.mu * (sF.a[kk] + sT.a[kk]) + gF1.a[kk]; gR.a[kk] = -2.f * S.mu * sF.a[kk]; }
@@ -2239,8 +2243,6 @@ __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-}  // namespace
-
 // ------------------------------------------------------------------ slot re-sort
 // Every R substeps the particles' slots are re-sorted into the cell order of
 // the binning of the substep that ends at the boundary frame (a relabelling
@@ -2252,14 +2254,14 @@ __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
 // the fixed-order mode's binning gives a fixed-order re-sort.
 // Pass 1, one workgroup per scene: exclusive sums of the occupied blocks'
 // particle counts in block-index order.
-__global__ void __launch_bounds__(256) k_resort_blocks(BinArgs a, int* __restrict__ blk_pre) {
-    __shared__ int s_w[4];
+__global__ void __launch_bounds__(1024) k_resort_blocks(BinArgs a, int* __restrict__ blk_pre) {
+    __shared__ int s_w[16];
     const SceneC& S = a.scenes[blockIdx.x];
     int nbk = S.nb * S.nb * S.nb;
     int stamp = a.counters[CNT_STAMP];
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int run = 0;
-    for (int b0 = 0; b0 < nbk; b0 += 256) {
+    for (int b0 = 0; b0 < nbk; b0 += 1024) {
         int b = b0 + (int)threadIdx.x;
         int gb = S.blk_off + b;
         int c = (b < nbk && a.blk_stamp[gb] == stamp) ? a.blk_tot[gb] : 0;
@@ -2274,7 +2276,7 @@ __global__ void __launch_bounds__(256) k_resort_blocks(BinArgs a, int* __restric
         int off = run;
         for (int w = 0; w < wave; w++) off += s_w[w];
         if (b < nbk) blk_pre[gb] = off + incl - c;
-        run += s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        for (int w = 0; w < 16; w++) run += s_w[w];
         __syncthreads();
     }
 }
@@ -2293,18 +2295,6 @@ __global__ void __launch_bounds__(WG) k_resort_define(TileArgs ta, const int* __
         }
     }
 }
-// The adjoint of a boundary frame, in its (re-sorted) slot order, gathered
-// back into the order of the frame before: dst[row][p] = src[row][perm[p]].
-__global__ void __launch_bounds__(PPB) k_resort_adjoint(ChunkArgs ca, const float* __restrict__ src, float* __restrict__ dst,
-                                                       const int* __restrict__ perm) {
-    int4 ch = ca.chunks[blockIdx.x];
-    if ((int)threadIdx.x >= ch.z) return;
-    int p = ch.y + threadIdx.x;
-    int q = perm[p];
-    size_t stride = (size_t)ca.stride;
-#pragma unroll
-    for (int r = 0; r < NST; r++) dst[r * stride + p] = src[r * stride + q];
-}
 // caller's view: map[c] (slot of frame 0's particle c) through perm
 __global__ void k_resort_compose(int* __restrict__ map, const int* __restrict__ perm, const SceneC* __restrict__ scenes) {
     const SceneC& S = scenes[blockIdx.y];
@@ -2313,6 +2303,8 @@ __global__ void k_resort_compose(int* __restrict__ map, const int* __restrict__ 
         map[c] = perm[map[c]];
     }
 }
+
+}  // namespace
 
 // ==================================================================== host side
 struct mpm_ctx {
@@ -2348,7 +2340,6 @@ struct mpm_ctx {
     int resort = 0;
     int* rs_perm = nullptr;
     float* rs_fstage = nullptr;
-    float* rs_adj = nullptr;
     int *blk_tot = nullptr, *blk_pre = nullptr;
     bool is_bnd(int s) const { return resort > 0 && s > 0 && s % resort == 0 && s + 1 < max_frames; }
     int* perm_of(int s) const { return rs_perm + (size_t)(s / resort - 1) * stride; }
@@ -2497,8 +2488,8 @@ struct Layout {
     // records, per-block primitive adjoints, block bitmaps, integer mass grid
     size_t det_gn, det_garr, det_head, det_grp, det_padj, det_pbits, det_macc, det_pmd;
     // slot re-sort: permutations of the boundary frames, F of a boundary frame
-    // in the old order, one adjoint frame, per-block particle counts and sums
-    size_t off_resort, rs_perm, rs_fstage, rs_adj, rs_blk, n_bnd;
+    // in the old order, per-block particle counts and sums
+    size_t off_resort, rs_perm, rs_fstage, rs_blk, n_bnd;
     int n_bm_words;
     long long n_mass;
     // one bin slot: order, items, blk_stamp, blk_item0, blk_nitems, nblocks, nmap, counters
@@ -2603,7 +2594,6 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     size_t r = 0;
     L.rs_perm = r; r = align256(r + sizeof(int) * L.n_bnd * (size_t)stride);
     L.rs_fstage = r; r = align256(r + ff);
-    L.rs_adj = r; r = align256(r + ff);
     L.rs_blk = r; r = align256(r + sizeof(int) * 2 * (size_t)L.n_blocks);
     o = align256(o + r);
     L.total = o;
@@ -2661,7 +2651,7 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st, bool recompute = false) {
     c->aff_of[c->slot_of(f)] = f;
     if (run_bin(c, f, 1, st)) return -1;
     if (bnd && !recompute) {
-        hipLaunchKernelGGL(k_resort_blocks, dim3(c->n_scenes), dim3(256), 0, st, c->bargs(bs), c->blk_pre);
+        hipLaunchKernelGGL(k_resort_blocks, dim3(c->n_scenes), dim3(1024), 0, st, c->bargs(bs), c->blk_pre);
         hipLaunchKernelGGL(k_resort_define, dim3(c->tile_ctas), dim3(WG), 0, st, ta, (const int*)c->cell_start, (const int*)c->blk_pre,
                            c->perm_of(f + 1));
     }
@@ -2721,14 +2711,8 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     const float* in = c->frame_ptr(f);
     const float* in1 = c->frame_ptr(f + 1);
     const float* a1 = c->adj_ptr(f + 1);
-    // frame f+1 re-sorted: its adjoint gathered back into frame f's order,
-    // its positions read through the permutation
-    const int* perm1 = nullptr;
-    if (c->is_bnd(f + 1)) {
-        perm1 = c->perm_of(f + 1);
-        hipLaunchKernelGGL(k_resort_adjoint, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), a1, c->rs_adj, perm1);
-        a1 = c->rs_adj;
-    }
+    // frame f+1 re-sorted: its state and adjoint read through the permutation
+    const int* perm1 = c->is_bnd(f + 1) ? c->perm_of(f + 1) : nullptr;
     float* a0 = c->adj_ptr(f);
     int bs = c->bslot_of(f);
     // bins and grid of substep f kept from its forward run, or rebuilt here
@@ -2806,7 +2790,7 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
                          c->g_adjpm, pr, pr_lead); }
     { PhaseTimer t(c, MPM_PH_STRESS_BWD, st);
       hipLaunchKernelGGL(k_stress_bwd, dim3(c->n_chunks), dim3(PPB), 0, st, c->cargs(), in, (const float*)c->gaff, a1, a0,
-                         (const float*)c->usv_ptr(f), c->d_prims, c->n_prims, c->pstate, c->pvel, c->pstate_g, c->pvel_g, f); }
+                         (const float*)c->usv_ptr(f), c->d_prims, c->n_prims, c->pstate, c->pvel, c->pstate_g, c->pvel_g, f, perm1); }
     HIPCHK(hipGetLastError());
     if (fuse) c->fb_done = g;
     return 0;
@@ -2970,7 +2954,6 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->pmd = (float*)(w + L.off_det + L.det_pmd);
     c->rs_perm = (int*)(w + L.off_resort + L.rs_perm);
     c->rs_fstage = (float*)(w + L.off_resort + L.rs_fstage);
-    c->rs_adj = (float*)(w + L.off_resort + L.rs_adj);
     c->blk_tot = (int*)(w + L.off_resort + L.rs_blk);
     c->blk_pre = c->blk_tot + L.n_blocks;
 
