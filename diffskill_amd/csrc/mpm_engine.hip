@@ -2611,19 +2611,29 @@ int launch_fill(float* p, size_t n, hipStream_t st) {
 }
 
 // counting-sort frame f's particles into cell blocks and work items
-int run_bin(mpm_ctx* c, int f, int do_fk, hipStream_t st) {
-    const float* in = c->frame_ptr(f);
+// (stage 1: cell keys and ranks; stage 2: blocks, items, the sorted list)
+int run_bin_keys(mpm_ctx* c, int f, int do_fk, hipStream_t st) {
+    BinArgs ba = c->bargs(c->bslot_of(f));
+    { PhaseTimer t(c, MPM_PH_BIN, st);
+      hipLaunchKernelGGL(k_bin, dim3(c->n_chunks), dim3(PPB), 0, st, ba, (const float*)c->frame_ptr(f), do_fk, c->d_prims, c->n_prims,
+                         c->pstate, c->pvel, f); }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+int run_bin_lists(mpm_ctx* c, int f, hipStream_t st) {
     int bs = c->bslot_of(f);
     const mpm_ctx::BinSlot& B = c->bslots[bs];
     BinArgs ba = c->bargs(bs);
-    { PhaseTimer t(c, MPM_PH_BIN, st);
-      hipLaunchKernelGGL(k_bin, dim3(c->n_chunks), dim3(PPB), 0, st, ba, in, do_fk, c->d_prims, c->n_prims, c->pstate, c->pvel, f); }
     { PhaseTimer t(c, MPM_PH_BLOCKS, st);
       hipLaunchKernelGGL(k_blocks, dim3(c->grid_ctas), dim3(256), 0, st, ba, B.nblocks, B.nmap, B.nrec, c->rec_cap); }
     { PhaseTimer t(c, MPM_PH_SCATTER, st);
       hipLaunchKernelGGL(k_scatter, dim3(c->n_chunks), dim3(PPB), 0, st, ba, B.nrec, c->rec_cap); }
     HIPCHK(hipGetLastError());
     return 0;
+}
+int run_bin(mpm_ctx* c, int f, int do_fk, hipStream_t st) {
+    if (run_bin_keys(c, f, do_fk, st)) return -1;
+    return run_bin_lists(c, f, st);
 }
 
 // forward substep f -> f+1 (writes frame f+1 wherever its slot is).  When
@@ -2637,10 +2647,13 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st, bool recompute = false) {
     int bs = c->bslot_of(f);
     TileArgs ta = c->targs(bs);
     float* aff = c->aff_ptr(f);
-    // profiling serialises the two branches so that every kernel's event
-    // interval is its own
+    // The binning chain is the longer branch: its first kernel runs alone,
+    // then the constitutive update (which needs no binning) forks onto the
+    // second stream beside the rest of the chain.  Profiling serialises the
+    // two branches so that every kernel's event interval is its own.
     bool fork = !c->prof;
     hipStream_t sst = fork ? c->side : st;
+    if (run_bin_keys(c, f, 1, st)) return -1;
     if (fork) {
         HIPCHK(hipEventRecord(c->ev_fork, st));
         HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
@@ -2649,7 +2662,7 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st, bool recompute = false) {
       hipLaunchKernelGGL(k_stress, dim3(c->n_chunks), dim3(PPB), 0, sst, c->cargs(), in, bnd ? c->rs_fstage : out, aff, c->usv_ptr(f)); }
     if (fork) HIPCHK(hipEventRecord(c->ev_join, c->side));
     c->aff_of[c->slot_of(f)] = f;
-    if (run_bin(c, f, 1, st)) return -1;
+    if (run_bin_lists(c, f, st)) return -1;
     if (bnd && !recompute) {
         hipLaunchKernelGGL(k_resort_blocks, dim3(c->n_scenes), dim3(1024), 0, st, c->bargs(bs), c->blk_pre);
         hipLaunchKernelGGL(k_resort_define, dim3(c->tile_ctas), dim3(WG), 0, st, ta, (const int*)c->cell_start, (const int*)c->blk_pre,
