@@ -50,7 +50,7 @@ EXPORTS = [
     "mpm_update_prims", "mpm_copy_frame", "mpm_set_profiling", "mpm_profile_read", "mpm_last_error",
     "mpm_set_launch", "mpm_get_launch", "mpm_min_dist_cols", "mpm_min_dist", "mpm_min_dist_bwd",
     "mpm_collision_bytes", "mpm_set_collisions", "mpm_get_collisions", "mpm_set_deterministic", "mpm_get_deterministic",
-    "mpm_set_resort", "mpm_get_resort", "mpm_slot_order",
+    "mpm_resort_bytes", "mpm_set_resort", "mpm_get_resort", "mpm_slot_order",
     "pc_nearest", "pc_nearest_noalloc", "pc_last_error", "pc_sinkhorn_work_bytes", "pc_softmin", "pc_plan_grad",
     "pc_sinkhorn_step_work_bytes", "pc_sinkhorn_step",
 ]
@@ -110,7 +110,8 @@ def load():
         "mpm_profile_read": (i32, [vp, vp, vp, i32]),
         "mpm_set_deterministic": (i32, [vp, i32]),
         "mpm_get_deterministic": (i32, [vp]),
-        "mpm_set_resort": (i32, [vp, i32]),
+        "mpm_resort_bytes": (i64, [vp, i32]),
+        "mpm_set_resort": (i32, [vp, i32, vp, i64]),
         "mpm_get_resort": (i32, [vp]),
         "mpm_slot_order": (i32, [vp, i32, vp, vp]),
         "mpm_last_error": (ctypes.c_char_p, []),

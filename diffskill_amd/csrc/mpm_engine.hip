@@ -232,7 +232,7 @@ constexpr int MAXSEG = 4;    // distinct cells per wave the mass grid reduces th
 constexpr int NXCD = 8;        // MI355X: 8 XCDs, workgroup b dispatched to XCD b % 8
 constexpr int DET_GMAX = 16;   // record-array entries per cell (fixed-order ranks)
 constexpr int FB_LEAD = 64;    // leading workgroups that carry an overflow fallback
-constexpr int RESORT_MIN = 8;  // shortest slot re-sort interval (substeps) the workspace holds permutations for
+constexpr int RESORT_MIN = 8;  // shortest slot re-sort interval (substeps)
 enum { CNT_NBLK = 1, CNT_PART = 2, CNT_ITEMS = 3, CNT_STAMP = 4, CNT_X0 = 8, CNT_XB0 = 16 };
 // global binning state: the pass stamp base and the occupied-block count of
 // the pass in flight (both advanced/reset by the pass's last kernel)
@@ -281,7 +281,7 @@ struct BinArgs {
     // on one XCD; xlist lists the items grouped by tag
     int* blk_rep;
     int* xlist;
-    int* blk_tot;    // particles of each occupied block this pass
+    int* blk_tot;    // particles of each occupied block this pass (slot re-sort on), or null
 };
 
 DEV unsigned long long lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nbl
             a.blk_stamp[gb] = stamp;
             a.blk_item0[gb] = ib;
             a.blk_nitems[gb] = nit;
-            a.blk_tot[gb] = total;
+            if (a.blk_tot) a.blk_tot[gb] = total;
         }
     }
 }
@@ -2487,9 +2487,7 @@ struct Layout {
     // fixed-order mode buffers (within off_det): cell list heads, group
     // records, per-block primitive adjoints, block bitmaps, integer mass grid
     size_t det_gn, det_garr, det_head, det_grp, det_padj, det_pbits, det_macc, det_pmd;
-    // slot re-sort: permutations of the boundary frames, F of a boundary frame
-    // in the old order, per-block particle counts and sums
-    size_t off_resort, rs_perm, rs_fstage, rs_blk, n_bnd;
+
     int n_bm_words;
     long long n_mass;
     // one bin slot: order, items, blk_stamp, blk_item0, blk_nitems, nblocks, nmap, counters
@@ -2589,13 +2587,6 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     // lie past the stride (its first slot plus 256)
     L.det_pmd = d; d = align256(d + sizeof(float) * 8 * MAX_UNITS * (size_t)(stride / 64 + PPB / 64));
     o = align256(o + d);
-    L.off_resort = o;
-    L.n_bnd = (size_t)std::max(1, (max_frames - 2) / RESORT_MIN);
-    size_t r = 0;
-    L.rs_perm = r; r = align256(r + sizeof(int) * L.n_bnd * (size_t)stride);
-    L.rs_fstage = r; r = align256(r + ff);
-    L.rs_blk = r; r = align256(r + sizeof(int) * 2 * (size_t)L.n_blocks);
-    o = align256(o + r);
     L.total = o;
     return L;
 }
@@ -2965,10 +2956,6 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     c->pbits = (unsigned*)(w + L.off_det + L.det_pbits);
     c->macc = (unsigned long long*)(w + L.off_det + L.det_macc);
     c->pmd = (float*)(w + L.off_det + L.det_pmd);
-    c->rs_perm = (int*)(w + L.off_resort + L.rs_perm);
-    c->rs_fstage = (float*)(w + L.off_resort + L.rs_fstage);
-    c->blk_tot = (int*)(w + L.off_resort + L.rs_blk);
-    c->blk_pre = c->blk_tot + L.n_blocks;
 
     // host tables
     std::vector<int4> chunks;
@@ -3097,11 +3084,47 @@ int mpm_set_deterministic(mpm_ctx* ctx, int on) {
 
 int mpm_get_deterministic(mpm_ctx* ctx) { return ctx ? (ctx->det ? 1 : 0) : fail("mpm_get_deterministic: null context"); }
 
-int mpm_set_resort(mpm_ctx* ctx, int substeps) {
+// the caller-owned buffer of a slot re-sort every `substeps` substeps: the
+// permutation of every boundary frame (one int per slot each), a staging
+// frame for F of a boundary frame in the old order, per-block particle
+// counts and their per-scene sums
+struct ResortLayout { size_t perm, fstage, blk, total; };
+static ResortLayout resort_layout(const mpm_ctx* c, int substeps) {
+    ResortLayout r;
+    size_t n_bnd = (size_t)std::max(0, (c->max_frames - 2) / substeps);
+    size_t o = 0;
+    r.perm = o; o = align256(o + sizeof(int) * n_bnd * (size_t)c->stride);
+    r.fstage = o; o = align256(o + c->frame_floats() * sizeof(float));
+    r.blk = o; o = align256(o + sizeof(int) * 2 * (size_t)c->n_blocks);
+    r.total = o;
+    return r;
+}
+
+int64_t mpm_resort_bytes(mpm_ctx* ctx, int substeps) {
+    if (!ctx) return fail("mpm_resort_bytes: null context");
+    if (substeps < RESORT_MIN)
+        return fail("mpm_resort_bytes: interval must be at least " + std::to_string(RESORT_MIN) + " substeps");
+    return (int64_t)resort_layout(ctx, substeps).total;
+}
+
+int mpm_set_resort(mpm_ctx* ctx, int substeps, void* buffer, int64_t buffer_bytes) {
     if (!ctx) return fail("mpm_set_resort: null context");
     if (substeps != 0 && substeps < RESORT_MIN)
         return fail("mpm_set_resort: interval must be 0 (off) or at least " + std::to_string(RESORT_MIN) + " substeps");
-    if (substeps == ctx->resort) return 0;
+    if (substeps != 0) {
+        ResortLayout r = resort_layout(ctx, substeps);
+        if (!buffer || buffer_bytes < (int64_t)r.total)
+            return fail("mpm_set_resort: buffer of " + std::to_string(buffer_bytes) + " bytes, " + std::to_string(r.total) + " needed");
+        char* b = static_cast<char*>(buffer);
+        ctx->rs_perm = (int*)(b + r.perm);
+        ctx->rs_fstage = (float*)(b + r.fstage);
+        ctx->blk_tot = (int*)(b + r.blk);
+        ctx->blk_pre = ctx->blk_tot + ctx->n_blocks;
+    } else {
+        ctx->rs_perm = nullptr;
+        ctx->rs_fstage = nullptr;
+        ctx->blk_tot = ctx->blk_pre = nullptr;
+    }
     ctx->resort = substeps;
     // frames after 0 were written in the previous slot orders
     ctx->valid_hi = std::min(ctx->valid_hi, 0);

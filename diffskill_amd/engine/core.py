@@ -268,8 +268,16 @@ class Engine:
     def set_resort(self, substeps):
         """Re-sort the particles' slots into cell order every ``substeps``
         substeps (0: off); invalidates every frame after 0 (see
-        mpm_set_resort)."""
-        _lib.check(self.L.mpm_set_resort(self.ctx, int(substeps)), "mpm_set_resort")
+        mpm_set_resort).  The engine's buffer for it is held here."""
+        substeps = int(substeps)
+        buf = None
+        if substeps:
+            nbytes = self.L.mpm_resort_bytes(self.ctx, substeps)
+            _lib.check(0 if nbytes > 0 else -1, "mpm_resort_bytes")
+            buf = torch.empty(int(nbytes), dtype=torch.uint8, device="cuda")
+        _lib.check(self.L.mpm_set_resort(self.ctx, substeps, ctypes.c_void_p(buf.data_ptr() if buf is not None else 0),
+                                         int(buf.numel() if buf is not None else 0)), "mpm_set_resort")
+        self._resort_buf = buf
 
     @property
     def resort(self):

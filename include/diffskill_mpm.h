@@ -262,12 +262,16 @@ int mpm_get_deterministic(mpm_ctx *ctx);
  * checkpointed recompute applies the orders the first pass defined, and the
  * results are those of the unsorted run up to float summation order (bitwise
  * reproducible in the fixed-order mode).  0 (the default) turns it off.
- * Setting it invalidates every frame after 0.  mpm_frame and the other
- * per-particle accessors of frame f then see frame f's own slot order;
- * mpm_slot_order writes, for every slot c of frame 0, the slot its particle
- * holds in frame f (stride ints, device memory; synchronises the stream).
- * mpm_copy_frame refuses frames of different slot orders. */
-int mpm_set_resort(mpm_ctx *ctx, int substeps);
+ * The caller passes a device buffer of at least mpm_resort_bytes(ctx, R)
+ * bytes (the boundaries' permutations, a staging frame, per-block counts),
+ * kept until the re-sort is turned off or the context destroyed.  Setting it
+ * invalidates every frame after 0.  mpm_frame and the other per-particle
+ * accessors of frame f then see frame f's own slot order; mpm_slot_order
+ * writes, for every slot c of frame 0, the slot its particle holds in frame f
+ * (stride ints, device memory; synchronises the stream).  mpm_copy_frame
+ * refuses frames of different slot orders. */
+int64_t mpm_resort_bytes(mpm_ctx *ctx, int substeps);
+int mpm_set_resort(mpm_ctx *ctx, int substeps, void *buffer, int64_t buffer_bytes);
 int mpm_get_resort(mpm_ctx *ctx);
 int mpm_slot_order(mpm_ctx *ctx, int f, int *map, void *stream);
 
