@@ -115,3 +115,24 @@ def test_resort_refuses_copies_across_slot_orders():
     with pytest.raises(RuntimeError, match="slot orders"):
         b.engine.copy_frame(T, 0)
     b.engine.copy_frame(1, 2)   # same slot order: allowed
+
+
+def test_resort_arguments_are_checked():
+    """Intervals below 8 substeps and buffers smaller than mpm_resort_bytes
+    are refused; turning the re-sort off needs no buffer."""
+    import ctypes
+    from diffskill_amd.batch import SceneBatch
+    w = workload()
+    b = SceneBatch(w["scenes"], w["H"] * w["substeps"] + 1)
+    e = b.engine
+    with pytest.raises(RuntimeError, match="at least 8"):
+        e.set_resort(4)
+    need = e.L.mpm_resort_bytes(e.ctx, 10)
+    assert need > 0
+    small = torch.empty(int(need) - 256, dtype=torch.uint8, device="cuda")
+    assert e.L.mpm_set_resort(e.ctx, 10, ctypes.c_void_p(small.data_ptr()), int(small.numel())) != 0
+    assert e.resort == 0
+    e.set_resort(10)
+    assert e.resort == 10
+    e.set_resort(0)
+    assert e.resort == 0
