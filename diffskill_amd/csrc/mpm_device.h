@@ -603,7 +603,8 @@ DEV double box_sdf_d(const PrimC& P, d3 p, d3* g) {
     double L = sqrt(a0 * a0 + a1 * a1 + a2 * a2 + (double)LEN_EPS_SHAPE);
     double m12 = dmax(q1, q2), m = dmax(q0, m12);
     if (g) {
-        double g0 = (0.0 < q0) ? a0 / L : 0.0, g1 = (0.0 < q1) ? a1 / L : 0.0, g2 = (0.0 < q2) ? a2 / L : 0.0;
+        double iL = 1.0 / L;
+        double g0 = (0.0 < q0) ? a0 * iL : 0.0, g1 = (0.0 < q1) ? a1 * iL : 0.0, g2 = (0.0 < q2) ? a2 * iL : 0.0;
         if (m < 0.0) {
             if (m12 < q0) g0 += 1.0;
             else if (q2 < q1) g1 += 1.0;
@@ -647,7 +648,9 @@ DEV double fd_sdf_d(const PrimC& P, int sh, d3 p, d3* g) {
     if (g) *g = (b < a) ? g1 : g2;
     return dmax(a, b);
 }
-DEV v3 fd_normal_d(const PrimC& P, int sh, v3 pf) {
+// the normalised central difference; *Lout (when given) receives the
+// difference vector's length L, which the reverse needs
+DEV v3 fd_normal_d(const PrimC& P, int sh, v3 pf, float* Lout = nullptr) {
     const double d = (double)1e-4f;
     d3 p = {pf.x, pf.y, pf.z}, m;
 #pragma unroll
@@ -658,30 +661,29 @@ DEV v3 fd_normal_d(const PrimC& P, int sh, v3 pf) {
         dsetc(m, i, (0.5 / d) * (fd_sdf_d(P, sh, inc, nullptr) - fd_sdf_d(P, sh, dec, nullptr)));
     }
     double L = sqrt(m.x * m.x + m.y * m.y + m.z * m.z + (double)LEN_EPS_SHAPE);
+    if (Lout) *Lout = (float)L;
     return mk((float)(m.x / L), (float)(m.y / L), (float)(m.z / L));
 }
-DEV v3 fd_normal_rev_d(const PrimC& P, int sh, v3 pf, v3 gnf) {
+// adjoint of fd_normal_d from its forward result n = m / L and L:
+// dn/dm = (I - n n^T) / L, and dm_i/dp = (grad sdf(p + d e_i) - grad sdf(p - d e_i)) * 0.5 / d,
+// the sdf gradients in fp64 (their difference cancels to O(d)), one axis at a time
+DEV v3 fd_normal_rev_d(const PrimC& P, int sh, v3 pf, v3 n, float L, v3 gnf) {
     const double d = (double)1e-4f;
-    d3 p = {pf.x, pf.y, pf.z}, m, gi[3], gd[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        d3 inc = p, dec = p;
-        dsetc(inc, i, dcomp(p, i) + d);
-        dsetc(dec, i, dcomp(p, i) - d);
-        double a = fd_sdf_d(P, sh, inc, &gi[i]);
-        double b = fd_sdf_d(P, sh, dec, &gd[i]);
-        dsetc(m, i, (0.5 / d) * (a - b));
-    }
-    double gn[3] = {gnf.x, gnf.y, gnf.z};
-    double L = sqrt(m.x * m.x + m.y * m.y + m.z * m.z + (double)LEN_EPS_SHAPE);
-    double dd = m.x * gn[0] + m.y * gn[1] + m.z * gn[2], L3 = L * L * L;
+    d3 p = {pf.x, pf.y, pf.z};
+    double nd = (double)dot(n, gnf);
+    double inv = (0.5 / d) / (double)L;
     double gp[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        double gm = (gn[i] / L - dcomp(m, i) * dd / L3) * (0.5 / d);
-        gp[0] += gm * (gi[i].x - gd[i].x);
-        gp[1] += gm * (gi[i].y - gd[i].y);
-        gp[2] += gm * (gi[i].z - gd[i].z);
+        d3 inc = p, dec = p, gi, gd;
+        dsetc(inc, i, dcomp(p, i) + d);
+        dsetc(dec, i, dcomp(p, i) - d);
+        fd_sdf_d(P, sh, inc, &gi);
+        fd_sdf_d(P, sh, dec, &gd);
+        double gm = ((double)comp(gnf, i) - (double)comp(n, i) * nd) * inv;
+        gp[0] += gm * (gi.x - gd.x);
+        gp[1] += gm * (gi.y - gd.y);
+        gp[2] += gm * (gi.z - gd.z);
     }
     return mk((float)gp[0], (float)gp[1], (float)gp[2]);
 }
@@ -689,7 +691,9 @@ DEV v3 fd_normal_rev_d(const PrimC& P, int sh, v3 pf, v3 gnf) {
 // normals: analytic for sphere / capsule / chopsticks / cylinder / torus,
 // central FD with the float32 step 1e-4 for box / prism / knife
 // (primitives.py:382-393,720-731,796-807)
-DEV v3 local_normal(const PrimC& P, int sh, float gap, v3 p) {
+// *L (when given): the FD length of box-like shapes (1 for the others)
+DEV v3 local_normal(const PrimC& P, int sh, float gap, v3 p, float* L = nullptr) {
+    if (L) *L = 1.f;
     if (sh == SH_CAPSULE) return capsule_normal(P.h, p);
     if (sh == SH_CYL) return cylinder_normal(P.h, P.r, p);
     if (sh == SH_TORUS) return torus_normal(P.h, p);
@@ -700,9 +704,10 @@ DEV v3 local_normal(const PrimC& P, int sh, float gap, v3 p) {
         float a = capsule_sdf(P.h, P.r, pa, nullptr), b = capsule_sdf(P.h, P.r, pb, nullptr);
         return (a <= b) ? capsule_normal(P.h, pa) : capsule_normal(P.h, pb);
     }
-    return fd_normal_d(P, sh, p);
+    return fd_normal_d(P, sh, p, L);
 }
-DEV v3 local_normal_rev(const PrimC& P, int sh, float gap, v3 p, v3 gn, float* ggap) {
+// n, L: local_normal's result and FD length at p
+DEV v3 local_normal_rev(const PrimC& P, int sh, float gap, v3 p, v3 gn, float* ggap, v3 n, float L) {
     if (sh == SH_CAPSULE) return capsule_normal_rev(P.h, p, gn);
     if (sh == SH_CYL) return cylinder_normal_rev(P.h, P.r, p, gn);
     if (sh == SH_TORUS) return torus_normal_rev(P.h, p, gn);
@@ -719,7 +724,7 @@ DEV v3 local_normal_rev(const PrimC& P, int sh, float gap, v3 p, v3 gn, float* g
         else { g3 = capsule_normal_rev(P.h, pb, gn); *ggap += g3.x / 2.f; }
         return g3;
     }
-    return fd_normal_rev_d(P, sh, p, gn);
+    return fd_normal_rev_d(P, sh, p, n, L, gn);
 }
 
 // ------------------------------------------------------------------ primitive frames
@@ -795,19 +800,27 @@ DEV v3 unit_sdf_rev_x(const PrimC& P, const Unit& u, const PS& s, v3 x, float gd
     return gx;
 }
 // world normal qrot(rot, _normal(local)) (primive_base.py:81-85)
-DEV v3 unit_normal(const PrimC& P, const Unit& u, const PS& s, v3 x) {
-    v3 n = local_normal(P, u.sh, s.gap, to_local(s, u, x));
+// *nl, *L (when given): the local normal and its FD length, for the reverse
+DEV v3 unit_normal(const PrimC& P, const Unit& u, const PS& s, v3 x, v3* nl = nullptr, float* L = nullptr) {
+    v3 n = local_normal(P, u.sh, s.gap, to_local(s, u, x), L);
+    if (nl) *nl = n;
     return u.norot ? n : qrot(s.rot, n);
 }
-DEV void unit_normal_rev(const PrimC& P, const Unit& u, const PS& s, v3 x, v3 gD, PSg* g) {
+// nl, L: unit_normal's local normal and FD length at x
+DEV void unit_normal_rev(const PrimC& P, const Unit& u, const PS& s, v3 x, v3 gD, PSg* g, v3 nl, float L) {
     v3 p = to_local(s, u, x);
     v3 gn;
     if (u.norot) gn = gD;
-    else gn = qrot_rev(s.rot, local_normal(P, u.sh, s.gap, p), gD, &g->rot);
+    else gn = qrot_rev(s.rot, nl, gD, &g->rot);
     float ggap = 0.f;
-    v3 gp = local_normal_rev(P, u.sh, s.gap, p, gn, &ggap);
+    v3 gp = local_normal_rev(P, u.sh, s.gap, p, gn, &ggap, nl, L);
     if (u.sh == SH_CHOP) g->gap += ggap;
     to_local_rev(s, u, x, gp, g);
+}
+DEV void unit_normal_rev(const PrimC& P, const Unit& u, const PS& s, v3 x, v3 gD, PSg* g) {
+    float L;
+    v3 nl = local_normal(P, u.sh, s.gap, to_local(s, u, x), &L);
+    unit_normal_rev(P, u, s, x, gD, g, nl, L);
 }
 // collider_v (primive_base.py:87-94, primitives.py:497-502)
 DEV v3 unit_collider_v(const Unit& u, const PS& s0, const PS& s1, v3 x, float dt) {
@@ -841,8 +854,8 @@ DEV int prim_units(const PrimC& P, Unit* u) {
 // Primitive.collide (primive_base.py:96-120) / Gripper.collide2 (primitives.py:508-536)
 struct CollTr {
     int branch, flag;
-    float dist, ex, infl, nc, mn, vtn, mx;
-    v3 D, vc, u, vt, vt2;
+    float dist, ex, infl, nc, mn, vtn, mx, Ln;
+    v3 D, vc, u, vt, vt2, nl;   // nl, Ln: the local normal and its FD length
 };
 DEV v3 unit_collide(const PrimC& P, const Unit& un, const PS& s0, const PS& s1, v3 x, float dt, v3 v, CollTr* tr) {
     CollTr t;
@@ -852,7 +865,7 @@ DEV v3 unit_collide(const PrimC& P, const Unit& un, const PS& s0, const PS& s1, 
     t.branch = (P.softness > 0.f && t.infl > 0.1f) || t.dist <= 0.f;
     v3 out = v;
     if (t.branch) {
-        t.D = unit_normal(P, un, s0, x);
+        t.D = unit_normal(P, un, s0, x, &t.nl, &t.Ln);
         t.vc = unit_collider_v(un, s0, s1, x, dt);
         t.u = v - t.vc;
         t.nc = dot(t.u, t.D);
@@ -898,7 +911,7 @@ DEV v3 unit_collide_rev(const PrimC& P, const Unit& un, const PS& s0, const PS& 
     gD += gnc * t.u;
     v3 gv = gu;
     gvc -= gu;
-    unit_normal_rev(P, un, s0, x, gD, g0);
+    unit_normal_rev(P, un, s0, x, gD, g0, t.nl, t.Ln);
     unit_collider_v_rev(un, s0, s1, x, dt, gvc, g0, g1);
     if (t.ex < 1.f) {
         float gdist = ginfl * t.ex * (-P.softness);

@@ -382,7 +382,8 @@ __global__ void __launch_bounds__(PPB) k_bin(BinArgs a, const float* __restrict_
 // workgroup takes 4 blocks at a time and claims their ranges with one 64-bit
 // atomic on the (particles, items) counter pair and one on the node count.
 // Node-block records: for active node block e < rec_cap, NREC ints
-// {gb, scene, -, -, item0[8], nitems[8]} with the work-item ranges of the 8 cell
+// {gb, scene, scene's first primitive, its primitive count, item0[8], nitems[8]}
+// with the work-item ranges of the 8 cell
 // blocks whose 6x6x6 footprint covers it (cell block B - (d>>2&1, d>>1&1, d&1);
 // 0 items when it is not occupied this pass), so the grid phases read one record
 // instead of chasing block list -> scene -> block stamps -> item ranges.
@@ -457,6 +458,8 @@ __global__ void __launch_bounds__(256) k_blocks(BinArgs a, int* __restrict__ nbl
                 int* r = nrec + (size_t)pos * NREC;
                 r[0] = nb;
                 r[1] = sc;
+                r[2] = S.prim_off;
+                r[3] = S.n_prims;
 #pragma unroll
                 for (int d = 0; d < 8; d++) {
                     int P0 = ni - ((d >> 2) & 1), P1 = nj - ((d >> 1) & 1), P2 = nk - (d & 1);
@@ -820,11 +823,13 @@ DEV int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 // work-item ranges (i0, ni) of the 8 covering cell blocks -- from its record
 // when e < rec_cap, else from the block list and the block stamps.  All of it
 // is wave-uniform (scalar loads).
-struct BlockMeta { int gb, sc, i0[8], ni[8]; };
+struct BlockMeta { int gb, sc, poff, np, i0[8], ni[8]; };
 DEV void read_rec(const GridArgs& ga, int e, BlockMeta& m) {
     const int* r = ga.nrec + (size_t)e * NREC;
     m.gb = uniform(r[0]);
     m.sc = uniform(r[1]);
+    m.poff = uniform(r[2]);
+    m.np = uniform(r[3]);
 #pragma unroll
     for (int d = 0; d < 8; d++) { m.i0[d] = uniform(r[4 + d]); m.ni[d] = uniform(r[12 + d]); }
 }
@@ -836,6 +841,8 @@ DEV void block_meta(const GridArgs& ga, int e, int stamp, BlockMeta& m) {
     m.gb = uniform(ga.nblocks[e]);
     m.sc = uniform(ga.blk_scene[m.gb]);
     const SceneC& S = ga.scenes[m.sc];
+    m.poff = uniform(S.prim_off);
+    m.np = uniform(S.n_prims);
     int B0, B1, B2;
     block_origin(S, m.gb, B0, B1, B2);
     int pgb[8], st[8];
@@ -852,6 +859,40 @@ DEV void block_meta(const GridArgs& ga, int e, int stamp, BlockMeta& m) {
         m.i0[d] = uniform(hit ? ga.blk_item0[pgb[d]] : 0);
         m.ni[d] = uniform(hit ? ga.blk_nitems[pgb[d]] : 0);
     }
+}
+
+// A scene's constants, its primitives and their states at frames f and f+1,
+// staged in LDS by one wave for the node blocks of that scene: the node update
+// reads them many times per node and under divergent branches, where they
+// would otherwise be vector loads from memory, each a full memory latency on
+// the wave's critical path.
+static_assert(sizeof(SceneC) == 32 * 4 && sizeof(PrimC) == 40 * 4, "staged as whole dwords");
+struct SceneStage {
+    SceneC sc;
+    PrimC pr[MAX_UNITS];
+    float ps[2][MAX_UNITS * 8];
+};
+// (the block's record carries the scene's primitive range, so every load here
+// is issued at once)
+DEV void stage_scene(SceneStage& st, const GridArgs& ga, const BlockMeta& m, int lane) {
+    // the previous block's reads of the stage are issued before it is rewritten
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const int* src = reinterpret_cast<const int*>(ga.scenes + m.sc);
+    int v = lane < 32 ? src[lane] : 0;
+    int poff = m.poff, np = m.np;
+    if (lane < 32) reinterpret_cast<int*>(&st.sc)[lane] = v;
+    const int* ps = reinterpret_cast<const int*>(ga.prims + poff);
+    int* dp = reinterpret_cast<int*>(st.pr);
+    for (int k = lane; k < np * 40; k += 64) dp[k] = ps[k];
+    const float* s0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + poff) * 8;
+    const float* s1 = s0 + (size_t)ga.n_prims_total * 8;
+    for (int k = lane; k < np * 8; k += 64) {
+        st.ps[0][k] = s0[k];
+        st.ps[1][k] = s1[k];
+    }
+    // LDS operations of one wave complete in order: the stage is visible to
+    // every lane of the wave once the compiler keeps the reads after the writes
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
 // owner-computes gather: the node's value is the sum, in a fixed order, of the
@@ -901,11 +942,12 @@ DEV float4 gather_tiles(const BlockMeta& m, int lane, const float4* __restrict__
 // With only_overflow set it runs only when the kept grid overflowed (the
 // adjoint then recomputes the grid instead of reading the kept one).
 DEV void grid_op_body(const GridArgs& ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm, float4* __restrict__ g_v,
-                      int only_overflow, int lead = 0, int nwg = 0) {
+                      SceneStage* stages, int only_overflow, int lead = 0, int nwg = 0) {
     int lane = threadIdx.x & 63;
     int wave = threadIdx.x >> 6;
+    int wpg = (int)blockDim.x >> 6;
     int nb = nwg > 0 ? nwg : (int)gridDim.x - lead;
-    int e = uniform(((int)blockIdx.x - lead) * 4 + wave);
+    int e = uniform(((int)blockIdx.x - lead) * wpg + wave);
     // the first block's record is read together with the counters (a record
     // beyond the active count is stale and left unused)
     BlockMeta m;
@@ -914,20 +956,18 @@ DEV void grid_op_body(const GridArgs& ga, const float4* __restrict__ tiles, floa
     if (only_overflow && kept_fits(ga.counters, ga.keep_cap)) return;
     int n_active = ga.counters[CNT_NBLK];
     int stamp = ga.counters[CNT_STAMP];
-    for (; e < n_active; e += nb * 4) {
+    SceneStage& stg = stages[wave];
+    for (; e < n_active; e += nb * wpg) {
         if (!have) block_meta(ga, e, stamp, m);
         have = false;
-        const SceneC& S = ga.scenes[m.sc];
+        stage_scene(stg, ga, m, lane);
+        const SceneC& S = stg.sc;
         int B0, B1, B2, I0, I1, I2;
         long long node;
         node_coords(S, m.gb, lane, B0, B1, B2, I0, I1, I2, node);
         float4 pm = gather_tiles<8>(m, lane, tiles);
         v3 v = mk(0.f, 0.f, 0.f);
-        if (pm.w > 1e-12f && in_grid(S, I0, I1, I2)) {
-            const float* ps0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
-            const float* ps1 = ga.pstate + ((size_t)(ga.f + 1) * ga.n_prims_total + S.prim_off) * 8;
-            v = grid_node(S, ga.prims + S.prim_off, ps0, ps1, I0, I1, I2, pm, nullptr);
-        }
+        if (pm.w > 1e-12f && in_grid(S, I0, I1, I2)) v = grid_node(S, stg.pr, stg.ps[0], stg.ps[1], I0, I1, I2, pm, nullptr);
         float4 v4 = make_float4(v.x, v.y, v.z, 0.f);
         g_pm[node] = pm;
         g_v[node] = v4;
@@ -940,14 +980,16 @@ DEV void grid_op_body(const GridArgs& ga, const float4* __restrict__ tiles, floa
 // the forward substep's grid operator
 __global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
                                                  float4* __restrict__ g_v) {
-    grid_op_body(ga, tiles, g_pm, g_v, 0);
+    __shared__ SceneStage stages[4];
+    grid_op_body(ga, tiles, g_pm, g_v, stages, 0);
 }
 // the same operator rebuilding the grid of a substep for its adjoint (the
 // grid was not kept, or, with only_overflow, its active list outgrew the
 // kept grid): a kernel of its own so that profiles separate the two
 __global__ void __launch_bounds__(256) k_grid_op_rebuild(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
                                                          float4* __restrict__ g_v, int only_overflow) {
-    grid_op_body(ga, tiles, g_pm, g_v, only_overflow);
+    __shared__ SceneStage stages[4];
+    grid_op_body(ga, tiles, g_pm, g_v, stages, only_overflow);
 }
 
 // load the 6x6x6 node tile of a cell block from a node grid into LDS
@@ -1236,17 +1278,19 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
 
 // reverse of grid_op: adjoint of (momentum, mass) per node, plus the adjoint
 // of the primitive states of frames f and f+1 (wave-reduced, then atomics).
-__global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* __restrict__ g_pm, const float4* __restrict__ tiles_v,
-                                                     float4* __restrict__ g_adjpm, float* __restrict__ gpstate, int use_kept,
-                                                     GridRebuild rb) {
+// One wave per workgroup and per node block: at two waves per SIMD (its
+// register count), a wave slot frees as soon as its own block is done.
+__global__ void __launch_bounds__(64) k_grid_op_bwd(GridArgs ga, const float4* __restrict__ g_pm, const float4* __restrict__ tiles_v,
+                                                    float4* __restrict__ g_adjpm, float* __restrict__ gpstate, int use_kept,
+                                                    GridRebuild rb) {
+    __shared__ SceneStage stage[1];
     if ((int)blockIdx.x < rb.lead) {
-        grid_op_body(rb.ga, rb.tiles, rb.g_pm, rb.g_v, 1, 0, rb.lead);
+        grid_op_body(rb.ga, rb.tiles, rb.g_pm, rb.g_v, stage, 1, 0, rb.lead);
         return;
     }
     int lane = threadIdx.x & 63;
-    int wave = threadIdx.x >> 6;
     int nb = (int)gridDim.x - rb.lead;
-    int e = uniform(((int)blockIdx.x - rb.lead) * 4 + wave);
+    int e = uniform((int)blockIdx.x - rb.lead);
     // the first block's record is read together with the counters (a record
     // beyond the active count is stale and left unused)
     BlockMeta m;
@@ -1255,25 +1299,27 @@ __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* 
     int n_active = ga.counters[CNT_NBLK];
     bool kept = use_kept && kept_fits(ga.counters, ga.keep_cap);
     int stamp = ga.counters[CNT_STAMP];
-    for (; e < n_active; e += nb * 4) {
+    for (; e < n_active; e += nb) {
         float4 pm = kept ? ga.keep_pm[(size_t)e * NODES_PER_BLK + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
         if (!have) block_meta(ga, e, stamp, m);
         have = false;
-        const SceneC& S = ga.scenes[m.sc];
+        stage_scene(stage[0], ga, m, lane);
+        const SceneC& S = stage[0].sc;
         int B0, B1, B2, I0, I1, I2;
         long long node;
         node_coords(S, m.gb, lane, B0, B1, B2, I0, I1, I2, node);
         if (!kept) pm = g_pm[node];
         float4 gv4 = gather_tiles<8>(m, lane, tiles_v);
         bool act = pm.w > 1e-12f && in_grid(S, I0, I1, I2);
-        const float* ps0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
-        const float* ps1 = ga.pstate + ((size_t)(ga.f + 1) * ga.n_prims_total + S.prim_off) * 8;
+        const PrimC* prims = stage[0].pr;
+        const float* ps0 = stage[0].ps[0];
+        const float* ps1 = stage[0].ps[1];
         float* gps0 = gpstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
         float* gps1 = gpstate + ((size_t)(ga.f + 1) * ga.n_prims_total + S.prim_off) * 8;
         NodeTr tr;
         v3 g = mk(gv4.x, gv4.y, gv4.z);
         if (act) {
-            grid_node(S, ga.prims + S.prim_off, ps0, ps1, I0, I1, I2, pm, &tr);
+            grid_node(S, prims, ps0, ps1, I0, I1, I2, pm, &tr);
             int Iv[3] = {I0, I1, I2};
             for (int d = 2; d >= 0; d--) {
                 v3 v = tr.vw[d];
@@ -1316,9 +1362,9 @@ __global__ void __launch_bounds__(256) k_grid_op_bwd(GridArgs ga, const float4* 
         const int blk_gb = m.gb;
         bool blk_any = false;
         int uidx = 0;
-        for (int q = 0; q < S.n_prims; q++) { Unit u[2]; uidx += prim_units(ga.prims[S.prim_off + q], u); }
+        for (int q = 0; q < S.n_prims; q++) { Unit u[2]; uidx += prim_units(prims[q], u); }
         for (int q = S.n_prims - 1; q >= 0; q--) {
-            const PrimC& P = ga.prims[S.prim_off + q];
+            const PrimC& P = prims[q];
             PS s0 = load_ps(ps0 + q * 8), s1 = load_ps(ps1 + q * 8);
             Unit u[2];
             int m = prim_units(P, u);
@@ -2760,14 +2806,15 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
     if (fuse) {
         int bg = c->bslot_of(g);
         prb = P2GRebuild{c->targs(bg), c->frame_ptr(g), c->aff_ptr(g), c->tiles_pm, FB_LEAD};
-        grb = GridRebuild{c->gargs(g, bg, false), c->tiles_pm, c->g_pm(g), c->g_v(g), FB_LEAD};
+        // (the grid adjoint runs one-wave workgroups: 4 * FB_LEAD of them lead)
+        grb = GridRebuild{c->gargs(g, bg, false), c->tiles_pm, c->g_pm(g), c->g_v(g), 4 * FB_LEAD};
     }
     int use_kept = kept ? 1 : 0;
     { PhaseTimer t(c, MPM_PH_G2P_BWD, st);
       hipLaunchKernelGGL(k_g2p_bwd_tile, dim3(c->tile_ctas + prb.lead), dim3(WG), 0, st, ta, in, in1, a1, a0, c->g_v(f), c->tiles_v,
                          use_kept, prb, perm1); }
     { PhaseTimer t(c, MPM_PH_GRID_BWD, st);
-      hipLaunchKernelGGL(k_grid_op_bwd, dim3(c->grid_ctas + grb.lead), dim3(256), 0, st, ga, c->g_pm(f), c->tiles_v, c->g_adjpm,
+      hipLaunchKernelGGL(k_grid_op_bwd, dim3(2 * c->grid_ctas + grb.lead), dim3(64), 0, st, ga, c->g_pm(f), c->tiles_v, c->g_adjpm,
                          c->pstate_g, use_kept, grb); }
     // fixed-order mode: the primitive adjoints are reduced by their own launch
     // when the collision adjoint needs them, else by the p2g adjoint's leading
