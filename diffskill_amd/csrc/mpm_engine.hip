@@ -1291,16 +1291,20 @@ __global__ void __launch_bounds__(64) k_grid_op_bwd(GridArgs ga, const float4* _
     int lane = threadIdx.x & 63;
     int nb = (int)gridDim.x - rb.lead;
     int e = uniform((int)blockIdx.x - rb.lead);
-    // the first block's record is read together with the counters (a record
-    // beyond the active count is stale and left unused)
+    // the first block's record and kept (momentum, mass) are read together
+    // with the counters (a record or kept block beyond the active count is
+    // stale and left unused)
+    float4 pm0 = (use_kept && e < ga.keep_cap) ? ga.keep_pm[(size_t)e * NODES_PER_BLK + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
     BlockMeta m;
     bool have = e < ga.rec_cap;
     if (have) read_rec(ga, e, m);
     int n_active = ga.counters[CNT_NBLK];
     bool kept = use_kept && kept_fits(ga.counters, ga.keep_cap);
     int stamp = ga.counters[CNT_STAMP];
+    bool first = true;
     for (; e < n_active; e += nb) {
-        float4 pm = kept ? ga.keep_pm[(size_t)e * NODES_PER_BLK + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 pm = kept ? (first ? pm0 : ga.keep_pm[(size_t)e * NODES_PER_BLK + lane]) : make_float4(0.f, 0.f, 0.f, 0.f);
+        first = false;
         if (!have) block_meta(ga, e, stamp, m);
         have = false;
         stage_scene(stage[0], ga, m, lane);
