@@ -597,13 +597,18 @@ DEV void dsetc(d3& a, int i, double v) { if (i == 0) a.x = v; else if (i == 1) a
 DEV double dmax(double a, double b) { return b < a ? a : b; }
 DEV double dmin(double a, double b) { return a < b ? a : b; }
 DEV double dsgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+// (the FD normals' fp64 square roots and divisions take the Newton-refined
+// reciprocal square root, a few ulp from correct rounding: the normal is
+// rounded to fp32 after a 2e-4 difference, so those ulps never reach it;
+// the argument is at least LEN_EPS_SHAPE)
 DEV double box_sdf_d(const PrimC& P, d3 p, d3* g) {
     double q0 = fabs(p.x) - (double)P.size[0], q1 = fabs(p.y) - (double)P.size[1], q2 = fabs(p.z) - (double)P.size[2];
     double a0 = dmax(q0, 0.0), a1 = dmax(q1, 0.0), a2 = dmax(q2, 0.0);
-    double L = sqrt(a0 * a0 + a1 * a1 + a2 * a2 + (double)LEN_EPS_SHAPE);
+    double L2 = a0 * a0 + a1 * a1 + a2 * a2 + (double)LEN_EPS_SHAPE;
+    double iL = rot_rsq(L2);
+    double L = L2 * iL;
     double m12 = dmax(q1, q2), m = dmax(q0, m12);
     if (g) {
-        double iL = 1.0 / L;
         double g0 = (0.0 < q0) ? a0 * iL : 0.0, g1 = (0.0 < q1) ? a1 * iL : 0.0, g2 = (0.0 < q2) ? a2 * iL : 0.0;
         if (m < 0.0) {
             if (m12 < q0) g0 += 1.0;
@@ -660,9 +665,10 @@ DEV v3 fd_normal_d(const PrimC& P, int sh, v3 pf, float* Lout = nullptr) {
         dsetc(dec, i, dcomp(p, i) - d);
         dsetc(m, i, (0.5 / d) * (fd_sdf_d(P, sh, inc, nullptr) - fd_sdf_d(P, sh, dec, nullptr)));
     }
-    double L = sqrt(m.x * m.x + m.y * m.y + m.z * m.z + (double)LEN_EPS_SHAPE);
-    if (Lout) *Lout = (float)L;
-    return mk((float)(m.x / L), (float)(m.y / L), (float)(m.z / L));
+    double L2 = m.x * m.x + m.y * m.y + m.z * m.z + (double)LEN_EPS_SHAPE;
+    double iL = rot_rsq(L2);
+    if (Lout) *Lout = (float)(L2 * iL);
+    return mk((float)(m.x * iL), (float)(m.y * iL), (float)(m.z * iL));
 }
 // adjoint of fd_normal_d from its forward result n = m / L and L:
 // dn/dm = (I - n n^T) / L, and dm_i/dp = (grad sdf(p + d e_i) - grad sdf(p - d e_i)) * 0.5 / d,
