@@ -215,15 +215,17 @@ def config3_memory():
         env = TaichiEnv(dict(SIMULATOR=w["sim"], PRIMITIVES=w["prims"], SHAPES=[]), init_particles=w["x0"], ckpt_k=K)
         env.initialize()
         gm = GradModel(env, softness=666.0)
-        t0 = time.perf_counter()
-        obs = gm.reset()
-        a = torch.tensor(acts, dtype=torch.float32, device="cuda", requires_grad=True)
-        for s in range(H):
-            obs = gm.forward(s, a[s], *obs)
-        loss = chamfer_distance(obs[0][:, :3], t)
-        loss.backward()
-        torch.cuda.synchronize()
-        sec = time.perf_counter() - t0
+        # two passes, the second timed (the first includes first-use costs)
+        for rep in range(2):
+            t0 = time.perf_counter()
+            obs = gm.reset()
+            a = torch.tensor(acts, dtype=torch.float32, device="cuda", requires_grad=True)
+            for s in range(H):
+                obs = gm.forward(s, a[s], *obs)
+            loss = chamfer_distance(obs[0][:, :3], t)
+            loss.backward()
+            torch.cuda.synchronize()
+            sec = time.perf_counter() - t0
         peak = torch.cuda.max_memory_allocated() - base
         ws = env.simulator.engine.ws.numel()
         out["store-all" if K is None else "K=%d" % K] = dict(peak_bytes=int(peak), workspace_bytes=int(ws), seconds=round(sec, 3),
