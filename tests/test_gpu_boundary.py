@@ -2,7 +2,8 @@
 
 - a scene with six tools, the task file plb/envs/triplemove.yml (three dough
   boxes of 3333 particles, six Spheres; its values restated here), loads
-  through TaichiEnv and a short GradModel rollout matches the oracle;
+  through TaichiEnv and a short GradModel rollout matches the oracle; so does
+  a scene with eight Spheres, the most collision units a scene carries;
 - a scene with more collision units than the engine carries is refused with
   a clear error;
 - the entry points that add into a frame's adjoint refuse frames outside the
@@ -26,24 +27,38 @@ def triplemove_cfg():
     return dict(SIMULATOR=dict(yield_stress=200.0), SHAPES=shapes, PRIMITIVES=prims)
 
 
-def test_six_sphere_task_loads_and_matches_oracle():
+def eight_sphere_cfg():
+    """Eight Spheres (eight collision units, the most a scene carries: the
+    grid kernels stage that many primitives per scene), four around each of
+    two dough boxes."""
+    shapes = [dict(shape="box", width=(0.1, 0.1, 0.1), init_pos=(x, 0.05, 0.5), n_particles=1500) for x in (0.35, 0.65)]
+    pos = []
+    for x in (0.35, 0.65):
+        pos += [(x - 0.075, 0.05, 0.5), (x + 0.075, 0.05, 0.5), (x, 0.05, 0.425), (x, 0.05, 0.575)]
+    prims = [dict(shape="Sphere", radius=0.03, init_pos=p, color=(0.8, 0.8, 0.8), friction=0.9,
+                  action=dict(dim=3, scale=(0.01, 0.01, 0.01))) for p in pos]
+    return dict(SIMULATOR=dict(yield_stress=200.0), SHAPES=shapes, PRIMITIVES=prims)
+
+
+def rollout_matches_oracle(cfg, n_particles, n_tools, seed):
+    """A short GradModel rollout of the task (2 env steps, density-grid L1
+    loss) against the fp32 / fp64 oracle: final positions, loss and action
+    gradients; every tool's actions carry gradient."""
     from diffskill_amd.engine import GradModel, TaichiEnv
-    cfg = triplemove_cfg()
     env = TaichiEnv(cfg)
     env.initialize()
     sim = env.simulator
     n = env.n_particles
-    assert n == 3 * 3333 and len(env.primitives) == 6
-    print("case triplemove: %d particles, %d spheres, %d substeps per env step" % (n, len(env.primitives), sim.substeps),
-          flush=True)
+    assert n == n_particles and len(env.primitives) == n_tools
+    print("case %d spheres: %d particles, %d substeps per env step" % (n_tools, n, sim.substeps), flush=True)
     gm = GradModel(env, softness=666.0, output_grid=(1,))
     obs = gm.reset()
     x0 = sim.get_state(0)[0]
     H = 2
-    acts = np.random.default_rng(7).uniform(-1, 1, (H, 18))
+    acts = np.random.default_rng(seed).uniform(-1, 1, (H, 3 * n_tools))
     a = torch.tensor(acts, dtype=torch.float32, device="cuda", requires_grad=True)
     ng = sim.n_grid
-    target = np.random.default_rng(8).uniform(0, 2e-4, (ng, ng, ng))
+    target = np.random.default_rng(seed + 1).uniform(0, 2e-4, (ng, ng, ng))
     for s in range(H):
         obs = gm.forward(s, a[s], *obs)
     loss = (obs[2] - torch.as_tensor(target, dtype=torch.float32, device="cuda")).abs().sum()
@@ -75,7 +90,15 @@ def test_six_sphere_task_loads_and_matches_oracle():
     assert abs(float(loss) - res[64][1]) < 1e-4 * res[64][1]
     assert rel(g, res[32][2]) < 1e-3
     # every sphere moved the dough: each tool's actions carry gradient
-    assert all(np.abs(g[:, 3 * j:3 * j + 3]).max() > 0 for j in range(6))
+    assert all(np.abs(g[:, 3 * j:3 * j + 3]).max() > 0 for j in range(n_tools))
+
+
+def test_six_sphere_task_loads_and_matches_oracle():
+    rollout_matches_oracle(triplemove_cfg(), 3 * 3333, 6, seed=7)
+
+
+def test_eight_sphere_scene_matches_oracle():
+    rollout_matches_oracle(eight_sphere_cfg(), 2 * 1500, 8, seed=11)
 
 
 def test_too_many_collision_units_refused():
