@@ -867,7 +867,8 @@ DEV void block_meta(const GridArgs& ga, int e, int stamp, BlockMeta& m) {
 // reads them many times per node and under divergent branches, where they
 // would otherwise be vector loads from memory, each a full memory latency on
 // the wave's critical path.
-static_assert(sizeof(SceneC) == 32 * 4 && sizeof(PrimC) == 40 * 4, "staged as whole dwords");
+constexpr int SCENE_DW = sizeof(SceneC) / 4, PRIM_DW = sizeof(PrimC) / 4;
+static_assert(SCENE_DW * 4 == sizeof(SceneC) && SCENE_DW <= 64 && PRIM_DW * 4 == sizeof(PrimC), "staged as whole dwords");
 struct SceneStage {
     SceneC sc;
     PrimC pr[MAX_UNITS];
@@ -879,12 +880,12 @@ DEV void stage_scene(SceneStage& st, const GridArgs& ga, const BlockMeta& m, int
     // the previous block's reads of the stage are issued before it is rewritten
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const int* src = reinterpret_cast<const int*>(ga.scenes + m.sc);
-    int v = lane < 32 ? src[lane] : 0;
+    int v = lane < SCENE_DW ? src[lane] : 0;
     int poff = m.poff, np = m.np;
-    if (lane < 32) reinterpret_cast<int*>(&st.sc)[lane] = v;
+    if (lane < SCENE_DW) reinterpret_cast<int*>(&st.sc)[lane] = v;
     const int* ps = reinterpret_cast<const int*>(ga.prims + poff);
     int* dp = reinterpret_cast<int*>(st.pr);
-    for (int k = lane; k < np * 40; k += 64) dp[k] = ps[k];
+    for (int k = lane; k < np * PRIM_DW; k += 64) dp[k] = ps[k];
     const float* s0 = ga.pstate + ((size_t)ga.f * ga.n_prims_total + poff) * 8;
     const float* s1 = s0 + (size_t)ga.n_prims_total * 8;
     for (int k = lane; k < np * 8; k += 64) {
