@@ -81,6 +81,25 @@ def test_checkpointed_recompute_is_bitwise_store_all():
         assert same(out[0], ref[0]), K
 
 
+def test_forked_constitutive_update_is_bitwise_repeatable():
+    """A context of 2^17 particles or more runs the forward's constitutive
+    update on its second stream beside the binning (smaller ones run it in
+    line): repeated eager passes, graph replays and checkpointed recompute
+    stay bitwise identical there too."""
+    from diffskill_amd import workloads
+    w = workloads.config4(n_scenes=8, n_env_steps=2, substeps=10, lo=16000, hi=30000)
+    assert int(np.sum(w["sizes"])) >= 1 << 17
+    targets = targets_for(w)
+    print("case determinism: forked update, %d particles" % int(np.sum(w["sizes"])), flush=True)
+    ref, _ = batched_pass(w, targets, reps=2)
+    assert same(ref[0], ref[1])
+    out, _ = batched_pass(w, targets, graph=True, reps=2)
+    for o in out:
+        assert same(o, ref[0])
+    out, _ = batched_pass(w, targets, ckpt_k=7)
+    assert same(out[0], ref[0])
+
+
 def test_fixed_order_results_match_default_mode_and_oracle():
     """The fixed-order mode changes only summation order: within 1e-4 of the
     default mode and within 1e-3 of the fp32 oracle on every scene."""
