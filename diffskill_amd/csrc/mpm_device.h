@@ -847,14 +847,19 @@ DEV void unit_collider_v_rev(const Unit& u, const PS& s0, const PS& s1, v3 x, fl
     unit_center_rev(s0, u.jaw, -gd, g0);
 }
 
-DEV int prim_units(const PrimC& P, Unit* u) {
+// the number of collision units of a primitive, and unit j by value (no
+// array: a dynamically indexed local array would be placed in LDS, indexed by
+// a flat work-item id read from the dispatch packet)
+DEV int prim_unit_count(const PrimC& P) { return (P.type == P_GRIPPER || P.type == P_GRIPPER2) ? 2 : 1; }
+DEV Unit prim_unit(const PrimC& P, int j) {
+    Unit u;
+    u.sh = P.unit_shape;
     if (P.type == P_GRIPPER || P.type == P_GRIPPER2) {
-        u[0].jaw = -1; u[0].sh = P.unit_shape; u[0].norot = 0; u[0].len_eps = LEN_EPS_SHAPE;
-        u[1] = u[0]; u[1].jaw = 1;
-        return 2;
+        u.jaw = j == 0 ? -1 : 1; u.norot = 0; u.len_eps = LEN_EPS_SHAPE;
+    } else {
+        u.jaw = 0; u.norot = (P.unit_shape == SH_SPHERE); u.len_eps = LEN_EPS_BASE;
     }
-    u[0].jaw = 0; u[0].sh = P.unit_shape; u[0].norot = (P.unit_shape == SH_SPHERE); u[0].len_eps = LEN_EPS_BASE;
-    return 1;
+    return u;
 }
 
 // Primitive.collide (primive_base.py:96-120) / Gripper.collide2 (primitives.py:508-536)

@@ -175,11 +175,10 @@ DEV v3 grid_node(const SceneC& S, const PrimC* prims, const float* ps0, const fl
     for (int q = 0; q < S.n_prims; q++) {
         const PrimC& P = prims[q];
         PS s0 = load_ps(ps0 + q * 8), s1 = load_ps(ps1 + q * 8);
-        Unit u[2];
-        int m = prim_units(P, u);
+        int m = prim_unit_count(P);
         for (int j = 0; j < m; j++) {
             if (tr && nu < MAX_UNITS) tr->vu[nu] = v;
-            v = unit_collide(P, u[j], s0, s1, x, S.dt, v, nullptr);
+            v = unit_collide(P, prim_unit(P, j), s0, s1, x, S.dt, v, nullptr);
             nu++;
         }
     }
@@ -943,18 +942,22 @@ DEV float4 gather_tiles(const BlockMeta& m, int lane, const float4* __restrict__
 // and the velocity.
 // With only_overflow set it runs only when the kept grid overflowed (the
 // adjoint then recomputes the grid instead of reading the kept one).
+// WPG: waves per workgroup of the calling launch (a constant: blockDim.x is a
+// 16-bit field that only a vector load reads, one more latency at entry)
+template <int WPG>
 DEV void grid_op_body(const GridArgs& ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm, float4* __restrict__ g_v,
                       SceneStage* stages, int only_overflow, int lead = 0, int nwg = 0) {
     int lane = threadIdx.x & 63;
     int wave = threadIdx.x >> 6;
-    int wpg = (int)blockDim.x >> 6;
+    constexpr int wpg = WPG;
     int nb = nwg > 0 ? nwg : (int)gridDim.x - lead;
     int e = uniform(((int)blockIdx.x - lead) * wpg + wave);
     // the first block's record is read together with the counters (a record
-    // beyond the active count is stale and left unused)
+    // beyond the active count is stale and left unused; the read is
+    // unconditional, clamped to the last record, so that both issue at once)
     BlockMeta m;
     bool have = e < ga.rec_cap;
-    if (have) read_rec(ga, e, m);
+    read_rec(ga, min(e, ga.rec_cap - 1), m);
     if (only_overflow && kept_fits(ga.counters, ga.keep_cap)) return;
     int n_active = ga.counters[CNT_NBLK];
     int stamp = ga.counters[CNT_STAMP];
@@ -983,7 +986,7 @@ DEV void grid_op_body(const GridArgs& ga, const float4* __restrict__ tiles, floa
 __global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
                                                  float4* __restrict__ g_v) {
     __shared__ SceneStage stages[4];
-    grid_op_body(ga, tiles, g_pm, g_v, stages, 0);
+    grid_op_body<4>(ga, tiles, g_pm, g_v, stages, 0);
 }
 // the same operator rebuilding the grid of a substep for its adjoint (the
 // grid was not kept, or, with only_overflow, its active list outgrew the
@@ -991,7 +994,7 @@ __global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, const float4* __re
 __global__ void __launch_bounds__(256) k_grid_op_rebuild(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
                                                          float4* __restrict__ g_v, int only_overflow) {
     __shared__ SceneStage stages[4];
-    grid_op_body(ga, tiles, g_pm, g_v, stages, only_overflow);
+    grid_op_body<4>(ga, tiles, g_pm, g_v, stages, only_overflow);
 }
 
 // load the 6x6x6 node tile of a cell block from a node grid into LDS
@@ -1287,7 +1290,7 @@ __global__ void __launch_bounds__(64) k_grid_op_bwd(GridArgs ga, const float4* _
                                                     GridRebuild rb) {
     __shared__ SceneStage stage[1];
     if ((int)blockIdx.x < rb.lead) {
-        grid_op_body(rb.ga, rb.tiles, rb.g_pm, rb.g_v, stage, 1, 0, rb.lead);
+        grid_op_body<1>(rb.ga, rb.tiles, rb.g_pm, rb.g_v, stage, 1, 0, rb.lead);
         return;
     }
     int lane = threadIdx.x & 63;
@@ -1299,7 +1302,7 @@ __global__ void __launch_bounds__(64) k_grid_op_bwd(GridArgs ga, const float4* _
     float4 pm0 = (use_kept && e < ga.keep_cap) ? ga.keep_pm[(size_t)e * NODES_PER_BLK + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
     BlockMeta m;
     bool have = e < ga.rec_cap;
-    if (have) read_rec(ga, e, m);
+    read_rec(ga, min(e, ga.rec_cap - 1), m);
     int n_active = ga.counters[CNT_NBLK];
     bool kept = use_kept && kept_fits(ga.counters, ga.keep_cap);
     int stamp = ga.counters[CNT_STAMP];
@@ -1368,16 +1371,15 @@ __global__ void __launch_bounds__(64) k_grid_op_bwd(GridArgs ga, const float4* _
         const int blk_gb = m.gb;
         bool blk_any = false;
         int uidx = 0;
-        for (int q = 0; q < S.n_prims; q++) { Unit u[2]; uidx += prim_units(prims[q], u); }
+        for (int q = 0; q < S.n_prims; q++) uidx += prim_unit_count(prims[q]);
         for (int q = S.n_prims - 1; q >= 0; q--) {
             const PrimC& P = prims[q];
             PS s0 = load_ps(ps0 + q * 8), s1 = load_ps(ps1 + q * 8);
-            Unit u[2];
-            int m = prim_units(P, u);
+            int m = prim_unit_count(P);
             for (int j = m - 1; j >= 0; j--) {
                 uidx--;
                 PSg g0 = zero_psg(), g1 = zero_psg();
-                if (act && uidx < MAX_UNITS) g = unit_collide_rev(P, u[j], s0, s1, x, S.dt, tr.vu[uidx], g, &g0, &g1);
+                if (act && uidx < MAX_UNITS) g = unit_collide_rev(P, prim_unit(P, j), s0, s1, x, S.dt, tr.vu[uidx], g, &g0, &g1);
                 float vals[16] = {g0.pos.x, g0.pos.y, g0.pos.z, g0.rot.w, g0.rot.x, g0.rot.y, g0.rot.z, g0.gap,
                                   g1.pos.x, g1.pos.y, g1.pos.z, g1.rot.w, g1.rot.x, g1.rot.y, g1.rot.z, g1.gap};
                 bool any = false;
@@ -2016,9 +2018,8 @@ __global__ void __launch_bounds__(PPB) k_min_dist(ChunkArgs ca, const float* __r
     for (int q = 0; q < S.n_prims; q++) {
         const PrimC& P = prims[S.prim_off + q];
         PS s = load_ps(ps + (size_t)(S.prim_off + q) * 8);
-        Unit u[2];
-        int m = prim_units(P, u);
-        for (int j = 0; j < m; j++, col++) out[(size_t)col * ca.stride + slot] = unit_sdf(P, u[j], s, x);
+        int m = prim_unit_count(P);
+        for (int j = 0; j < m; j++, col++) out[(size_t)col * ca.stride + slot] = unit_sdf(P, prim_unit(P, j), s, x);
     }
 }
 
@@ -2042,12 +2043,11 @@ __global__ void __launch_bounds__(PPB) k_min_dist_bwd(ChunkArgs ca, const float*
     for (int q = 0; q < S.n_prims; q++) {
         const PrimC& P = prims[S.prim_off + q];
         PS s = load_ps(ps + (size_t)(S.prim_off + q) * 8);
-        Unit u[2];
-        int m = prim_units(P, u);
+        int m = prim_unit_count(P);
         PSg g = zero_psg();
         for (int j = 0; j < m; j++, col++) {
             float gd = valid ? gout[(size_t)col * ca.stride + slot] : 0.f;
-            if (gd != 0.f) gx += unit_sdf_rev_x(P, u[j], s, x, gd, &g);
+            if (gd != 0.f) gx += unit_sdf_rev_x(P, prim_unit(P, j), s, x, gd, &g);
         }
         float vals[8] = {g.pos.x, g.pos.y, g.pos.z, g.rot.w, g.rot.x, g.rot.y, g.rot.z, g.gap};
         bool any = false;
@@ -2120,11 +2120,10 @@ struct CollArgs {
 // adjoint (Taichi's min gives it to the left operand only when strictly
 // smaller), *jn the jaw whose normal is used.
 DEV float prim_sdf_jaws(const PrimC& P, const PS& s, v3 x, int* js, int* jn) {
-    Unit u[2];
-    int m = prim_units(P, u);
-    float a = unit_sdf(P, u[0], s, x);
+    int m = prim_unit_count(P);
+    float a = unit_sdf(P, prim_unit(P, 0), s, x);
     if (m == 1) { *js = *jn = 0; return a; }
-    float b = unit_sdf(P, u[1], s, x);
+    float b = unit_sdf(P, prim_unit(P, 1), s, x);
     *js = a < b ? 0 : 1;
     *jn = a <= b ? 0 : 1;
     return tmin(a, b);
@@ -2193,9 +2192,7 @@ __global__ void __launch_bounds__(256) k_tool_collide(CollArgs ca, const PrimC* 
         v3 x = mk(x4.x, x4.y, x4.z);
         int js, jn;
         float d = prim_sdf_jaws(Pi, si, x, &js, &jn);
-        Unit u[2];
-        prim_units(Pi, u);
-        v3 n = unit_normal(Pi, u[jn], si, x);
+        v3 n = unit_normal(Pi, prim_unit(Pi, jn), si, x);
         float L = sqrtf(dot(n, n));
         psi[0] += n.x / L * d;
         psi[1] += n.y / L * d;
@@ -2225,9 +2222,7 @@ __global__ void __launch_bounds__(64) k_tool_collide_bwd(CollArgs ca, const Prim
         v3 x = mk(x4.x, x4.y, x4.z);
         int js, jn;
         float d = prim_sdf_jaws(Pi, s, x, &js, &jn);
-        Unit u[2];
-        prim_units(Pi, u);
-        v3 n = unit_normal(Pi, u[jn], s, x);
+        v3 n = unit_normal(Pi, prim_unit(Pi, jn), s, x);
         float L = sqrtf(dot(n, n));
         v3 nh = mk(n.x / L, n.y / L, n.z / L);
         float* gi = gpstate + o;
@@ -2236,8 +2231,8 @@ __global__ void __launch_bounds__(64) k_tool_collide_bwd(CollArgs ca, const Prim
         float tt = dot(nh, gnh);
         v3 gn = mk((gnh.x - nh.x * tt) / L, (gnh.y - nh.y * tt) / L, (gnh.z - nh.z * tt) / L);
         PSg acc = zero_psg();
-        unit_sdf_rev(Pi, u[js], s, x, dot(nh, g), &acc);
-        unit_normal_rev(Pi, u[jn], s, x, gn, &acc);
+        unit_sdf_rev(Pi, prim_unit(Pi, js), s, x, dot(nh, g), &acc);
+        unit_normal_rev(Pi, prim_unit(Pi, jn), s, x, gn, &acc);
         gi[0] += acc.pos.x; gi[1] += acc.pos.y; gi[2] += acc.pos.z;
         gi[3] += acc.rot.w; gi[4] += acc.rot.x; gi[5] += acc.rot.y; gi[6] += acc.rot.z;
         gi[7] += acc.gap;
