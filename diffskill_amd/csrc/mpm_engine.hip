@@ -164,7 +164,11 @@ DEV void load_usv(const float* usv, int stride, int p, PTrace& t) {
 }
 
 // grid_op for one node with m > 1e-12 (mpm_simulator.py:230-262)
-struct NodeTr { v3 vu[MAX_UNITS]; v3 vw[3]; };
+// the node chain's intermediate velocities for its adjoint: before each
+// collision unit (in LDS, vu[(unit * 3 + c) * 64 + lane]: indexed by the unit
+// at run time, which in registers would be a scratch array in memory) and
+// before each wall stage (registers)
+struct NodeTr { float* vu; v3 vw[3]; };
 DEV v3 grid_node(const SceneC& S, const PrimC* prims, const float* ps0, const float* ps1, int I0, int I1, int I2,
                  float4 pm, NodeTr* tr) {
     float inv_m = 1.f / pm.w;
@@ -177,7 +181,12 @@ DEV v3 grid_node(const SceneC& S, const PrimC* prims, const float* ps0, const fl
         PS s0 = load_ps(ps0 + q * 8), s1 = load_ps(ps1 + q * 8);
         int m = prim_unit_count(P);
         for (int j = 0; j < m; j++) {
-            if (tr && nu < MAX_UNITS) tr->vu[nu] = v;
+            if (tr && nu < MAX_UNITS) {
+                int lane = threadIdx.x & 63;
+                tr->vu[(nu * 3 + 0) * 64 + lane] = v.x;
+                tr->vu[(nu * 3 + 1) * 64 + lane] = v.y;
+                tr->vu[(nu * 3 + 2) * 64 + lane] = v.z;
+            }
             v = unit_collide(P, prim_unit(P, j), s0, s1, x, S.dt, v, nullptr);
             nu++;
         }
@@ -1289,6 +1298,7 @@ __global__ void __launch_bounds__(64) k_grid_op_bwd(GridArgs ga, const float4* _
                                                     float4* __restrict__ g_adjpm, float* __restrict__ gpstate, int use_kept,
                                                     GridRebuild rb) {
     __shared__ SceneStage stage[1];
+    __shared__ float s_vu[MAX_UNITS * 3 * 64];
     if ((int)blockIdx.x < rb.lead) {
         grid_op_body<1>(rb.ga, rb.tiles, rb.g_pm, rb.g_v, stage, 1, 0, rb.lead);
         return;
@@ -1326,6 +1336,7 @@ __global__ void __launch_bounds__(64) k_grid_op_bwd(GridArgs ga, const float4* _
         float* gps0 = gpstate + ((size_t)ga.f * ga.n_prims_total + S.prim_off) * 8;
         float* gps1 = gpstate + ((size_t)(ga.f + 1) * ga.n_prims_total + S.prim_off) * 8;
         NodeTr tr;
+        tr.vu = s_vu;
         v3 g = mk(gv4.x, gv4.y, gv4.z);
         if (act) {
             grid_node(S, prims, ps0, ps1, I0, I1, I2, pm, &tr);
@@ -1379,7 +1390,10 @@ __global__ void __launch_bounds__(64) k_grid_op_bwd(GridArgs ga, const float4* _
             for (int j = m - 1; j >= 0; j--) {
                 uidx--;
                 PSg g0 = zero_psg(), g1 = zero_psg();
-                if (act && uidx < MAX_UNITS) g = unit_collide_rev(P, prim_unit(P, j), s0, s1, x, S.dt, tr.vu[uidx], g, &g0, &g1);
+                if (act && uidx < MAX_UNITS) {
+                    v3 vin = mk(tr.vu[(uidx * 3 + 0) * 64 + lane], tr.vu[(uidx * 3 + 1) * 64 + lane], tr.vu[(uidx * 3 + 2) * 64 + lane]);
+                    g = unit_collide_rev(P, prim_unit(P, j), s0, s1, x, S.dt, vin, g, &g0, &g1);
+                }
                 float vals[16] = {g0.pos.x, g0.pos.y, g0.pos.z, g0.rot.w, g0.rot.x, g0.rot.y, g0.rot.z, g0.gap,
                                   g1.pos.x, g1.pos.y, g1.pos.z, g1.rot.w, g1.rot.x, g1.rot.y, g1.rot.z, g1.gap};
                 bool any = false;
