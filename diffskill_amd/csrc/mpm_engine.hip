@@ -287,9 +287,11 @@ struct BinArgs {
     // first reached the block this pass; the block's work items are tagged
     // (item.w) with the XCD that owns that slot's eighth of the particle
     // columns, so items whose particles share cache lines of the SoA rows run
-    // on one XCD; xlist lists the items grouped by tag
+    // on one XCD; xlist lists the items grouped by tag, each entry the item
+    // itself with .w its index (a tile kernel then reads one entry, not an
+    // index and then the item)
     int* blk_rep;
-    int* xlist;
+    int4* xlist;
     int* blk_tot;    // particles of each occupied block this pass (slot re-sort on), or null
 };
 
@@ -547,7 +549,8 @@ __global__ void __launch_bounds__(PPB) k_scatter(BinArgs a, int* __restrict__ nr
         for (int x = 0; x < NXCD; x++) s_cnt[t][x] = off[x];
         for (int e = e0; e < e1; e++) {
             int x = a.items[e].w;
-            a.xlist[s_cnt[t][x]] = e;
+            int4 it = a.items[e];
+            a.xlist[s_cnt[t][x]] = make_int4(it.x, it.y, it.z, e);
             s_cnt[t][x] += 1;
         }
     }
@@ -760,7 +763,7 @@ struct TileArgs {
     const float4* keep_v;
     const int* nmap;
     int keep_cap;
-    const int* xlist;      // work items grouped by XCD (see BinArgs)
+    const int4* xlist;     // work items grouped by XCD, .w the item index (see BinArgs)
 };
 
 // The work items a workgroup of a tile kernel takes: those of its own XCD's
@@ -769,7 +772,7 @@ struct TileArgs {
 // every item in turn.  The item workgroups are the launch's workgroups
 // [lead, lead + nwg) (lead a multiple of NXCD; nwg <= 0: the rest of the
 // launch); the others do other work.
-struct ItemIter { const int* list; int j, step, n; };
+struct ItemIter { const int4* list; int j, step, n; };
 DEV ItemIter item_iter(const TileArgs& ta, int lead = 0, int nwg = 0) {
     ItemIter r;
     int b = (int)blockIdx.x - lead, nb = nwg > 0 ? nwg : (int)gridDim.x - lead;
@@ -786,6 +789,16 @@ DEV ItemIter item_iter(const TileArgs& ta, int lead = 0, int nwg = 0) {
         r.step = nb;
     }
     return r;
+}
+// entry j of a workgroup's items: the item and its index e
+DEV int4 item_at(const TileArgs& ta, const ItemIter& ir, int j, int& e) {
+    if (ir.list) {
+        int4 v = ir.list[j];
+        e = v.w;
+        return v;
+    }
+    e = j;
+    return ta.items[j];
 }
 
 struct GridArgs {
@@ -1044,8 +1057,8 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(TileArgs ta, const float* __res
     ItemIter ir = item_iter(ta);
     int stride = ta.stride;
     for (int j = ir.j; j < ir.n; j += ir.step) {
-        int e = ir.list ? ir.list[j] : j;
-        int4 it = ta.items[e];
+        int e;
+        int4 it = item_at(ta, ir, j, e);
         // the first batch's positions do not depend on the tile: their loads
         // are issued first, so both latencies overlap; every later batch's
         // are issued one batch ahead
@@ -1160,8 +1173,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) k_
     int n_acc = (WG / 64) * 3 * TILE;
     float* wacc = acc + (threadIdx.x >> 6) * 3 * TILE;
     for (int j = ir.j; j < ir.n; j += ir.step) {
-        int e = ir.list ? ir.list[j] : j;
-        int4 it = ta.items[e];
+        int e;
+        int4 it = item_at(ta, ir, j, e);
         // the first batch's particle data does not depend on the tile: its
         // loads are issued first, so both latencies overlap
         int p = ta.order[it.y + min((int)threadIdx.x, it.z - 1)];
@@ -1499,8 +1512,8 @@ DEV void p2g_scatter_body(float* tile, const TileArgs& ta, const float* __restri
     int n_tile = (WG / 64) * 4 * TILE;
     float* wtile = tile + (threadIdx.x >> 6) * 4 * TILE;
     for (int j = ir.j; j < ir.n; j += ir.step) {
-        int e = ir.list ? ir.list[j] : j;
-        int4 it = ta.items[e];
+        int e;
+        int4 it = item_at(ta, ir, j, e);
         const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
         int bi, bj, bk;
         block_origin(S, it.x, bi, bj, bk);
@@ -1668,8 +1681,8 @@ __global__ void __launch_bounds__(WG) k_p2g_bwd_gather(TileArgs ta, const float*
     ItemIter ir = item_iter(ta, lead);
     int stride = ta.stride;
     for (int j = ir.j; j < ir.n; j += ir.step) {
-        int e = ir.list ? ir.list[j] : j;
-        int4 it = ta.items[e];
+        int e;
+        int4 it = item_at(ta, ir, j, e);
         // the first batch's particle data does not depend on the tile: its
         // loads are issued first, so both latencies overlap
         int k = threadIdx.x;
@@ -1958,8 +1971,8 @@ __global__ void __launch_bounds__(WG) k_grid_mass_items(TileArgs ta, const float
                                                         unsigned long long* __restrict__ macc, int det) {
     ItemIter ir = item_iter(ta);
     for (int j = ir.j; j < ir.n; j += ir.step) {
-        int e = ir.list ? ir.list[j] : j;
-        int4 it = ta.items[e];
+        int e;
+        int4 it = item_at(ta, ir, j, e);
         const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
         for (int k0 = 0; k0 < it.z; k0 += WG) {
             int k = k0 + (int)threadIdx.x;
@@ -2346,8 +2359,8 @@ __global__ void __launch_bounds__(WG) k_resort_define(TileArgs ta, const int* __
                                                       int* __restrict__ perm) {
     ItemIter ir = item_iter(ta);
     for (int j = ir.j; j < ir.n; j += ir.step) {
-        int e = ir.list ? ir.list[j] : j;
-        int4 it = ta.items[e];
+        int e;
+        int4 it = item_at(ta, ir, j, e);
         const SceneC& S = ta.scenes[ta.blk_scene[it.x]];
         int base = S.p_off + blk_pre[it.x] - cell_start[it.x * 64];
         for (int k = threadIdx.x; k < it.z; k += WG) {
@@ -2414,7 +2427,7 @@ struct mpm_ctx {
     // slot, so the adjoint of a substep the forward ran reuses both instead of
     // binning and rebuilding the grid again; otherwise one shared slot.
     struct BinSlot {
-        int* order; int4* items; int* blk_stamp; int* blk_item0; int* blk_nitems; int* nblocks; int* nmap; int* counters; int* nrec; int* xlist;
+        int* order; int4* items; int* blk_stamp; int* blk_item0; int* blk_nitems; int* nblocks; int* nmap; int* counters; int* nrec; int4* xlist;
         float4* keep_pm; float4* keep_v;
     };
     std::vector<BinSlot> bslots;
@@ -2625,7 +2638,7 @@ Layout make_layout(const mpm_scene_desc* sc, int n, int max_frames, int ckpt_k, 
     L.bs_nmap = b; b = align256(b + sizeof(int) * (size_t)L.n_blocks);
     L.bs_counters = b; b = align256(b + sizeof(int) * 64);
     L.bs_nrec = b; b = align256(b + sizeof(int) * NREC * (size_t)L.rec_cap);
-    L.bs_xlist = b; b = align256(b + sizeof(int) * (size_t)L.max_items);
+    L.bs_xlist = b; b = align256(b + sizeof(int4) * (size_t)L.max_items);
     L.bs_bytes = b;
     L.keep_bytes = (size_t)L.keep_cap * NODES_PER_BLK * sizeof(float4) * 2;
     L.off_bslots = o; o = align256(o + L.bs_bytes * L.n_bslots);
@@ -2993,7 +3006,7 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
         S.nmap = (int*)(b + L.bs_nmap);
         S.counters = (int*)(b + L.bs_counters);
         S.nrec = (int*)(b + L.bs_nrec);
-        S.xlist = (int*)(b + L.bs_xlist);
+        S.xlist = (int4*)(b + L.bs_xlist);
         S.keep_pm = S.keep_v = nullptr;
         if (c->keep) {
             S.keep_pm = (float4*)(w + L.off_keep + (size_t)k * L.keep_bytes);
