@@ -920,9 +920,10 @@ DEV void stage_scene(SceneStage& st, const GridArgs& ga, const BlockMeta& m, int
 
 // owner-computes gather: the node's value is the sum, in a fixed order, of the
 // tiles of the work items of the (up to 8) cell blocks whose footprint covers
-// it, loaded BATCH at a time across blocks.
+// it, loaded BATCH at a time across blocks; [kb, ke) restricts the sum to
+// those pairs (ke < 0: to the last)
 template <int BATCH>
-DEV float4 gather_tiles(const BlockMeta& m, int lane, const float4* __restrict__ tiles) {
+DEV float4 gather_tiles(const BlockMeta& m, int lane, const float4* __restrict__ tiles, int kb = 0, int ke = -1) {
     int l0 = lane >> 4, l1 = (lane >> 2) & 3, l2 = lane & 3;
     const int* i0 = m.i0;
     const int* ni = m.ni;
@@ -933,9 +934,9 @@ DEV float4 gather_tiles(const BlockMeta& m, int lane, const float4* __restrict__
     pre[0] = 0;
 #pragma unroll
     for (int d = 1; d < 8; d++) pre[d] = pre[d - 1] + ni[d - 1];
-    int total = pre[7] + ni[7];
+    int total = ke < 0 ? pre[7] + ni[7] : ke;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int k0 = 0; k0 < total; k0 += BATCH) {
+    for (int k0 = kb; k0 < total; k0 += BATCH) {
         float4 u[BATCH];
 #pragma unroll
         for (int s = 0; s < BATCH; s++) {
@@ -959,21 +960,24 @@ DEV float4 gather_tiles(const BlockMeta& m, int lane, const float4* __restrict__
     return acc;
 }
 
-// grid_op over the active node blocks (mpm_simulator.py:230-262); one wave per
-// 4x4x4 node block.  Writes the gathered momentum/mass (kept for the adjoint)
-// and the velocity.
+// grid_op over the active node blocks (mpm_simulator.py:230-262), one block
+// per workgroup.  Writes the gathered momentum/mass (kept for the adjoint) and
+// the velocity.  The covering tiles are summed in two halves, pairs [0, h) and
+// [h, total) with h = (total + 1) / 2, each in pair order, then added: with
+// WPG = 2 a second wave sums the second half at the same time and hands it
+// over through LDS (part), halving the gather of the densest blocks (40+
+// tiles), the operator's longest waves; with WPG = 1 the one wave sums both
+// halves in turn -- the same bits either way.
 // With only_overflow set it runs only when the kept grid overflowed (the
 // adjoint then recomputes the grid instead of reading the kept one).
-// WPG: waves per workgroup of the calling launch (a constant: blockDim.x is a
-// 16-bit field that only a vector load reads, one more latency at entry)
 template <int WPG>
 DEV void grid_op_body(const GridArgs& ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm, float4* __restrict__ g_v,
-                      SceneStage* stages, int only_overflow, int lead = 0, int nwg = 0) {
+                      SceneStage& stg, float4* part, int only_overflow, int lead = 0, int nwg = 0) {
+    static_assert(WPG == 1 || WPG == 2, "one or two waves per block");
     int lane = threadIdx.x & 63;
-    int wave = threadIdx.x >> 6;
-    constexpr int wpg = WPG;
+    int wave = WPG == 2 ? (int)(threadIdx.x >> 6) : 0;
     int nb = nwg > 0 ? nwg : (int)gridDim.x - lead;
-    int e = uniform(((int)blockIdx.x - lead) * wpg + wave);
+    int e = uniform((int)blockIdx.x - lead);
     // the first block's record is read together with the counters (a record
     // beyond the active count is stale and left unused; the read is
     // unconditional, clamped to the last record, so that both issue at once)
@@ -983,16 +987,34 @@ DEV void grid_op_body(const GridArgs& ga, const float4* __restrict__ tiles, floa
     if (only_overflow && kept_fits(ga.counters, ga.keep_cap)) return;
     int n_active = ga.counters[CNT_NBLK];
     int stamp = ga.counters[CNT_STAMP];
-    SceneStage& stg = stages[wave];
-    for (; e < n_active; e += nb * wpg) {
+    for (; e < n_active; e += nb) {
         if (!have) block_meta(ga, e, stamp, m);
         have = false;
+        int total = 0;
+#pragma unroll
+        for (int d = 0; d < 8; d++) total += m.ni[d];
+        int h = (total + 1) / 2;
+        if (WPG == 2 && wave == 1) {
+            part[lane] = gather_tiles<8>(m, lane, tiles, h, total);
+            __syncthreads();   // the second half is in LDS
+            __syncthreads();   // and read before the next block's
+            continue;
+        }
         stage_scene(stg, ga, m, lane);
         const SceneC& S = stg.sc;
         int B0, B1, B2, I0, I1, I2;
         long long node;
         node_coords(S, m.gb, lane, B0, B1, B2, I0, I1, I2, node);
-        float4 pm = gather_tiles<8>(m, lane, tiles);
+        float4 pm = gather_tiles<8>(m, lane, tiles, 0, h);
+        float4 b;
+        if (WPG == 2) {
+            __syncthreads();
+            b = part[lane];
+            __syncthreads();
+        } else {
+            b = gather_tiles<8>(m, lane, tiles, h, total);
+        }
+        pm.x += b.x; pm.y += b.y; pm.z += b.z; pm.w += b.w;
         v3 v = mk(0.f, 0.f, 0.f);
         if (pm.w > 1e-12f && in_grid(S, I0, I1, I2)) v = grid_node(S, stg.pr, stg.ps[0], stg.ps[1], I0, I1, I2, pm, nullptr);
         float4 v4 = make_float4(v.x, v.y, v.z, 0.f);
@@ -1004,19 +1026,21 @@ DEV void grid_op_body(const GridArgs& ga, const float4* __restrict__ tiles, floa
         }
     }
 }
-// the forward substep's grid operator
-__global__ void __launch_bounds__(256) k_grid_op(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
+// the forward substep's grid operator (two waves per node block)
+__global__ void __launch_bounds__(128) k_grid_op(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
                                                  float4* __restrict__ g_v) {
-    __shared__ SceneStage stages[4];
-    grid_op_body<4>(ga, tiles, g_pm, g_v, stages, 0);
+    __shared__ SceneStage stage;
+    __shared__ float4 part[64];
+    grid_op_body<2>(ga, tiles, g_pm, g_v, stage, part, 0);
 }
 // the same operator rebuilding the grid of a substep for its adjoint (the
 // grid was not kept, or, with only_overflow, its active list outgrew the
 // kept grid): a kernel of its own so that profiles separate the two
-__global__ void __launch_bounds__(256) k_grid_op_rebuild(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
+__global__ void __launch_bounds__(128) k_grid_op_rebuild(GridArgs ga, const float4* __restrict__ tiles, float4* __restrict__ g_pm,
                                                          float4* __restrict__ g_v, int only_overflow) {
-    __shared__ SceneStage stages[4];
-    grid_op_body<4>(ga, tiles, g_pm, g_v, stages, only_overflow);
+    __shared__ SceneStage stage;
+    __shared__ float4 part[64];
+    grid_op_body<2>(ga, tiles, g_pm, g_v, stage, part, only_overflow);
 }
 
 // load the 6x6x6 node tile of a cell block from a node grid into LDS
@@ -1313,7 +1337,7 @@ __global__ void __launch_bounds__(64) k_grid_op_bwd(GridArgs ga, const float4* _
     __shared__ SceneStage stage[1];
     __shared__ float s_vu[MAX_UNITS * 3 * 64];
     if ((int)blockIdx.x < rb.lead) {
-        grid_op_body<1>(rb.ga, rb.tiles, rb.g_pm, rb.g_v, stage, 1, 0, rb.lead);
+        grid_op_body<1>(rb.ga, rb.tiles, rb.g_pm, rb.g_v, stage[0], nullptr, 1, 0, rb.lead);
         return;
     }
     int lane = threadIdx.x & 63;
@@ -2744,7 +2768,7 @@ int run_fwd(mpm_ctx* c, int f, hipStream_t st, bool recompute = false) {
     { PhaseTimer t(c, MPM_PH_P2G, st);
       hipLaunchKernelGGL(k_p2g_scatter, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm); }
     { PhaseTimer t(c, MPM_PH_GRID, st);
-      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(256), 0, st, c->gargs(f, bs, c->keep), c->tiles_pm, c->g_pm(f),
+      hipLaunchKernelGGL(k_grid_op, dim3(c->grid_ctas), dim3(128), 0, st, c->gargs(f, bs, c->keep), c->tiles_pm, c->g_pm(f),
                          c->g_v(f)); }
     { PhaseTimer t(c, MPM_PH_G2P, st);
       hipLaunchKernelGGL(k_g2p_tile, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, out, c->g_v(f),
@@ -2781,7 +2805,7 @@ int launch_fallbacks(mpm_ctx* c, int f, hipStream_t s) {
       hipLaunchKernelGGL(k_p2g_scatter_rebuild, dim3(64), dim3(WG), 0, s, ta, (const float*)c->frame_ptr(f), (const float*)c->aff_ptr(f),
                          c->tiles_pm, 1); }
     { PhaseTimer t(c, MPM_PH_GRID_KEEP, s);
-      hipLaunchKernelGGL(k_grid_op_rebuild, dim3(64), dim3(256), 0, s, ga, c->tiles_pm, c->g_pm(f), c->g_v(f), 1); }
+      hipLaunchKernelGGL(k_grid_op_rebuild, dim3(256), dim3(128), 0, s, ga, c->tiles_pm, c->g_pm(f), c->g_v(f), 1); }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -2821,7 +2845,7 @@ int run_bwd(mpm_ctx* c, int f, hipStream_t st) {
         { PhaseTimer t(c, MPM_PH_P2G_GRID, st);
           hipLaunchKernelGGL(k_p2g_scatter_rebuild, dim3(c->tile_ctas), dim3(WG), 0, st, ta, in, (const float*)aff, c->tiles_pm, 0); }
         { PhaseTimer t(c, MPM_PH_GRID_KEEP, st);
-          hipLaunchKernelGGL(k_grid_op_rebuild, dim3(c->grid_ctas), dim3(256), 0, st, ga, c->tiles_pm, c->g_pm(f), c->g_v(f), 0); }
+          hipLaunchKernelGGL(k_grid_op_rebuild, dim3(c->grid_ctas), dim3(128), 0, st, ga, c->tiles_pm, c->g_pm(f), c->g_v(f), 0); }
     } else if (c->keep_cap < c->n_blocks && c->fb_done != f) {
         if (launch_fallbacks(c, f, st)) return -1;
     }
