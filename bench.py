@@ -17,8 +17,8 @@ resident in HBM before timing.  Reported next to it:
 - the gather scene's device memory, store-all against checkpoints every K
   in {10, 50, 100} frames, and the large cutting scene's fwd+adjoint rate;
 - live HIP-event kernel timings with the HBM roofline of the dominant kernel;
-- the fp32 CPU oracle timed on a bounded sample on all host cores the box
-  grants this job and on one core.
+- the fp32 CPU oracle timed on a bounded sample on all host cores the
+  machine grants this job and on one core.
 """
 import argparse
 import json
