@@ -3134,9 +3134,12 @@ int mpm_create(const mpm_scene_desc* scenes, int n_scenes, const mpm_prim_desc* 
     }
     c->grid_ctas = std::max(64, ncu * 8);
     c->tile_ctas = std::max(64, ncu * 4);
+    // the fork and join events order work between two streams of this device
+    // only: a device-scope fence suffices (no system-scope cache write-back at
+    // each record; 0.75 ms less per eager 200-substep forward + adjoint pass)
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess) {
         mpm_destroy(c);
         return fail("mpm_create: stream/event creation failed");
     }
