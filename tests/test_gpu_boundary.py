@@ -43,8 +43,15 @@ def eight_sphere_cfg():
 def rollout_matches_oracle(cfg, n_particles, n_tools, seed):
     """A short GradModel rollout of the task (2 env steps, density-grid L1
     loss) against the fp32 / fp64 oracle: final positions, loss and action
-    gradients; every tool's actions carry gradient."""
+    gradients; every tool's actions carry gradient.  Positions and gradients
+    are held to the fp32 oracle (the reference's ti.f32 arithmetic), with the
+    fp64 distances reported: over 38 substeps of eight spheres pressing the
+    dough, fp32 rounding alone moves particles by up to 1.2e-4 dx from the
+    fp64 trajectory (the fp32 oracle's own distance), past the 1e-4 dx bound.
+    The dough clouds come from numpy's global generator, seeded here so that
+    each run samples the same particles."""
     from diffskill_amd.engine import GradModel, TaichiEnv
+    np.random.seed(seed)
     env = TaichiEnv(cfg)
     env.initialize()
     sim = env.simulator
@@ -84,9 +91,12 @@ def rollout_matches_oracle(cfg, n_particles, n_tools, seed):
         res[bits] = (frames[T], float(np.abs(m - target).sum()), np.asarray(ga, np.float64))
     dx = 1.0 / ng
     rel = lambda u, v: float(np.abs(u - v).max() / np.abs(v).max())  # noqa: E731
-    print("x err %.3g dx; loss %.7g vs %.7g; grad rel vs fp32 oracle %.3g, vs fp64 %.3g"
-          % (np.abs(xT - res[64][0][:, 0:3]).max() / dx, float(loss), res[64][1], rel(g, res[32][2]), rel(g, res[64][2])))
-    assert np.abs(xT - res[64][0][:, 0:3]).max() < 1e-4 * dx
+    xerr = lambda u, v: float(np.abs(u[:, 0:3] - v[:, 0:3]).max() / dx)  # noqa: E731
+    print("x err vs fp32 oracle %.3g dx, vs fp64 %.3g dx (fp32 oracle vs fp64 %.3g dx); loss %.7g vs %.7g; "
+          "grad rel vs fp32 oracle %.3g, vs fp64 %.3g"
+          % (xerr(xT, res[32][0]), xerr(xT, res[64][0]), xerr(res[32][0], res[64][0]), float(loss), res[64][1],
+             rel(g, res[32][2]), rel(g, res[64][2])))
+    assert xerr(xT, res[32][0]) < 1e-4
     assert abs(float(loss) - res[64][1]) < 1e-4 * res[64][1]
     assert rel(g, res[32][2]) < 1e-3
     # every sphere moved the dough: each tool's actions carry gradient
